@@ -1,0 +1,569 @@
+// hl_api.hip -- the C ABI of libhelio_hip.so: context, scratch memory,
+// argument validation and the entry points declared in helio_hip/helio_hip.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "hl_internal.hpp"
+
+using namespace hl;
+
+// ------------------------------------------------------------------ context
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct hl_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf hot, cull, col_list, col_count, raw, words, accum;
+    // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
+};
+
+struct hl_event {
+    hipEvent_t ev;
+};
+
+namespace {
+
+hl_status fail(hl_context* c, hl_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+hl_status fail(hl_context* c, hl_status s, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return s;
+}
+
+hl_status hip_fail(hl_context* c, hipError_t e, const char* what) {
+    return fail(c, HL_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HL_HIP(c, expr)                                       \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return hip_fail((c), _e, #expr); \
+    } while (0)
+
+hl_status ensure(hl_context* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return HL_OK;
+    HL_HIP(c, hipSetDevice(c->device));
+    if (b.p) {
+        HL_HIP(c, hipStreamSynchronize(c->stream));
+        HL_HIP(c, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, HL_ENOMEM, "scratch allocation of %zu bytes failed: %s", want,
+                    hipGetErrorString(e));
+    }
+    b.bytes = want;
+    return HL_OK;
+}
+
+uint32_t texel_bytes(uint32_t fmt) {
+    switch (fmt) {
+        case HL_FMT_RGBA8_UNORM: return 4;
+        case HL_FMT_RGBA16_FLOAT: return 8;
+        case HL_FMT_RGBA32_FLOAT: return 16;
+        case HL_FMT_R32_FLOAT: return 4;
+        case HL_FMT_RG16_FLOAT: return 4;
+        default: return 0;
+    }
+}
+
+hl_status check_tex(hl_context* c, const hl_texture& t, uint32_t w, const char* name) {
+    const uint32_t tb = texel_bytes(t.format);
+    if (!t.data_d) return fail(c, HL_EINVAL, "gbuffer.%s: null data pointer", name);
+    if (tb == 0) return fail(c, HL_EINVAL, "gbuffer.%s: unknown format %u", name, t.format);
+    if ((uint64_t)t.pitch_bytes < (uint64_t)w * tb || t.pitch_bytes % tb != 0)
+        return fail(c, HL_EINVAL, "gbuffer.%s: pitch %u too small or not a multiple of %u", name,
+                    t.pitch_bytes, tb);
+    if (((uintptr_t)t.data_d) % tb != 0)
+        return fail(c, HL_EINVAL, "gbuffer.%s: data not aligned to %u bytes", name, tb);
+    return HL_OK;
+}
+
+hl_status check_gbuffer(hl_context* c, const hl_gbuffer* g, int* set) {
+    if (!g) return fail(c, HL_EINVAL, "gbuffer is null");
+    if (g->width == 0 || g->height == 0) return fail(c, HL_EINVAL, "gbuffer has zero size");
+    if (g->width > 32768 || g->height > 32768)
+        return fail(c, HL_EINVAL, "gbuffer %ux%u exceeds 32768", g->width, g->height);
+    hl_status s;
+    if ((s = check_tex(c, g->albedo, g->width, "albedo")) != HL_OK) return s;
+    if ((s = check_tex(c, g->normal, g->width, "normal")) != HL_OK) return s;
+    if ((s = check_tex(c, g->orm, g->width, "orm")) != HL_OK) return s;
+    if ((s = check_tex(c, g->emissive, g->width, "emissive")) != HL_OK) return s;
+    if ((s = check_tex(c, g->depth, g->width, "depth")) != HL_OK) return s;
+    if (g->depth.format != HL_FMT_R32_FLOAT)
+        return fail(c, HL_EINVAL, "gbuffer.depth must be R32_FLOAT (Depth32Float)");
+    if (g->lightmap_uv.data_d)
+        return fail(c, HL_EUNSUPPORTED,
+                    "gbuffer.lightmap_uv must be NULL: baked lightmaps are outside the hot path");
+    const uint32_t fa = g->albedo.format, fo = g->orm.format, fn = g->normal.format,
+                   fe = g->emissive.format;
+    if (fa != fo || fn != fe)
+        return fail(c, HL_EUNSUPPORTED, "albedo/orm and normal/emissive must share formats");
+    if (fa == HL_FMT_RGBA8_UNORM && fn == HL_FMT_RGBA16_FLOAT) *set = kSetRef;
+    else if (fa == HL_FMT_RGBA16_FLOAT && fn == HL_FMT_RGBA16_FLOAT) *set = kSet16;
+    else if (fa == HL_FMT_RGBA32_FLOAT && fn == HL_FMT_RGBA32_FLOAT) *set = kSet32;
+    else return fail(c, HL_EUNSUPPORTED, "unsupported G-buffer format set (%u,%u)", fa, fn);
+    return HL_OK;
+}
+
+bool debug_supported(uint32_t m) {
+    return m == 0 || m == 1 || m == 2 || m == 4 || m == 5 || m == 20 || m == 21;
+}
+
+hl_status check_cluster_params(hl_context* c, const hl_cluster_params* p) {
+    if (!p) return fail(c, HL_EINVAL, "cluster params are null");
+    if (p->grid_x == 0 || p->grid_y == 0 || p->grid_z == 0 || p->grid_z > HL_MAX_SLICES)
+        return fail(c, HL_EINVAL, "cluster grid %ux%ux%u invalid (1..%d slices)", p->grid_x,
+                    p->grid_y, p->grid_z, HL_MAX_SLICES);
+    if (p->cell_w == 0 || p->cell_h == 0 || p->width == 0 || p->height == 0)
+        return fail(c, HL_EINVAL, "cluster cell or screen size is zero");
+    if ((uint64_t)p->grid_x * p->cell_w < p->width || (uint64_t)p->grid_y * p->cell_h < p->height)
+        return fail(c, HL_EINVAL, "cluster grid does not cover the screen");
+    if ((uint64_t)p->grid_x * p->grid_y * p->grid_z > (1ull << 26))
+        return fail(c, HL_EINVAL, "cluster grid too large");
+    return HL_OK;
+}
+
+ClusterArgs make_cluster_args(const hl_cluster_params* p, uint32_t num_lights) {
+    ClusterArgs a;
+    memset(&a, 0, sizeof a);
+    a.grid_x = p->grid_x;
+    a.grid_y = p->grid_y;
+    a.grid_z = p->grid_z;
+    a.cell_w = p->cell_w;
+    a.cell_h = p->cell_h;
+    a.width = p->width;
+    a.height = p->height;
+    a.cap = p->cap;
+    a.num_lights = num_lights;
+    a.r_pad_column = p->r_pad_column;
+    memcpy(a.slice_near, p->slice_near, sizeof a.slice_near);
+    memcpy(a.slice_far, p->slice_far, sizeof a.slice_far);
+    memcpy(a.slice_r_pad, p->slice_r_pad, sizeof a.slice_r_pad);
+    return a;
+}
+
+uint32_t* ctx_words(hl_context* c) { return (uint32_t*)c->words.p; }
+
+// Shared set-up of the shading entry points.
+hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
+                        const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights,
+                        const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0) {
+    if (!c) return HL_EINVAL;
+    if (!camera_d) return fail(c, HL_EINVAL, "camera is null");
+    if (!g) return fail(c, HL_EINVAL, "globals are null");
+    if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
+    hl_status s = check_gbuffer(c, gb, set);
+    if (s != HL_OK) return s;
+    if (!debug_supported(g->debug_mode))
+        return fail(c, HL_EUNSUPPORTED, "debug_mode %u needs shadow data outside the hot path",
+                    g->debug_mode);
+    if (need_debug0 && g->debug_mode != 0)
+        return fail(c, HL_EUNSUPPORTED, "the adjoint is defined for debug_mode 0 only");
+    if (!grid) return fail(c, HL_EINVAL, "light grid is null");
+    memset(a, 0, sizeof *a);
+    const uint32_t W = gb->width, H = gb->height;
+    if (grid->mode == HL_GRID_TILES) {
+        const uint32_t tx = (W + 15) / 16, ty = (H + 15) / 16;
+        if (!grid->lists_d || !grid->counts_d) return fail(c, HL_EINVAL, "tile lists/counts are null");
+        if (grid->num_tiles_x < tx || grid->num_tiles_y < ty)
+            return fail(c, HL_EINVAL, "tile grid %ux%u does not cover %ux%u px", grid->num_tiles_x,
+                        grid->num_tiles_y, W, H);
+        if (g->num_tiles_x != grid->num_tiles_x)
+            return fail(c, HL_EINVAL, "globals.num_tiles_x (%u) != grid.num_tiles_x (%u)",
+                        g->num_tiles_x, grid->num_tiles_x);
+        a->grid_mode = HL_GRID_TILES;
+        a->tile_stride_x = g->num_tiles_x;
+        uint64_t cap = (uint64_t)grid->num_tiles_x * grid->num_tiles_y * HL_MAX_LIGHTS_PER_TILE;
+        if (grid->lists_capacity && grid->lists_capacity < cap) cap = grid->lists_capacity;
+        a->lists_capacity = cap > 0xffffffffull ? 0xffffffffu : (uint32_t)cap;
+        a->grid_z = 1;
+    } else if (grid->mode == HL_GRID_CLUSTERS) {
+        const hl_cluster_params* p = grid->clusters;
+        if ((s = check_cluster_params(c, p)) != HL_OK) return s;
+        if (p->width != W || p->height != H)
+            return fail(c, HL_EINVAL, "cluster params are for %ux%u, G-buffer is %ux%u", p->width,
+                        p->height, W, H);
+        if (!grid->lists_d || !grid->counts_d || !grid->offsets_d)
+            return fail(c, HL_EINVAL, "cluster offsets/counts/pool are null");
+        a->grid_mode = HL_GRID_CLUSTERS;
+        a->grid_x = p->grid_x;
+        a->grid_y = p->grid_y;
+        a->grid_z = p->grid_z;
+        a->cell_w = p->cell_w;
+        a->cell_h = p->cell_h;
+        a->lists_capacity = grid->lists_capacity;
+        memcpy(a->depth_split, p->depth_split, sizeof a->depth_split);
+        const double ln = log2((double)p->near_plane), lf = log2((double)p->far_plane);
+        const double scale = (double)p->grid_z / (lf - ln);
+        a->log_scale = (float)scale;
+        a->log_bias = (float)(-ln * scale);
+    } else {
+        return fail(c, HL_EINVAL, "unknown grid mode %u", grid->mode);
+    }
+    a->lists = grid->lists_d;
+    a->counts = grid->counts_d;
+    a->offsets = grid->offsets_d;
+    a->cam = camera_d;
+    a->width = W;
+    a->height = H;
+    a->albedo = {(const uint8_t*)gb->albedo.data_d, gb->albedo.pitch_bytes};
+    a->normal = {(const uint8_t*)gb->normal.data_d, gb->normal.pitch_bytes};
+    a->orm = {(const uint8_t*)gb->orm.data_d, gb->orm.pitch_bytes};
+    a->emissive = {(const uint8_t*)gb->emissive.data_d, gb->emissive.pitch_bytes};
+    a->depth = {(const uint8_t*)gb->depth.data_d, gb->depth.pitch_bytes};
+    a->num_lights = num_lights;
+    for (int k = 0; k < 3; ++k) {
+#pragma clang fp contract(off)
+        a->sky[k] = g->ambient_color[k] * g->ambient_intensity;
+    }
+    a->debug_mode = g->debug_mode;
+    a->block_tiles_x = (W + 31) / 32;
+    a->num_blocks = a->block_tiles_x * ((H + 7) / 8);
+    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(num_lights ? num_lights : 1))) != HL_OK) return s;
+    if ((s = ensure(c, c->words, 64)) != HL_OK) return s;
+    a->lights = (const HotLight*)c->hot.p;
+    HL_HIP(c, hipSetDevice(c->device));
+    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, num_lights, camera_d, (HotLight*)c->hot.p,
+                          nullptr, ctx_words(c) + 1));
+    return HL_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ exports
+extern "C" {
+
+HL_API const char* hl_version(void) { return "helio_hip 0.1.0 (gfx950)"; }
+
+HL_API const char* hl_status_string(hl_status s) {
+    switch (s) {
+        case HL_OK: return "ok";
+        case HL_EINVAL: return "invalid argument";
+        case HL_EHIP: return "HIP runtime error";
+        case HL_ECAPACITY: return "capacity exceeded";
+        case HL_EUNSUPPORTED: return "unsupported input";
+        case HL_ENOMEM: return "out of device memory";
+        default: return "unknown status";
+    }
+}
+
+HL_API hl_status hl_context_create(int device, hl_context** out) {
+    if (!out) return HL_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return HL_EHIP;
+    if (device < 0 || device >= n) return HL_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return HL_EHIP;
+    hl_context* c = new hl_context();
+    c->device = device;
+    if (ensure(c, c->words, 64) != HL_OK) {
+        delete c;
+        return HL_ENOMEM;
+    }
+    if (hipMemset(c->words.p, 0, c->words.bytes) != hipSuccess) {
+        (void)hipFree(c->words.p);
+        delete c;
+        return HL_EHIP;
+    }
+    *out = c;
+    return HL_OK;
+}
+
+HL_API hl_status hl_context_destroy(hl_context* c) {
+    if (!c) return HL_EINVAL;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&c->hot, &c->cull, &c->col_list, &c->col_count, &c->raw, &c->words, &c->accum})
+        if (b->p) (void)hipFree(b->p);
+    delete c;
+    return HL_OK;
+}
+
+HL_API hl_status hl_set_stream(hl_context* c, void* stream) {
+    if (!c) return HL_EINVAL;
+    c->stream = (hipStream_t)stream;
+    return HL_OK;
+}
+
+HL_API void* hl_get_stream(const hl_context* c) { return c ? (void*)c->stream : nullptr; }
+
+HL_API hl_status hl_synchronize(hl_context* c) {
+    if (!c) return HL_EINVAL;
+    HL_HIP(c, hipStreamSynchronize(c->stream));
+    return HL_OK;
+}
+
+HL_API const char* hl_last_error(const hl_context* c) {
+    return c ? c->err.c_str() : "null context";
+}
+
+HL_API hl_status hl_alloc(hl_context* c, size_t bytes, void** out) {
+    if (!c || !out) return HL_EINVAL;
+    HL_HIP(c, hipSetDevice(c->device));
+    hipError_t e = hipMalloc(out, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(c, HL_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return HL_OK;
+}
+
+HL_API hl_status hl_free(hl_context* c, void* p) {
+    if (!c) return HL_EINVAL;
+    if (!p) return HL_OK;
+    HL_HIP(c, hipSetDevice(c->device));
+    HL_HIP(c, hipStreamSynchronize(c->stream));
+    HL_HIP(c, hipFree(p));
+    return HL_OK;
+}
+
+HL_API hl_status hl_memcpy_htod(hl_context* c, void* dst, const void* src, size_t n) {
+    if (!c || (n && (!dst || !src))) return HL_EINVAL;
+    HL_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    HL_HIP(c, hipStreamSynchronize(c->stream));
+    return HL_OK;
+}
+
+HL_API hl_status hl_memcpy_dtoh(hl_context* c, void* dst, const void* src, size_t n) {
+    if (!c || (n && (!dst || !src))) return HL_EINVAL;
+    HL_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->stream));
+    HL_HIP(c, hipStreamSynchronize(c->stream));
+    return HL_OK;
+}
+
+HL_API hl_status hl_memcpy_dtod(hl_context* c, void* dst, const void* src, size_t n) {
+    if (!c || (n && (!dst || !src))) return HL_EINVAL;
+    HL_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, c->stream));
+    return HL_OK;
+}
+
+HL_API hl_status hl_memset_d32(hl_context* c, void* dst, uint32_t v, size_t count) {
+    if (!c || (count && !dst)) return HL_EINVAL;
+    HL_HIP(c, launch_fill_u32(c->stream, (uint32_t*)dst, v, count));
+    return HL_OK;
+}
+
+HL_API hl_status hl_event_create(hl_context* c, hl_event** out) {
+    if (!c || !out) return HL_EINVAL;
+    hl_event* e = new hl_event();
+    hipError_t r = hipEventCreate(&e->ev);
+    if (r != hipSuccess) {
+        delete e;
+        return hip_fail(c, r, "hipEventCreate");
+    }
+    *out = e;
+    return HL_OK;
+}
+
+HL_API hl_status hl_event_destroy(hl_event* e) {
+    if (!e) return HL_EINVAL;
+    (void)hipEventDestroy(e->ev);
+    delete e;
+    return HL_OK;
+}
+
+HL_API hl_status hl_event_record(hl_context* c, hl_event* e) {
+    if (!c || !e) return HL_EINVAL;
+    HL_HIP(c, hipEventRecord(e->ev, c->stream));
+    return HL_OK;
+}
+
+HL_API hl_status hl_event_elapsed_ms(hl_event* a, hl_event* b, float* ms) {
+    if (!a || !b || !ms) return HL_EINVAL;
+    if (hipEventSynchronize(b->ev) != hipSuccess) return HL_EHIP;
+    if (hipEventElapsedTime(ms, a->ev, b->ev) != hipSuccess) return HL_EHIP;
+    return HL_OK;
+}
+
+HL_API hl_status hl_cluster_params_init(const GpuCameraUniforms* cam, uint32_t width,
+                                        uint32_t height, uint32_t gx, uint32_t gy, uint32_t gz,
+                                        uint32_t cap, hl_cluster_params* out) {
+    if (!cam || !out) return HL_EINVAL;
+    if (width == 0 || height == 0 || gx == 0 || gy == 0 || gz == 0 || gz > HL_MAX_SLICES)
+        return HL_EINVAL;
+    // standard-Z perspective: proj[2][3] = -1, proj[3][3] = 0, no off-centre terms
+    if (cam->proj[11] != -1.0f || cam->proj[15] != 0.0f || cam->proj[8] != 0.0f ||
+        cam->proj[9] != 0.0f)
+        return HL_EUNSUPPORTED;
+    const double p22 = cam->proj[10], p32 = cam->proj[14];
+    const double near_ = cam->position_near[3], far_ = cam->forward_far[3];
+    if (!(near_ > 0.0) || !(far_ > near_)) return HL_EINVAL;
+    memset(out, 0, sizeof *out);
+    out->grid_x = gx;
+    out->grid_y = gy;
+    out->grid_z = gz;
+    out->cell_w = (width + gx - 1) / gx;
+    out->cell_h = (height + gy - 1) / gy;
+    out->width = width;
+    out->height = height;
+    out->cap = cap;
+    out->near_plane = (float)near_;
+    out->far_plane = (float)far_;
+    // NDC depth of view distance z: (p22 * (-z) + p32) / z; view distance of depth d: p32 / (d + p22)
+    for (uint32_t k = 0; k <= gz; ++k) {
+        const double z = near_ * pow(far_ / near_, (double)k / gz);
+        double d = (p22 * (-z) + p32) / z;
+        if (k == 0) d = 0.0;
+        if (k == gz) d = 1.0;
+        out->depth_split[k] = (float)d;
+    }
+    // View-distance range of each slice, padded for the FP32 reconstruction
+    // error of a world position from its depth: w = m23 * depth + m33 with
+    // |m33| = 1/near cancels to 1/distance, so w carries an absolute error of
+    // ~1.2e-7/near, a relative error of ~1.2e-7 * distance / near (x5 margin
+    // below).  The radius padding bounds the same error as a world-space
+    // distance (relative error times the point's distance from the origin).
+    const double cam_dist = sqrt((double)cam->position_near[0] * cam->position_near[0] +
+                                 (double)cam->position_near[1] * cam->position_near[1] +
+                                 (double)cam->position_near[2] * cam->position_near[2]);
+    double maxpad = 0.0;
+    for (uint32_t k = 0; k < gz; ++k) {
+        const double d0 = out->depth_split[k], d1 = out->depth_split[k + 1];
+        double z0 = k == 0 ? near_ : p32 / (d0 + p22);
+        double z1 = k + 1 == gz ? far_ : p32 / (d1 + p22);
+        const double rel = 2e-3 + 6e-7 * z1 / near_;
+        z0 *= (1.0 - rel);
+        z1 *= (1.0 + rel);
+        out->slice_near[k] = (float)z0;
+        out->slice_far[k] = (float)z1;
+        const double pad = rel * (z1 + cam_dist) + 1e-3;
+        out->slice_r_pad[k] = (float)pad;
+        if (pad > maxpad) maxpad = pad;
+    }
+    out->r_pad_column = (float)maxpad;
+    return HL_OK;
+}
+
+HL_API hl_status hl_light_cull_tiles(hl_context* c, const GpuCameraUniforms* camera_d,
+                                     const LightCullParams* params, const GpuLight* lights_d,
+                                     uint32_t* lists_d, uint32_t* counts_d) {
+    if (!c) return HL_EINVAL;
+    if (!params || !lists_d || !counts_d) return fail(c, HL_EINVAL, "null params or outputs");
+    const uint32_t tiles = params->num_tiles_x * params->num_tiles_y;
+    HL_HIP(c, hipSetDevice(c->device));
+    if (params->num_lights == 0) {  // LightCullPass::execute, lib.rs:285-292
+        HL_HIP(c, launch_fill_u32(c->stream, lists_d, 0, (size_t)tiles * HL_MAX_LIGHTS_PER_TILE));
+        HL_HIP(c, launch_fill_u32(c->stream, counts_d, 0, tiles));
+        return HL_OK;
+    }
+    if (!camera_d || !lights_d) return fail(c, HL_EINVAL, "null camera or lights");
+    if (params->screen_width == 0 || params->screen_height == 0)
+        return fail(c, HL_EINVAL, "zero screen size");
+    hl_status s = ensure(c, c->cull, sizeof(CullLight) * (size_t)params->num_lights);
+    if (s != HL_OK) return s;
+    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)params->num_lights)) != HL_OK) return s;
+    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, params->num_lights, camera_d,
+                          (HotLight*)c->hot.p, (CullLight*)c->cull.p, ctx_words(c) + 1));
+    HL_HIP(c, launch_cull_tiles(c->stream, camera_d, *params, (const CullLight*)c->cull.p, lists_d,
+                                counts_d));
+    return HL_OK;
+}
+
+HL_API hl_status hl_light_cull_clusters(hl_context* c, const GpuCameraUniforms* camera_d,
+                                        const hl_cluster_params* p, const GpuLight* lights_d,
+                                        uint32_t num_lights, uint32_t* offsets_d,
+                                        uint32_t* counts_d, uint32_t* pool_d, uint32_t pool_cap,
+                                        uint32_t* overflow_d) {
+    if (!c) return HL_EINVAL;
+    hl_status s = check_cluster_params(c, p);
+    if (s != HL_OK) return s;
+    if (!camera_d) return fail(c, HL_EINVAL, "camera is null");
+    if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
+    if (!offsets_d || !counts_d || (!pool_d && pool_cap)) return fail(c, HL_EINVAL, "null outputs");
+    const uint32_t ncol = p->grid_x * p->grid_y;
+    const size_t ncell = (size_t)ncol * p->grid_z;
+    const uint32_t n = num_lights;
+    if ((s = ensure(c, c->cull, sizeof(CullLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
+    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
+    if ((s = ensure(c, c->col_list, sizeof(uint32_t) * (size_t)ncol * (n ? n : 1))) != HL_OK) return s;
+    if ((s = ensure(c, c->col_count, sizeof(uint32_t) * ncol)) != HL_OK) return s;
+    if ((s = ensure(c, c->raw, sizeof(uint32_t) * ncell)) != HL_OK) return s;
+    HL_HIP(c, hipSetDevice(c->device));
+    uint32_t* flags = overflow_d ? overflow_d : ctx_words(c);
+    HL_HIP(c, launch_fill_u32(c->stream, flags, 0, 1));
+    if (overflow_d) HL_HIP(c, launch_fill_u32(c->stream, ctx_words(c), 0, 1));
+    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, n, camera_d, (HotLight*)c->hot.p,
+                          (CullLight*)c->cull.p, ctx_words(c) + 1));
+    ClusterArgs a = make_cluster_args(p, n);
+    HL_HIP(c, launch_cluster_cull(c->stream, a, camera_d, (const CullLight*)c->cull.p,
+                                  (uint32_t*)c->col_list.p, (uint32_t*)c->col_count.p,
+                                  (uint32_t*)c->raw.p, offsets_d, counts_d, pool_d, pool_cap, flags));
+    if (overflow_d) HL_HIP(c, hipMemcpyAsync(ctx_words(c), overflow_d, 4, hipMemcpyDeviceToDevice, c->stream));
+    return HL_OK;
+}
+
+HL_API hl_status hl_query_overflow(hl_context* c, uint32_t* flags) {
+    if (!c || !flags) return HL_EINVAL;
+    HL_HIP(c, hipMemcpyAsync(flags, ctx_words(c), 4, hipMemcpyDeviceToHost, c->stream));
+    HL_HIP(c, hipStreamSynchronize(c->stream));
+    return HL_OK;
+}
+
+HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* camera_d,
+                                     const DeferredGlobals* g, const hl_gbuffer* gb,
+                                     const GpuLight* lights_d, uint32_t num_lights,
+                                     const hl_light_grid* grid, float* out_d,
+                                     uint32_t out_pitch_bytes, uint32_t flags) {
+    if (!c) return HL_EINVAL;
+    if (!out_d) return fail(c, HL_EINVAL, "output is null");
+    ShadeArgs a;
+    int set;
+    hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, num_lights, grid, &a, &set, false);
+    if (s != HL_OK) return s;
+    if (out_pitch_bytes < 16u * gb->width || out_pitch_bytes % 16 != 0 || ((uintptr_t)out_d) % 16)
+        return fail(c, HL_EINVAL, "output pitch %u invalid for width %u (RGBA32F, 16-B aligned)",
+                    out_pitch_bytes, gb->width);
+    a.out = out_d;
+    a.out_pitch = out_pitch_bytes;
+    a.flags = flags;
+    HL_HIP(c, launch_shade(c->stream, set, kModeForward, a));
+    return HL_OK;
+}
+
+HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUniforms* camera_d,
+                                            const DeferredGlobals* g, const hl_gbuffer* gb,
+                                            const GpuLight* lights_d, uint32_t num_lights,
+                                            const hl_light_grid* grid, uint64_t* pairs) {
+    if (!c) return HL_EINVAL;
+    if (!pairs) return fail(c, HL_EINVAL, "pairs is null");
+    DeferredGlobals g0;
+    if (!g) return fail(c, HL_EINVAL, "globals are null");
+    g0 = *g;
+    g0.debug_mode = 0;
+    ShadeArgs a;
+    int set;
+    hl_status s = prepare_shade(c, camera_d, &g0, gb, lights_d, num_lights, grid, &a, &set, false);
+    if (s != HL_OK) return s;
+    unsigned long long* ctr = (unsigned long long*)(ctx_words(c) + 2);
+    HL_HIP(c, launch_fill_u32(c->stream, (uint32_t*)ctr, 0, 2));
+    a.pair_counter = ctr;
+    HL_HIP(c, launch_shade(c->stream, set, kModeCount, a));
+    unsigned long long v = 0;
+    HL_HIP(c, hipMemcpyAsync(&v, ctr, 8, hipMemcpyDeviceToHost, c->stream));
+    HL_HIP(c, hipStreamSynchronize(c->stream));
+    *pairs = v;
+    return HL_OK;
+}
+
+}  // extern "C"

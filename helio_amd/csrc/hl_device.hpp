@@ -1,0 +1,315 @@
+// hl_device.hpp -- device-side building blocks shared by the gfx950 kernels.
+//
+// Two kinds of arithmetic live here:
+//  * "exact" functions (hl_x_*) compile with FP contraction off and use the
+//    correctly rounded IEEE division and square root (the HIP default), one
+//    binary32 operation per source operation, in the order light_cull.wgsl and
+//    deferred_lighting.wgsl are restated in.  Everything that decides list
+//    membership, the pixel-to-cell mapping, the reconstructed world position and
+//    the all(radiance < 0.002) cut-off goes through them, so those decisions
+//    are identical to an IEEE FP32 evaluation of the shaders.
+//  * "fast" functions (hl_f_*) use contraction and the hardware rcp/rsq/sqrt
+//    for the BRDF arithmetic, whose result only needs 1e-4 relative accuracy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/helio_hip/helio_hip.h"
+
+#define HL_WAVE 64
+
+namespace hl {
+
+// ------------------------------------------------------------------ records
+
+// Per-light record the shading kernels read (64 B, one scalar load).  Built
+// from GpuLight by the prep kernel each call:
+//   p0 = (position.xyz, range)          range = +inf for directional lights
+//   p1 = (color.rgb * intensity, cos inner)
+//   p2 = (direction.xyz, cos outer)     cos outer = -2 for non-spot lights
+//   p3 = (color.rgb, intensity)
+struct alignas(16) HotLight {
+    float4 p0, p1, p2, p3;
+};
+
+// Per-light record the culling kernels read (32 B):
+//   vpos = (view-space position, range)  (range = +inf: directional, always listed)
+//   cone = (view-space direction (not normalized), cos outer)  (cos outer = -2: no cone)
+struct alignas(16) CullLight {
+    float4 vpos, cone;
+};
+
+constexpr float kNoCone = -2.0f;
+
+// Texel formats the shading kernels are instantiated for.
+enum GbufSet : int { kSetRef = 0, kSet16 = 1, kSet32 = 2 };
+
+struct Tex {
+    const uint8_t* data;
+    uint32_t pitch;
+};
+
+// Everything a shading launch needs, passed by value as a kernel argument.
+struct ShadeArgs {
+    // camera (copied from device memory by the prep kernel into this form)
+    const GpuCameraUniforms* cam;
+    uint32_t width, height;
+    Tex albedo, normal, orm, emissive, depth;
+    const HotLight* lights;
+    uint32_t num_lights;
+    // grid
+    uint32_t grid_mode;     // HL_GRID_TILES / HL_GRID_CLUSTERS
+    uint32_t grid_x, grid_y, grid_z, cell_w, cell_h;
+    const uint32_t* lists;
+    const uint32_t* counts;
+    const uint32_t* offsets;
+    uint32_t lists_capacity;  // entries readable in lists
+    uint32_t tile_stride_x;   // tiles: DeferredGlobals.num_tiles_x
+    float log_scale, log_bias;// slice guess: floor(log2(z)*scale + bias), fixed up by depth_split
+    float depth_split[HL_MAX_SLICES + 1];  // hl_cluster_params.depth_split
+    // globals
+    float sky[3];
+    uint32_t debug_mode;
+    uint32_t flags;
+    // outputs
+    float* out;
+    uint32_t out_pitch;
+    unsigned long long* pair_counter;
+    // adjoint
+    const float* dout;       // dL/dout RGBA32F (or target for the fused L2 variant)
+    uint32_t dout_pitch;
+    float* d_pixel;          // 5 planes
+    double* d_light;         // 4 per light: dpos.xyz, dI (FP64 accumulation)
+    double* loss;            // fused L2 loss accumulator
+    uint32_t block_tiles_x;  // blocks per row of the launch grid
+    uint32_t num_blocks;
+};
+
+// ------------------------------------------------------------- exact math
+__device__ __forceinline__ float hl_x_dot3(float ax, float ay, float az, float bx, float by,
+                                           float bz) {
+#pragma clang fp contract(off)
+    float s = ax * bx;
+    s = s + ay * by;
+    s = s + az * bz;
+    return s;
+}
+
+// M (column-major) * (x, y, z, w), one component, columns accumulated in order.
+__device__ __forceinline__ float hl_x_mat_row(const float* m, int row, float x, float y, float z,
+                                              float w) {
+#pragma clang fp contract(off)
+    float s = m[0 * 4 + row] * x;
+    s = s + m[1 * 4 + row] * y;
+    s = s + m[2 * 4 + row] * z;
+    s = s + m[3 * 4 + row] * w;
+    return s;
+}
+
+__device__ __forceinline__ float3 hl_x_normalize(float x, float y, float z) {
+#pragma clang fp contract(off)
+    float l = sqrtf(hl_x_dot3(x, y, z, x, y, z));
+    return make_float3(x / l, y / l, z / l);
+}
+
+// light_cull.wgsl:85-100 make_plane_from_ndc_edge (d = 0 dropped; the caller adds 0.0f).
+__device__ __forceinline__ float3 hl_x_plane(float p00, float p11, float p0x, float p0y,
+                                             float p1x, float p1y) {
+#pragma clang fp contract(off)
+    float3 r0 = hl_x_normalize(p0x / p00, p0y / p11, -1.0f);
+    float3 r1 = hl_x_normalize(p1x / p00, p1y / p11, -1.0f);
+    float cx = r0.y * r1.z - r0.z * r1.y;
+    float cy = r0.z * r1.x - r0.x * r1.z;
+    float cz = r0.x * r1.y - r0.y * r1.x;
+    return hl_x_normalize(cx, cy, cz);
+}
+
+// light_cull.wgsl:105-117 sphere_inside_tile_frustum for one plane.
+__device__ __forceinline__ bool hl_x_plane_pass(float3 n, float4 c, float r) {
+#pragma clang fp contract(off)
+    float d = hl_x_dot3(n.x, n.y, n.z, c.x, c.y, c.z) + 0.0f;
+    return !(d < -r);
+}
+
+__device__ __forceinline__ bool hl_x_sphere_aabb(float4 c, float r, float3 lo, float3 hi) {
+#pragma clang fp contract(off)
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    if (c.x < lo.x) dx = lo.x - c.x; else if (c.x > hi.x) dx = c.x - hi.x;
+    if (c.y < lo.y) dy = lo.y - c.y; else if (c.y > hi.y) dy = c.y - hi.y;
+    if (c.z < lo.z) dz = lo.z - c.z; else if (c.z > hi.z) dz = c.z - hi.z;
+    float d2 = 0.0f;
+    d2 = d2 + dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    return d2 <= r * r;
+}
+
+// Cone against the AABB's bounding sphere (see the cluster membership rule).
+__device__ __forceinline__ bool hl_x_cone_aabb(float4 c, float4 cone, float pad, float3 lo,
+                                               float3 hi) {
+#pragma clang fp contract(off)
+    float alen = sqrtf(hl_x_dot3(cone.x, cone.y, cone.z, cone.x, cone.y, cone.z));
+    if (!(alen > 0.0f)) return true;
+    float cos_t = cone.w / alen;
+    if (!(cos_t > 0.0f)) return true;
+    if (cos_t > 1.0f) cos_t = 1.0f;
+    float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    float mx = (lo.x + hi.x) * 0.5f, my = (lo.y + hi.y) * 0.5f, mz = (lo.z + hi.z) * 0.5f;
+    float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
+    float R = sqrtf(hl_x_dot3(ex, ey, ez, ex, ey, ez)) * 0.5f + pad;
+    float vx = mx - c.x, vy = my - c.y, vz = mz - c.z;
+    float ax = cone.x / alen, ay = cone.y / alen, az = cone.z / alen;
+    float vlen2 = hl_x_dot3(vx, vy, vz, vx, vy, vz);
+    float v1 = hl_x_dot3(vx, vy, vz, ax, ay, az);
+    float perp = sqrtf(fmaxf(0.0f, vlen2 - v1 * v1));
+    float dcp = cos_t * perp - v1 * sin_t;
+    if (dcp > R) return false;
+    if (v1 < -R) return false;
+    return true;
+}
+
+// WGSL smoothstep, exact.
+__device__ __forceinline__ float hl_x_smoothstep(float e0, float e1, float x) {
+#pragma clang fp contract(off)
+    float t = fminf(fmaxf((x - e0) / (e1 - e0), 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// max channel of radiance exactly as pbr_direct_light (deferred_lighting.wgsl:724-775)
+// computes it for a point/spot light, and whether dist <= range.
+__device__ __forceinline__ float hl_x_radiance_max(const HotLight& L, float wx, float wy, float wz,
+                                                   bool* in_range) {
+#pragma clang fp contract(off)
+    float tx = L.p0.x - wx, ty = L.p0.y - wy, tz = L.p0.z - wz;
+    float dist = sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz));
+    *in_range = !(dist > L.p0.w);
+    float lx = tx / dist, ly = ty / dist, lz = tz / dist;
+    float atten = 1.0f / (dist * dist + 0.0001f);
+    float nd = dist / L.p0.w;
+    atten = atten * fmaxf(0.0f, 1.0f - nd * nd * nd * nd);
+    if (L.p2.w != kNoCone) {
+        float cos_a = hl_x_dot3(-lx, -ly, -lz, L.p2.x, L.p2.y, L.p2.z);
+        atten = atten * hl_x_smoothstep(L.p2.w, L.p1.w, cos_a);
+    }
+    float r = L.p1.x * atten, g = L.p1.y * atten, b = L.p1.z * atten;
+    return fmaxf(fmaxf(r, g), b);
+}
+
+// -------------------------------------------------------------- fast math
+__device__ __forceinline__ float hl_f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hl_f_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float hl_f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// ----------------------------------------------------------- texel decode
+__device__ __forceinline__ float hl_half(uint16_t h) {
+    _Float16 v;
+    __builtin_memcpy(&v, &h, 2);
+    return (float)v;
+}
+
+// v / 255 exactly: one multiply by the rounded reciprocal and one residual
+// correction (Markstein); equal to the IEEE quotient for every v in 0..255.
+__device__ __forceinline__ float hl_unorm8(uint32_t v) {
+    const float r = 1.0f / 255.0f;
+    float fv = (float)v;
+    float q = fv * r;
+    float e = __builtin_fmaf(-q, 255.0f, fv);
+    return __builtin_fmaf(e, r, q);
+}
+
+typedef unsigned int hl_u32x2 __attribute__((ext_vector_type(2)));
+typedef float hl_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void hl_store_nt(float4* p, float4 v) {
+    hl_f32x4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<hl_f32x4*>(p));
+}
+
+template <int FMT>
+__device__ __forceinline__ float4 hl_load_rgba(const Tex& t, uint32_t x, uint32_t y) {
+    const uint8_t* row = t.data + (size_t)y * t.pitch;
+    if constexpr (FMT == HL_FMT_RGBA8_UNORM) {
+        uint32_t v = __builtin_nontemporal_load((const uint32_t*)row + x);
+        return make_float4(hl_unorm8(v & 0xffu), hl_unorm8((v >> 8) & 0xffu),
+                           hl_unorm8((v >> 16) & 0xffu), hl_unorm8(v >> 24));
+    } else if constexpr (FMT == HL_FMT_RGBA16_FLOAT) {
+        hl_u32x2 v = __builtin_nontemporal_load((const hl_u32x2*)row + x);
+        return make_float4(hl_half((uint16_t)(v.x & 0xffffu)), hl_half((uint16_t)(v.x >> 16)),
+                           hl_half((uint16_t)(v.y & 0xffffu)), hl_half((uint16_t)(v.y >> 16)));
+    } else {
+        hl_f32x4 v = __builtin_nontemporal_load((const hl_f32x4*)row + x);
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+}
+
+__device__ __forceinline__ float hl_load_depth(const Tex& t, uint32_t x, uint32_t y) {
+    return __builtin_nontemporal_load((const float*)(t.data + (size_t)y * t.pitch) + x);
+}
+
+// -------------------------------------------------------------- wave ops
+__device__ __forceinline__ uint64_t hl_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t hl_lane() { return __lane_id(); }
+
+// number of set bits of m below this lane
+__device__ __forceinline__ uint32_t hl_prefix(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T hl_readlane(T v, uint32_t lane) {
+    static_assert(sizeof(T) == 4, "32-bit only");
+    int iv;
+    __builtin_memcpy(&iv, &v, 4);
+    iv = __builtin_amdgcn_readlane(iv, (int)lane);
+    T r;
+    __builtin_memcpy(&r, &iv, 4);
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T hl_uniform(T v) {
+    static_assert(sizeof(T) == 4, "32-bit only");
+    int iv;
+    __builtin_memcpy(&iv, &v, 4);
+    iv = __builtin_amdgcn_readfirstlane(iv);
+    T r;
+    __builtin_memcpy(&r, &iv, 4);
+    return r;
+}
+
+__device__ __forceinline__ float hl_wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, HL_WAVE));
+    return v;
+}
+__device__ __forceinline__ float hl_wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, HL_WAVE));
+    return v;
+}
+__device__ __forceinline__ float hl_wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, HL_WAVE);
+    return v;
+}
+
+// Wave-local ordering point between LDS writes and reads of other lanes.
+__device__ __forceinline__ void hl_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// blockIdx -> work index so consecutive work lands on one XCD (blocks are
+// dispatched round-robin over the 8 XCDs of an MI355X).
+__device__ __forceinline__ uint32_t hl_xcd_remap(uint32_t b, uint32_t nb) {
+    const uint32_t nx = 8;
+    uint32_t q = nb / nx, r = nb % nx;
+    uint32_t xcd = b % nx, k = b / nx;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+}  // namespace hl

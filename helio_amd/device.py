@@ -1,0 +1,275 @@
+"""Host-side handle on libhelio_hip.so: a context, device buffers and the
+pass-level calls, mirroring the reference passes.
+
+``LightCullPass`` and ``DeferredLightPass`` keep the names, the published
+frame slots and the prepare/execute split of
+crates/passes/3d/helio-pass-light-cull/src/lib.rs:41-363 and
+crates/passes/3d/helio-pass-deferred-light/src/lib.rs:54-1230; the work of
+``execute`` is one C-ABI call each.  Missing native code raises instead of
+falling back: there is no CPU path here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from .abi import HelioError
+
+_LIB: abi.Lib | None = None
+
+
+def lib() -> abi.Lib:
+    global _LIB
+    if _LIB is None:
+        _LIB = abi.Lib()
+    return _LIB
+
+
+class DeviceArray:
+    """A device allocation made through hl_alloc."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx.lib.call("hl_alloc", ctx.h, max(self.nbytes, 16), C.byref(p))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes, (a.nbytes, self.nbytes)
+        if a.nbytes:
+            self.ctx.lib.call("hl_memcpy_htod", self.ctx.h, self.ptr, a.ctypes.data, a.nbytes)
+        return self
+
+    def download(self, dtype, shape=None, nbytes=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes if nbytes is None else nbytes
+        out = np.empty(n // dt.itemsize, dtype=dt)
+        if out.nbytes:
+            self.ctx.lib.call("hl_memcpy_dtoh", self.ctx.h, out.ctypes.data, self.ptr, out.nbytes)
+        return out.reshape(shape) if shape is not None else out
+
+    def fill_u32(self, value: int, count: int | None = None) -> None:
+        n = self.nbytes // 4 if count is None else count
+        self.ctx.lib.call("hl_memset_d32", self.ctx.h, self.ptr, value & 0xFFFFFFFF, n)
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx.lib.call("hl_free", self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.ctx.lib.raw["hl_free"](self.ctx.h, self.ptr)
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = lib()
+        h = C.c_void_p()
+        self.lib.call("hl_context_create", device, C.byref(h))
+        self.h = h.value
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.call("hl_context_destroy", self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def alloc(self, nbytes: int) -> DeviceArray:
+        return DeviceArray(self, nbytes)
+
+    def upload(self, arr: np.ndarray) -> DeviceArray:
+        a = np.ascontiguousarray(arr)
+        return DeviceArray(self, a.nbytes).upload(a)
+
+    def synchronize(self) -> None:
+        self.lib.call("hl_synchronize", self.h)
+
+    def last_error(self) -> str:
+        return self.lib.raw["hl_last_error"](self.h).decode()
+
+    # events
+    def event(self):
+        e = C.c_void_p()
+        self.lib.call("hl_event_create", self.h, C.byref(e))
+        return e.value
+
+    def record(self, ev) -> None:
+        self.lib.call("hl_event_record", self.h, ev)
+
+    def elapsed_ms(self, a, b) -> float:
+        ms = C.c_float()
+        self.lib.call("hl_event_elapsed_ms", a, b, C.byref(ms))
+        return ms.value
+
+
+@dataclass
+class DeviceScene:
+    """A HostScene resident in HBM."""
+    host: object
+    camera: DeviceArray
+    lights: DeviceArray
+    planes: dict
+    gbuffer: abi.hl_gbuffer
+
+    @property
+    def num_lights(self) -> int:
+        return len(self.host.lights)
+
+
+def upload_scene(ctx: Context, scene) -> DeviceScene:
+    cam = ctx.upload(np.frombuffer(bytes(scene.camera), dtype=np.uint8))
+    lights = ctx.upload(scene.lights) if len(scene.lights) else ctx.alloc(128)
+    planes = {k: ctx.upload(getattr(scene, k)) for k in ("albedo", "normal", "orm", "emissive", "depth")}
+    g = abi.hl_gbuffer()
+    g.width, g.height = scene.width, scene.height
+    fa, ta, fn, tn = scene.formats
+    for name, fmt, tb in (("albedo", fa, ta), ("orm", fa, ta), ("normal", fn, tn), ("emissive", fn, tn)):
+        t = getattr(g, name)
+        t.data_d, t.pitch_bytes, t.format = planes[name].ptr, scene.width * tb, fmt
+    g.depth.data_d, g.depth.pitch_bytes, g.depth.format = planes["depth"].ptr, scene.width * 4, abi.HL_FMT_R32_FLOAT
+    return DeviceScene(scene, cam, lights, planes, g)
+
+
+class TileGrid:
+    """tile_light_lists / tile_light_counts (crates/libhelio/src/frame.rs:230-234)."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ntx = (width + 15) // 16
+        self.nty = (height + 15) // 16
+        self.lists = ctx.alloc(self.ntx * self.nty * abi.HL_MAX_LIGHTS_PER_TILE * 4)
+        self.counts = ctx.alloc(self.ntx * self.nty * 4)
+
+    def grid(self) -> abi.hl_light_grid:
+        g = abi.hl_light_grid()
+        g.mode, g.num_tiles_x, g.num_tiles_y = abi.HL_GRID_TILES, self.ntx, self.nty
+        g.lists_d, g.counts_d = self.lists.ptr, self.counts.ptr
+        g.lists_capacity = self.ntx * self.nty * abi.HL_MAX_LIGHTS_PER_TILE
+        return g
+
+
+class ClusterGrid:
+    """Compact clustered lists: offsets / counts / pool."""
+
+    def __init__(self, ctx: Context, params: abi.hl_cluster_params, pool_cap: int):
+        self.params = params
+        self.ncell = params.grid_x * params.grid_y * params.grid_z
+        self.offsets = ctx.alloc(self.ncell * 4)
+        self.counts = ctx.alloc(self.ncell * 4)
+        self.pool_cap = int(pool_cap)
+        self.pool = ctx.alloc(max(self.pool_cap, 1) * 4)
+        self.overflow = ctx.alloc(16)
+
+    def grid(self) -> abi.hl_light_grid:
+        g = abi.hl_light_grid()
+        g.mode = abi.HL_GRID_CLUSTERS
+        g.lists_d, g.counts_d, g.offsets_d = self.pool.ptr, self.counts.ptr, self.offsets.ptr
+        g.clusters = C.pointer(self.params)
+        g.lists_capacity = self.pool_cap
+        return g
+
+    def download(self):
+        off = self.offsets.download(np.uint32)[: self.ncell]
+        cnt = self.counts.download(np.uint32)[: self.ncell]
+        pool = self.pool.download(np.uint32)[: self.pool_cap]
+        return off, cnt, pool
+
+
+def light_cull_params(scene) -> abi.LightCullParams:
+    """LightCullPass::prepare (helio-pass-light-cull/src/lib.rs:264-282)."""
+    p = abi.LightCullParams()
+    p.num_tiles_x = (scene.width + 15) // 16
+    p.num_tiles_y = (scene.height + 15) // 16
+    p.num_lights = len(scene.lights)
+    p.screen_width, p.screen_height = scene.width, scene.height
+    return p
+
+
+class LightCullPass:
+    """Tile (reference) or cluster light assignment behind the RenderPass shape."""
+
+    def __init__(self, ctx: Context, width: int, height: int, clusters: abi.hl_cluster_params | None = None,
+                 pool_cap: int = 0):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self.clusters = clusters
+        self.tiles = TileGrid(ctx, width, height) if clusters is None else None
+        self.cgrid = ClusterGrid(ctx, clusters, pool_cap) if clusters is not None else None
+        self.params = None
+
+    def name(self) -> str:
+        return "LightCull"
+
+    def writes(self) -> tuple:
+        return ("tile_light_lists", "tile_light_counts")
+
+    def prepare(self, scene: DeviceScene) -> None:
+        self.params = light_cull_params(scene.host)
+
+    def execute(self, scene: DeviceScene) -> None:
+        L = self.ctx.lib
+        if self.tiles is not None:
+            L.call("hl_light_cull_tiles", self.ctx.h, scene.camera.ptr, C.byref(self.params),
+                   scene.lights.ptr, self.tiles.lists.ptr, self.tiles.counts.ptr)
+        else:
+            g = self.cgrid
+            L.call("hl_light_cull_clusters", self.ctx.h, scene.camera.ptr, C.byref(self.clusters),
+                   scene.lights.ptr, scene.num_lights, g.offsets.ptr, g.counts.ptr, g.pool.ptr,
+                   g.pool_cap, g.overflow.ptr)
+
+    def publish(self) -> abi.hl_light_grid:
+        return self.tiles.grid() if self.tiles is not None else self.cgrid.grid()
+
+    def overflow(self) -> int:
+        f = C.c_uint32()
+        self.ctx.lib.call("hl_query_overflow", self.ctx.h, C.byref(f))
+        return f.value
+
+
+class DeferredLightPass:
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self.out = ctx.alloc(width * height * 16)
+        self.debug_mode = 0
+        self.globals = None
+
+    def name(self) -> str:
+        return "DeferredLight"
+
+    def writes(self) -> tuple:
+        return ("pre_aa",)
+
+    def prepare(self, scene: DeviceScene) -> None:
+        g = abi.DeferredGlobals()
+        C.memmove(C.byref(g), C.byref(scene.host.globals), C.sizeof(g))
+        g.debug_mode = self.debug_mode
+        g.num_tiles_x = (self.width + 15) // 16
+        self.globals = g
+
+    def execute(self, scene: DeviceScene, grid: abi.hl_light_grid, flags: int = 0) -> None:
+        self.ctx.lib.call("hl_deferred_forward", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                          C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
+                          self.out.ptr, self.width * 16, flags)
+
+    def count_pairs(self, scene: DeviceScene, grid: abi.hl_light_grid) -> int:
+        n = C.c_uint64()
+        self.ctx.lib.call("hl_count_pixel_light_pairs", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                          C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid), C.byref(n))
+        return n.value
+
+    def output(self) -> np.ndarray:
+        return self.out.download(np.float32, (self.height, self.width, 4))

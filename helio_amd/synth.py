@@ -1,0 +1,287 @@
+"""Synthetic workloads of the deferred-lighting path (host side).
+
+Wraps libhelio_synth.so (helio_amd/csrc/synth.c) and names the five workloads
+the implementation is measured and checked on:
+
+  c1_720p_64    1280x720, reference G-buffer formats, 64 point lights,
+                tiles (16 px) and 16x9x24 log clusters
+  c2_1080p_1k   1920x1080, reference formats, 1024 lights (25 % spot),
+                16x9x24 clusters capped at 256
+  c3_4k_16k     3840x2160, fp16 G-buffer + fp32 depth, 16384 point lights,
+                32x18x48 clusters, unbounded
+  c4_512_adj    512x512, fp32 G-buffer, 256 point lights (adjoint workload)
+  c5_multiview  48 cube-probe faces + 2 XR eyes over an 8192-light pool
+
+Scene layout, seeds and camera poses are fixed here (the workload
+descriptions leave them open); every array is a pure function of the seed.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, replace
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+
+PKG = Path(__file__).resolve().parent
+SEED_BASE = 0x48454C494F000000
+
+
+class hl_synth_config(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("gbuf_format", C.c_uint32),
+        ("num_lights", C.c_uint32), ("spot_fraction", C.c_float),
+        ("spot_outer_min_deg", C.c_float), ("spot_outer_max_deg", C.c_float),
+        ("spot_inner_ratio", C.c_float), ("light_box_center", C.c_float * 3),
+        ("light_box_size", C.c_float * 3), ("range_min", C.c_float), ("range_max", C.c_float),
+        ("intensity_min", C.c_float), ("intensity_max", C.c_float), ("color_min", C.c_float),
+        ("color_max", C.c_float), ("eye", C.c_float * 3), ("target", C.c_float * 3),
+        ("fov_y_deg", C.c_float), ("near_plane", C.c_float), ("far_plane", C.c_float),
+        ("num_boxes", C.c_uint32), ("box_field_half", C.c_float), ("box_size_min", C.c_float),
+        ("box_size_max", C.c_float), ("box_height_min", C.c_float), ("box_height_max", C.c_float),
+        ("depth_noise", C.c_float), ("_pad", C.c_uint32), ("seed", C.c_uint64),
+    ]
+
+
+GBUF_REF, GBUF_F16, GBUF_F32 = 0, 1, 2
+_FMT = {  # gbuf_format -> (albedo/orm format, texel bytes, normal/emissive format, texel bytes)
+    GBUF_REF: (abi.HL_FMT_RGBA8_UNORM, 4, abi.HL_FMT_RGBA16_FLOAT, 8),
+    GBUF_F16: (abi.HL_FMT_RGBA16_FLOAT, 8, abi.HL_FMT_RGBA16_FLOAT, 8),
+    GBUF_F32: (abi.HL_FMT_RGBA32_FLOAT, 16, abi.HL_FMT_RGBA32_FLOAT, 16),
+}
+
+
+@dataclass(frozen=True)
+class SceneSpec:
+    name: str
+    width: int
+    height: int
+    gbuf_format: int
+    num_lights: int
+    light_box_center: tuple = (0.0, 5.0, 0.0)
+    light_box_size: tuple = (40.0, 10.0, 40.0)
+    range_min: float = 2.0
+    range_max: float = 10.0
+    intensity_min: float = 0.5
+    intensity_max: float = 5.0
+    color_min: float = 0.5
+    color_max: float = 1.0
+    spot_fraction: float = 0.0
+    spot_outer_min_deg: float = 10.0
+    spot_outer_max_deg: float = 45.0
+    spot_inner_ratio: float = 0.8
+    eye: tuple = (0.0, 8.0, 32.0)
+    target: tuple = (0.0, 0.0, 0.0)
+    fov_y_deg: float = 60.0
+    near: float = 0.1
+    far: float = 1000.0
+    num_boxes: int = 40
+    box_field_half: float = 20.0
+    box_size_min: float = 1.0
+    box_size_max: float = 4.0
+    box_height_min: float = 0.5
+    box_height_max: float = 6.0
+    depth_noise: float = 0.05
+    seed: int = SEED_BASE
+    # light grid of the workload
+    clusters: tuple = (16, 9, 24)
+    cluster_cap: int = 0
+    ambient_color: tuple = (0.5, 0.5, 0.6)   # DeferredLightPass::prepare fallback (lib.rs:842)
+    ambient_intensity: float = 1.0
+
+    def config(self) -> hl_synth_config:
+        c = hl_synth_config()
+        c.width, c.height, c.gbuf_format, c.num_lights = self.width, self.height, self.gbuf_format, self.num_lights
+        c.spot_fraction = self.spot_fraction
+        c.spot_outer_min_deg, c.spot_outer_max_deg = self.spot_outer_min_deg, self.spot_outer_max_deg
+        c.spot_inner_ratio = self.spot_inner_ratio
+        c.light_box_center[:] = self.light_box_center
+        c.light_box_size[:] = self.light_box_size
+        c.range_min, c.range_max = self.range_min, self.range_max
+        c.intensity_min, c.intensity_max = self.intensity_min, self.intensity_max
+        c.color_min, c.color_max = self.color_min, self.color_max
+        c.eye[:] = self.eye
+        c.target[:] = self.target
+        c.fov_y_deg, c.near_plane, c.far_plane = self.fov_y_deg, self.near, self.far
+        c.num_boxes, c.box_field_half = self.num_boxes, self.box_field_half
+        c.box_size_min, c.box_size_max = self.box_size_min, self.box_size_max
+        c.box_height_min, c.box_height_max = self.box_height_min, self.box_height_max
+        c.depth_noise = self.depth_noise
+        c.seed = self.seed
+        return c
+
+
+SPECS: dict[str, SceneSpec] = {
+    "c1_720p_64": SceneSpec("c1_720p_64", 1280, 720, GBUF_REF, 64, seed=SEED_BASE + 16 * 1),
+    "c2_1080p_1k": SceneSpec("c2_1080p_1k", 1920, 1080, GBUF_REF, 1024, spot_fraction=0.25,
+                             light_box_center=(0.0, 7.5, 0.0), light_box_size=(80.0, 15.0, 80.0),
+                             eye=(0.0, 10.0, 55.0), box_field_half=40.0, num_boxes=60,
+                             cluster_cap=256, seed=SEED_BASE + 16 * 2),
+    "c3_4k_16k": SceneSpec("c3_4k_16k", 3840, 2160, GBUF_F16, 16384,
+                           light_box_center=(0.0, 15.0, 0.0), light_box_size=(200.0, 30.0, 200.0),
+                           range_min=1.0, range_max=8.0, eye=(0.0, 12.0, 115.0),
+                           num_boxes=96, box_field_half=100.0, box_size_min=2.0, box_size_max=10.0,
+                           box_height_min=1.0, box_height_max=15.0, clusters=(32, 18, 48),
+                           seed=SEED_BASE + 16 * 3),
+    "c4_512_adj": SceneSpec("c4_512_adj", 512, 512, GBUF_F32, 256, seed=SEED_BASE + 16 * 4),
+}
+
+
+def spec(name: str, **over) -> SceneSpec:
+    return replace(SPECS[name], **over) if over else SPECS[name]
+
+
+_LIB = None
+
+
+def synth_lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        p = PKG / "libhelio_synth.so"
+        if not p.exists():
+            raise FileNotFoundError(f"{p} missing: run `python -m helio_amd.build`")
+        lib = C.CDLL(str(p))
+        lib.hl_synth_camera.argtypes = [C.POINTER(hl_synth_config), C.POINTER(abi.GpuCameraUniforms)]
+        lib.hl_synth_lights.argtypes = [C.POINTER(hl_synth_config), C.c_void_p]
+        lib.hl_synth_gbuffer.argtypes = [C.POINTER(hl_synth_config)] + [C.c_void_p] * 5
+        lib.hl_synth_subset.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+        lib.hl_synth_abi_size.restype = C.c_uint32
+        assert lib.hl_synth_abi_size() == C.sizeof(hl_synth_config)
+        _LIB = lib
+    return _LIB
+
+
+@dataclass
+class HostScene:
+    """Host-memory inputs of one view: camera, lights, globals and G-buffer planes."""
+    spec: SceneSpec
+    camera: abi.GpuCameraUniforms
+    lights: np.ndarray              # GPU_LIGHT_DTYPE
+    globals: abi.DeferredGlobals
+    albedo: np.ndarray
+    normal: np.ndarray
+    orm: np.ndarray
+    emissive: np.ndarray
+    depth: np.ndarray               # float32 [H, W]
+    formats: tuple = field(default=(0, 0, 0, 0))
+    texel: tuple = field(default=(0, 0))
+
+    @property
+    def width(self) -> int:
+        return self.spec.width
+
+    @property
+    def height(self) -> int:
+        return self.spec.height
+
+    def gbuffer_host(self) -> abi.hl_gbuffer:
+        """hl_gbuffer whose pointers are the host arrays (for the CPU oracle)."""
+        g = abi.hl_gbuffer()
+        g.width, g.height = self.width, self.height
+        fa, ta, fn, tn = self.formats
+        for name, arr, fmt, tb in (("albedo", self.albedo, fa, ta), ("orm", self.orm, fa, ta),
+                                   ("normal", self.normal, fn, tn), ("emissive", self.emissive, fn, tn)):
+            t = getattr(g, name)
+            t.data_d, t.pitch_bytes, t.format = arr.ctypes.data, self.width * tb, fmt
+        g.depth.data_d, g.depth.pitch_bytes, g.depth.format = self.depth.ctypes.data, self.width * 4, abi.HL_FMT_R32_FLOAT
+        return g
+
+
+def make_globals(s: SceneSpec) -> abi.DeferredGlobals:
+    g = abi.DeferredGlobals()
+    g.light_count = s.num_lights
+    g.ambient_intensity = s.ambient_intensity
+    g.ambient_color[:] = (*s.ambient_color, 1.0)
+    g.csm_splits[:] = (16.0, 80.0, 300.0, 1400.0)
+    g.num_tiles_x = (s.width + 15) // 16
+    g.enable_reflections = 1
+    return g
+
+
+def make_camera(s: SceneSpec) -> abi.GpuCameraUniforms:
+    cam = abi.GpuCameraUniforms()
+    cfg = s.config()
+    if synth_lib().hl_synth_camera(C.byref(cfg), C.byref(cam)) != 0:
+        raise ValueError("bad scene spec")
+    return cam
+
+
+def make_lights(s: SceneSpec) -> np.ndarray:
+    lights = np.zeros(s.num_lights, dtype=abi.GPU_LIGHT_DTYPE)
+    cfg = s.config()
+    if synth_lib().hl_synth_lights(C.byref(cfg), lights.ctypes.data if s.num_lights else None) != 0:
+        raise ValueError("bad light spec")
+    return lights
+
+
+def make_scene(s: SceneSpec | str, **over) -> HostScene:
+    if isinstance(s, str):
+        s = spec(s, **over)
+    elif over:
+        s = replace(s, **over)
+    fa, ta, fn, tn = _FMT[s.gbuf_format]
+    W, H = s.width, s.height
+    alb = np.zeros((H, W, ta), dtype=np.uint8)
+    orm = np.zeros((H, W, ta), dtype=np.uint8)
+    nrm = np.zeros((H, W, tn), dtype=np.uint8)
+    emi = np.zeros((H, W, tn), dtype=np.uint8)
+    dep = np.zeros((H, W), dtype=np.float32)
+    cfg = s.config()
+    r = synth_lib().hl_synth_gbuffer(C.byref(cfg), alb.ctypes.data, nrm.ctypes.data, orm.ctypes.data,
+                                     emi.ctypes.data, dep.ctypes.data)
+    if r != 0:
+        raise ValueError("bad G-buffer spec")
+    return HostScene(spec=s, camera=make_camera(s), lights=make_lights(s), globals=make_globals(s),
+                     albedo=alb, normal=nrm, orm=orm, emissive=emi, depth=dep,
+                     formats=(fa, ta, fn, tn), texel=(ta, tn))
+
+
+def light_subset(seed: int, view: int, pool: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint32)
+    if synth_lib().hl_synth_subset(seed, view, pool, n, out.ctypes.data if n else None) != 0:
+        raise ValueError("bad subset")
+    return out
+
+
+def cluster_params(lib: abi.Lib, scene: HostScene, grid=None, cap=None) -> abi.hl_cluster_params:
+    gx, gy, gz = grid or scene.spec.clusters
+    p = abi.hl_cluster_params()
+    lib.call("hl_cluster_params_init", C.byref(scene.camera), scene.width, scene.height, gx, gy, gz,
+             scene.spec.cluster_cap if cap is None else cap, C.byref(p))
+    return p
+
+
+# ----------------------------------------------------------- multi-view set
+def multiview_specs(seed: int = SEED_BASE + 16 * 5, small: bool = False) -> tuple[list[SceneSpec], list[np.ndarray], SceneSpec]:
+    """48 probe faces (16 each at 128, 256, 512 px) + 2 XR eyes at 2064x2208 over
+    an 8192-light pool; each view takes a log-uniform [32, 4096] subset.
+    small=True shrinks every view 8x per side (tests)."""
+    pool = SceneSpec("c5_pool", 1, 1, GBUF_REF, 8192, light_box_center=(0.0, 10.0, 0.0),
+                     light_box_size=(100.0, 20.0, 100.0), range_min=1.0, range_max=8.0, seed=seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    faces = [(0.0, 0.0, -1.0), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), (-1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, -1.0, 0.0)]
+    views: list[SceneSpec] = []
+    subsets: list[np.ndarray] = []
+    sizes = [128] * 16 + [256] * 16 + [512] * 16
+    probe = None
+    for i, sz in enumerate(sizes):
+        if i % 6 == 0:
+            probe = (float(rng.uniform(-40, 40)), float(rng.uniform(2, 15)), float(rng.uniform(-40, 40)))
+        f = faces[i % 6]
+        tgt = (probe[0] + f[0], probe[1] + f[1], probe[2] + f[2])
+        s = max(8, sz // 8) if small else sz
+        views.append(SceneSpec(f"face{i}", s, s, GBUF_REF, 0, eye=probe, target=tgt, fov_y_deg=90.0,
+                               num_boxes=60, box_field_half=50.0, seed=seed + 1000 + i))
+    head = (0.0, 1.7, 30.0)
+    for e, dx in enumerate((-0.032, 0.032)):
+        w, h = (2064, 2208) if not small else (2064 // 8, 2208 // 8)
+        views.append(SceneSpec(f"eye{e}", w, h, GBUF_REF, 0, eye=(head[0] + dx, head[1], head[2]),
+                               target=(head[0] + dx, 0.0, 0.0), fov_y_deg=100.0, num_boxes=60,
+                               box_field_half=50.0, seed=seed + 2000 + e))
+    for i in range(len(views)):
+        n = int(round(math.exp(rng.uniform(math.log(32), math.log(4096)))))
+        subsets.append(light_subset(seed, i, pool.num_lights, n))
+    return views, subsets, pool

@@ -1,0 +1,331 @@
+/*
+ * helio_hip.h -- C ABI of libhelio_hip.so, the MI355X (gfx950) implementation of
+ * Helio's deferred-lighting hot path: light assignment (reference 16-px tiles or
+ * log-depth clusters) and per-pixel Cook-Torrance GGX shading over a G-buffer,
+ * plus an adjoint of the shading and a batched multi-view launch.
+ *
+ * Contract (mirrors the pass contract of the reference, without wgpu types):
+ *  - The caller owns every device buffer (allocate with hl_alloc or hipMalloc).
+ *    A context owns only its scratch memory, plans and captured graphs.
+ *  - Every call is asynchronous on the context stream (hl_set_stream) and
+ *    returns an hl_status.  Nothing aborts or prints; the message of the last
+ *    failure of a context is read with hl_last_error.  The reference returns
+ *    helio_core::Error::InvalidPassConfig in the same places
+ *    (crates/helio-core/src/error.rs:62, helio-pass-deferred-light/src/lib.rs:914-919).
+ *  - A context is used by one host thread at a time (the reference runs passes
+ *    single-threaded per frame, crates/helio-core/src/context.rs:15).  Distinct
+ *    contexts may run concurrently on distinct streams.
+ *  - Pointers named *_d are device pointers; everything else is host memory
+ *    that is read during the call and not retained, except inside a batch plan
+ *    (hl_batch_plan_create), which keeps the device pointers of its views.
+ */
+#ifndef HELIO_HIP_H
+#define HELIO_HIP_H
+
+#include "types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define HL_API __declspec(dllexport)
+#else
+#define HL_API __attribute__((visibility("default")))
+#endif
+
+typedef int32_t hl_status;
+#define HL_OK 0
+#define HL_EINVAL (-1)       /* bad argument: null pointer, size, format, alignment   */
+#define HL_EHIP (-2)         /* a HIP runtime call failed                              */
+#define HL_ECAPACITY (-3)    /* cluster pool overflow: lists valid but truncated       */
+#define HL_EUNSUPPORTED (-4) /* input needs a feature outside the hot path             */
+#define HL_ENOMEM (-5)       /* device allocation failed                               */
+
+typedef struct hl_context hl_context;
+typedef struct hl_plan hl_plan;
+
+/* ---------------------------------------------------------------- context */
+HL_API hl_status hl_context_create(int device, hl_context** out);
+HL_API hl_status hl_context_destroy(hl_context* ctx);
+/* stream: a hipStream_t (NULL = the default stream).  Must be a default-priority stream. */
+HL_API hl_status hl_set_stream(hl_context* ctx, void* stream);
+HL_API void* hl_get_stream(const hl_context* ctx);
+HL_API hl_status hl_synchronize(hl_context* ctx);
+HL_API const char* hl_last_error(const hl_context* ctx);
+HL_API const char* hl_status_string(hl_status s);
+/* Library version string and the gfx target it was built for. */
+HL_API const char* hl_version(void);
+
+/* Device-memory helpers (a caller may use hipMalloc instead). */
+HL_API hl_status hl_alloc(hl_context* ctx, size_t bytes, void** out_d);
+HL_API hl_status hl_free(hl_context* ctx, void* ptr_d);
+HL_API hl_status hl_memcpy_htod(hl_context* ctx, void* dst_d, const void* src, size_t bytes);
+HL_API hl_status hl_memcpy_dtoh(hl_context* ctx, void* dst, const void* src_d, size_t bytes);
+HL_API hl_status hl_memcpy_dtod(hl_context* ctx, void* dst_d, const void* src_d, size_t bytes);
+HL_API hl_status hl_memset_d32(hl_context* ctx, void* dst_d, uint32_t value, size_t count);
+
+/* Timing on the context stream (HIP events).  hl_event_elapsed_ms synchronizes on `end`. */
+typedef struct hl_event hl_event;
+HL_API hl_status hl_event_create(hl_context* ctx, hl_event** out);
+HL_API hl_status hl_event_destroy(hl_event* ev);
+HL_API hl_status hl_event_record(hl_context* ctx, hl_event* ev);
+HL_API hl_status hl_event_elapsed_ms(hl_event* start, hl_event* end, float* ms);
+
+/* --------------------------------------------------------------- textures */
+/* Pixel formats the G-buffer attachments may use.  The reference binds
+ * albedo/orm as Rgba8Unorm and normal/emissive as Rgba16Float
+ * (crates/passes/3d/helio-pass-gbuffer/src/lib.rs:240-283) and its bind group
+ * accepts any non-filterable float format, so fp16 and fp32 G-buffers bind
+ * unchanged (helio-pass-deferred-light/src/lib.rs:203-236). */
+enum {
+    HL_FMT_RGBA8_UNORM = 1,
+    HL_FMT_RGBA16_FLOAT = 2,
+    HL_FMT_RGBA32_FLOAT = 3,
+    HL_FMT_R32_FLOAT = 4,   /* depth (Depth32Float, standard Z in [0,1]) */
+    HL_FMT_RG16_FLOAT = 5   /* lightmap UV */
+};
+
+typedef struct hl_texture {
+    const void* data_d;    /* first texel of row 0 (top row) */
+    uint32_t pitch_bytes;  /* bytes between rows; >= width * texel size, multiple of 4 */
+    uint32_t format;       /* HL_FMT_* */
+} hl_texture;
+
+/* The G-buffer the deferred pass reads (libhelio GBufferViews,
+ * crates/libhelio/src/frame.rs:29-38, plus depth).
+ * Supported format sets (albedo+orm / normal+emissive):
+ *   RGBA8_UNORM / RGBA16_FLOAT   (the reference formats)
+ *   RGBA16_FLOAT / RGBA16_FLOAT  (fp16 G-buffer)
+ *   RGBA32_FLOAT / RGBA32_FLOAT  (fp32 G-buffer)
+ * Depth is always R32_FLOAT.
+ * lightmap_uv.data_d must be NULL: every pixel then carries the (-1,-1)
+ * "no lightmap" sentinel the G-buffer pass writes for unbaked instances
+ * (deferred_lighting.wgsl:1036-1038).  Screen-space AO, SSS and anisotropy
+ * inputs are the reference's fallbacks (AO 1, flags 0). */
+typedef struct hl_gbuffer {
+    uint32_t width, height;
+    hl_texture albedo;      /* rgb albedo, a alpha                 */
+    hl_texture normal;      /* xyz world normal, a F0.r            */
+    hl_texture orm;         /* r ao, g roughness, b metallic, a F0.g */
+    hl_texture emissive;    /* rgb emissive, a F0.b                */
+    hl_texture depth;       /* NDC depth; >= 1.0 is sky            */
+    hl_texture lightmap_uv; /* must be NULL (see above)            */
+} hl_gbuffer;
+
+/* ------------------------------------------------------------- light grid */
+#define HL_MAX_SLICES 64
+
+enum { HL_GRID_TILES = 1, HL_GRID_CLUSTERS = 2 };
+
+/* Log-depth froxel grid.  Cell (x, y, z) covers pixels
+ * [x*cell_w, (x+1)*cell_w) x [y*cell_h, (y+1)*cell_h) and NDC depths
+ * [depth_split[z], depth_split[z+1]); its index is x + grid_x*(y + grid_y*z).
+ * Slice z spans view distances near*(far/near)^(z/grid_z) .. ^( (z+1)/grid_z ).
+ * Fill with hl_cluster_params_init; the same bytes must reach the cull and the
+ * shading calls (both the HIP path and any CPU checker read the thresholds
+ * from here, so pixel-to-cluster assignment is identical everywhere). */
+typedef struct hl_cluster_params {
+    uint32_t grid_x, grid_y, grid_z;
+    uint32_t cell_w, cell_h;      /* pixels per cell */
+    uint32_t width, height;       /* screen size */
+    uint32_t cap;                 /* indices kept per cluster, lowest first; 0 = unbounded */
+    float near_plane, far_plane;
+    float r_pad_column;           /* radius padding of the lateral-plane pre-test (max of slice_r_pad) */
+    float reserved;
+    float depth_split[HL_MAX_SLICES + 1];  /* NDC depth where slice k begins */
+    float slice_near[HL_MAX_SLICES];       /* padded view-distance range of slice k */
+    float slice_far[HL_MAX_SLICES];
+    float slice_r_pad[HL_MAX_SLICES];      /* radius padding of slice k (reconstruction error bound) */
+} hl_cluster_params;
+
+/* Host-side helper: compute the slice thresholds from the camera (a HOST copy
+ * of GpuCameraUniforms).  Returns HL_EUNSUPPORTED for a projection that is not
+ * a standard-Z perspective (proj[2][3] must be -1). */
+HL_API hl_status hl_cluster_params_init(const GpuCameraUniforms* camera, uint32_t width,
+                                        uint32_t height, uint32_t grid_x, uint32_t grid_y,
+                                        uint32_t grid_z, uint32_t cap, hl_cluster_params* out);
+
+/* The per-cell light lists the shading pass consumes.  Tile mode is the
+ * reference's tile_light_lists / tile_light_counts pair
+ * (crates/libhelio/src/frame.rs:230-234, 482-491); cluster mode is a compact
+ * offsets/counts/pool triple. */
+typedef struct hl_light_grid {
+    uint32_t mode;                 /* HL_GRID_TILES or HL_GRID_CLUSTERS          */
+    uint32_t num_tiles_x;          /* tiles: ceil(width/16)                       */
+    uint32_t num_tiles_y;          /* tiles: ceil(height/16)                      */
+    const uint32_t* lists_d;       /* tiles: [tile*64 + i]; clusters: pool        */
+    const uint32_t* counts_d;      /* per cell                                    */
+    const uint32_t* offsets_d;     /* clusters: per-cell start in the pool        */
+    const hl_cluster_params* clusters; /* clusters: host copy of the params       */
+    uint32_t lists_capacity;       /* entries readable at lists_d (tiles: 0 = tiles*64);
+                                      list reads beyond it are treated as empty */
+    uint32_t reserved;
+} hl_light_grid;
+
+/* --------------------------------------------------------------- culling */
+
+/* LightCull, reference tile mode.  Replaces the dispatch in
+ * LightCullPass::execute (crates/passes/3d/helio-pass-light-cull/src/lib.rs:284-362)
+ * running light_cull.wgsl::main (shaders/light_cull.wgsl:123-180).
+ * camera_d: the camera storage buffer (only element 0 is read).
+ * params: LightCullParams as written by LightCullPass::prepare (lib.rs:264-282).
+ * Outputs: lists_d[tile*64 + i] for i < counts_d[tile] hold the indices of the
+ * lights whose sphere touches the tile's four lateral planes (directional
+ * lights always), in ascending order, at most 64 (the lowest 64).
+ * num_lights == 0 clears both outputs like lib.rs:285-292.
+ * Bit-exact with the FP32 restatement of light_cull.wgsl. */
+HL_API hl_status hl_light_cull_tiles(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                     const LightCullParams* params, const GpuLight* lights_d,
+                                     uint32_t* lists_d, uint32_t* counts_d);
+
+/* LightCull, clustered mode (log-depth froxels; no counterpart in the reference,
+ * which only has tiles -- the north-star extension of light_cull.wgsl).
+ * A light is listed in a cluster when
+ *   directional: always;
+ *   otherwise: its sphere (view-space centre, radius + padding) passes the four
+ *   lateral planes of the cluster's screen cell AND overlaps the cluster's
+ *   padded view-space AABB; spot lights must also pass a cone test against the
+ *   AABB's bounding sphere (cone half-angle acos(direction_outer.w)).
+ * Outputs: counts_d[c] = min(hits, cap) (cap 0 = unbounded); offsets_d[c] =
+ * exclusive prefix sum of those counts in cluster order; pool_d[offsets[c] + i]
+ * the ascending indices.  If the total exceeds pool_cap, clusters are truncated
+ * (counts_d[c] = min(count, pool_cap - offsets[c]), offsets clamped to pool_cap)
+ * so that nothing is written past pool_cap, and *overflow_d gets
+ * HL_OVERFLOW_POOL; a cluster that hit its cap sets HL_OVERFLOW_CAP.  The call
+ * still returns HL_OK (the word lives on the device; read it with
+ * hl_query_overflow).
+ * overflow_d may be NULL (the context's own word is used). */
+HL_API hl_status hl_light_cull_clusters(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                        const hl_cluster_params* params,
+                                        const GpuLight* lights_d, uint32_t num_lights,
+                                        uint32_t* offsets_d, uint32_t* counts_d,
+                                        uint32_t* pool_d, uint32_t pool_cap,
+                                        uint32_t* overflow_d);
+
+#define HL_OVERFLOW_CAP 1u
+#define HL_OVERFLOW_POOL 2u
+
+/* Reads back the overflow word of the last hl_light_cull_clusters call on
+ * this context (synchronizes the stream).  0 = no truncation. */
+HL_API hl_status hl_query_overflow(hl_context* ctx, uint32_t* flags);
+
+/* ---------------------------------------------------------------- shading */
+enum {
+    HL_SHADE_CLEAR_SKY = 1u   /* write (0,0,0,0) to sky pixels (LoadOp::Clear); default leaves them */
+};
+
+/* DeferredLight fs_main over the G-buffer (deferred_lighting.wgsl:991-1259 with
+ * the reference's fallbacks bound: no shadow casters, no IES/gobo textures, no
+ * RC GI, no lightmaps, SSAO 1; fs_reflection then adds exactly 0,
+ * deferred_lighting.wgsl:1265-1341).  Replaces DeferredLightPass::execute
+ * (crates/passes/3d/helio-pass-deferred-light/src/lib.rs:913-1199).
+ * globals: DeferredGlobals as written by DeferredLightPass::prepare (lib.rs:836-878);
+ *   ambient_color/ambient_intensity/debug_mode/num_tiles_x are read.  Debug modes
+ *   0, 1, 2, 4, 5, 20, 21 are supported; others return HL_EUNSUPPORTED.
+ * out_d: RGBA32F target (pre_aa), row pitch out_pitch_bytes.  Sky pixels
+ *   (depth >= 1) are left untouched unless flags has HL_SHADE_CLEAR_SKY.
+ * Lights with shadow_index != HL_NO_SHADOW or IES/gobo indices >= 0 are shaded
+ *   as unshadowed, untextured lights (those inputs are outside the hot path).
+ * Radiance is within 1e-4 relative of the FP32 restatement evaluating the same
+ * lists. */
+HL_API hl_status hl_deferred_forward(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                     const DeferredGlobals* globals, const hl_gbuffer* gbuffer,
+                                     const GpuLight* lights_d, uint32_t num_lights,
+                                     const hl_light_grid* grid, float* out_d,
+                                     uint32_t out_pitch_bytes, uint32_t flags);
+
+/* Counts Q, the (pixel, light) pairs the forward pass evaluates past the range
+ * test (type != 0 implies dist <= range) -- the "pixel-light evals" of the
+ * throughput metric.  Same inputs as hl_deferred_forward; writes *pairs (host). */
+HL_API hl_status hl_count_pixel_light_pairs(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                            const DeferredGlobals* globals,
+                                            const hl_gbuffer* gbuffer, const GpuLight* lights_d,
+                                            uint32_t num_lights, const hl_light_grid* grid,
+                                            uint64_t* pairs);
+
+/* ---------------------------------------------------------------- adjoint */
+/* Per-pixel gradient planes: d_pixel_d holds 5 planes of width*height floats,
+ * plane-major: dL/d albedo.r, .g, .b, dL/d roughness, dL/d metallic. */
+#define HL_PIXEL_GRAD_PLANES 5
+
+/* Adjoint of hl_deferred_forward for debug mode 0 with respect to per-pixel
+ * albedo, roughness, metallic and per-light position and intensity.  The
+ * differentiated forward resolves F0 from albedo and metallic the way the
+ * G-buffer producer does (F0 = clamp(mix(0.04, albedo, metallic), 0, 0.999),
+ * crates/passes/3d/helio-pass-gbuffer/shaders/gbuffer.wgsl:324-328) instead of
+ * reading the stored F0 channels; with an fp32 G-buffer written by that
+ * producer both forwards agree bit for bit.  The no-contribution cut-offs
+ * (dist > range, NdL == 0, all(radiance < 0.002)) are treated as fixed.
+ * dL_dout_d: RGBA32F per pixel (alpha ignored), pitch dout_pitch_bytes.
+ * Per-light outputs are overwritten: d_light_pos_d[3*i + k], d_light_intensity_d[i].
+ * The per-cell lists are re-derived per wave from the grid, never stored. */
+HL_API hl_status hl_deferred_backward(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                      const DeferredGlobals* globals, const hl_gbuffer* gbuffer,
+                                      const GpuLight* lights_d, uint32_t num_lights,
+                                      const hl_light_grid* grid, const float* dL_dout_d,
+                                      uint32_t dout_pitch_bytes, float* d_pixel_d,
+                                      float* d_light_pos_d, float* d_light_intensity_d);
+
+/* Fused forward + per-pixel L2 loss against target_d + adjoint.  Writes the
+ * forward radiance to out_d, the gradients as hl_deferred_backward does with
+ * dL/dout = 2 (out - target), and the loss sum_p |out_p - target_p|^2 over the
+ * rgb channels of non-sky pixels to *loss_d (a device double, overwritten). */
+HL_API hl_status hl_deferred_forward_backward_l2(
+    hl_context* ctx, const GpuCameraUniforms* camera_d, const DeferredGlobals* globals,
+    const hl_gbuffer* gbuffer, const GpuLight* lights_d, uint32_t num_lights,
+    const hl_light_grid* grid, const float* target_d, uint32_t target_pitch_bytes, float* out_d,
+    uint32_t out_pitch_bytes, double* loss_d, float* d_pixel_d, float* d_light_pos_d,
+    float* d_light_intensity_d);
+
+/* ------------------------------------------------------ batched multi-view */
+/* One view of a batch: a camera, a G-buffer, a subset of a shared light pool
+ * and an output.  The reference renders each view with its own full graph
+ * submit (crates/helio/src/renderer/render.rs:804-873); a plan runs them all
+ * in one persistent launch captured in a HIP graph. */
+typedef struct hl_view_desc {
+    const GpuCameraUniforms* camera_d;
+    GpuCameraUniforms camera;          /* host copy of the same camera (for slice set-up) */
+    DeferredGlobals globals;
+    hl_gbuffer gbuffer;
+    const uint32_t* light_indices_d;   /* ascending pool indices, or NULL = whole pool */
+    uint32_t num_lights;               /* entries of light_indices_d (pool size if NULL) */
+    float* out_d;
+    uint32_t out_pitch_bytes;
+    /* Optional fused L2 adjoint (all NULL = forward only). */
+    const float* target_d;
+    uint32_t target_pitch_bytes;
+    double* loss_d;
+    float* d_pixel_d;                  /* 5 planes of width*height              */
+    float* d_light_pos_d;              /* pool-sized: 3 floats per pool light   */
+    float* d_light_intensity_d;        /* pool-sized                            */
+} hl_view_desc;
+
+typedef struct hl_autotune_opts {
+    uint32_t enable;         /* 0 = use the default shape for every view              */
+    uint32_t max_trials;     /* candidates timed per view class (0 = all)             */
+    const char* cache_path;  /* file the results are kept in (NULL = no file cache)   */
+} hl_autotune_opts;
+
+/* Chosen configuration of one view (after hl_batch_plan_create). */
+typedef struct hl_view_config {
+    uint32_t wave_tile_w, wave_tile_h;   /* pixels one wavefront shades           */
+    uint32_t grid_x, grid_y, grid_z;     /* cluster grid                          */
+    float est_ms;                        /* autotuner's measured time, 0 if none  */
+} hl_view_config;
+
+HL_API hl_status hl_batch_plan_create(hl_context* ctx, const GpuLight* pool_d, uint32_t pool_size,
+                                      const hl_view_desc* views, uint32_t num_views,
+                                      const hl_autotune_opts* opts, hl_plan** out);
+/* Record the plan's whole frame (light gather, clustering, shading, adjoints)
+ * into a HIP graph; hl_batch_launch then replays it.  Without a capture,
+ * hl_batch_launch runs the same launches directly. */
+HL_API hl_status hl_batch_capture(hl_plan* plan);
+HL_API hl_status hl_batch_launch(hl_plan* plan);
+HL_API hl_status hl_batch_get_config(const hl_plan* plan, uint32_t view, hl_view_config* cfg);
+HL_API hl_status hl_plan_destroy(hl_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HELIO_HIP_H */

@@ -1,0 +1,121 @@
+"""ctypes binding of oracle/liboracle.so (the CPU checker; tests only)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from helio_amd import abi
+
+ROOT = Path(__file__).resolve().parent.parent
+P = C.POINTER
+vp = C.c_void_p
+u32 = C.c_uint32
+
+
+class or64_pixel(C.Structure):
+    _fields_ = [("wp", C.c_double * 3), ("N", C.c_double * 3), ("V", C.c_double * 3),
+                ("albedo", C.c_double * 3), ("rough", C.c_double), ("metal", C.c_double),
+                ("ao", C.c_double), ("emissive", C.c_double * 3), ("alpha", C.c_double),
+                ("sky", C.c_int)]
+
+
+_SIGS = {
+    "or_light_cull_tiles": (None, [P(abi.GpuCameraUniforms), P(abi.LightCullParams), vp, vp, vp]),
+    "or_light_cull_clusters": (u32, [P(abi.GpuCameraUniforms), P(abi.hl_cluster_params), vp, u32, vp, vp, vp, u32]),
+    "or_cluster_hits": (u32, [P(abi.GpuCameraUniforms), P(abi.hl_cluster_params), vp, u32, u32]),
+    "or_slice_of_depth": (u32, [P(abi.hl_cluster_params), C.c_float]),
+    "or_shade": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
+                              P(abi.hl_light_grid), vp, u32, u32, u32, u32]),
+    "or_count_pairs": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
+                                    P(abi.hl_light_grid)]),
+    "or_check_conservative": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer),
+                                           vp, u32, P(abi.hl_light_grid), u32, u32, P(C.c_uint64)]),
+    "or_pbr_direct_light": (None, [vp, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
+                                   P(C.c_float), C.c_float, C.c_float, P(C.c_float)]),
+    "or64_decode": (None, [P(abi.GpuCameraUniforms), P(abi.hl_gbuffer), vp]),
+    "or64_shade_pixel": (C.c_uint64, [P(or64_pixel), P(abi.DeferredGlobals), vp, u32, P(C.c_double)]),
+    "or64_forward": (C.c_double, [vp, u32, u32, P(abi.DeferredGlobals), vp, u32, vp, vp, u32]),
+    "or64_backward_l2": (None, [vp, u32, u32, P(abi.DeferredGlobals), vp, u32, vp, u32, vp, vp, vp]),
+    "or64_backward": (None, [vp, u32, u32, P(abi.DeferredGlobals), vp, u32, vp, u32, vp, vp, vp]),
+    "or64_fd": (C.c_double, [vp, u32, u32, P(abi.DeferredGlobals), vp, u32, vp, u32, C.c_int, u32,
+                             C.c_double, P(C.c_int)]),
+}
+
+_LIB = None
+
+
+def oracle() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        p = ROOT / "oracle" / "liboracle.so"
+        if not p.exists():
+            raise FileNotFoundError(f"{p} missing: run `python -m helio_amd.build`")
+        lib = C.CDLL(str(p))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = lib
+    return _LIB
+
+
+def f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def cull_tiles(scene):
+    from helio_amd.device import light_cull_params
+    p = light_cull_params(scene)
+    tiles = p.num_tiles_x * p.num_tiles_y
+    lists = np.zeros(tiles * 64, dtype=np.uint32)
+    counts = np.zeros(tiles, dtype=np.uint32)
+    oracle().or_light_cull_tiles(C.byref(scene.camera), C.byref(p), scene.lights.ctypes.data,
+                                 lists.ctypes.data, counts.ctypes.data)
+    return lists.reshape(tiles, 64), counts
+
+
+def cull_clusters(scene, params, pool_cap):
+    n = params.grid_x * params.grid_y * params.grid_z
+    off = np.zeros(n, dtype=np.uint32)
+    cnt = np.zeros(n, dtype=np.uint32)
+    pool = np.zeros(max(pool_cap, 1), dtype=np.uint32)
+    flags = oracle().or_light_cull_clusters(C.byref(scene.camera), C.byref(params),
+                                            scene.lights.ctypes.data if len(scene.lights) else None,
+                                            len(scene.lights), off.ctypes.data, cnt.ctypes.data,
+                                            pool.ctypes.data, pool_cap)
+    return off, cnt, pool[:pool_cap], flags
+
+
+def tile_grid_host(scene, lists, counts):
+    g = abi.hl_light_grid()
+    g.mode = abi.HL_GRID_TILES
+    g.num_tiles_x = (scene.width + 15) // 16
+    g.num_tiles_y = (scene.height + 15) // 16
+    g.lists_d, g.counts_d = lists.ctypes.data, counts.ctypes.data
+    g.lists_capacity = lists.size
+    return g
+
+
+def cluster_grid_host(params, off, cnt, pool):
+    g = abi.hl_light_grid()
+    g.mode = abi.HL_GRID_CLUSTERS
+    g.lists_d, g.counts_d, g.offsets_d = pool.ctypes.data, cnt.ctypes.data, off.ctypes.data
+    g.clusters = C.pointer(params)
+    g.lists_capacity = pool.size
+    return g
+
+
+def shade(scene, grid, globals_=None, flags=0, rows=None, out=None):
+    """FP32 restatement of fs_main; returns (rgba [H,W,4], pairs)."""
+    W, H = scene.width, scene.height
+    if out is None:
+        out = np.zeros((H, W, 4), dtype=np.float32)
+    g = globals_ if globals_ is not None else scene.globals
+    gb = scene.gbuffer_host()
+    r0, r1 = rows if rows else (0, 0)
+    pairs = oracle().or_shade(C.byref(scene.camera), C.byref(g), C.byref(gb),
+                              scene.lights.ctypes.data if len(scene.lights) else None, len(scene.lights),
+                              C.byref(grid), out.ctypes.data, W * 16, flags, r0, r1)
+    return out, pairs

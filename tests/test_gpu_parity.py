@@ -1,0 +1,168 @@
+"""GPU parity of the HIP path against the FP32 restatement (oracle).
+
+Bars (stated per check):
+  * light lists (tiles and clusters): bit-exact, every entry, every count;
+  * radiance: |hip - oracle| <= 1e-4 * |oracle| + 1e-6 per channel, on the
+    oracle evaluating the same lists;
+  * Q (pixel-light pairs past the range test): equal.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helio_amd import abi, synth
+from helio_amd.device import (ClusterGrid, DeferredLightPass, LightCullPass, TileGrid,
+                              light_cull_params, upload_scene)
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 1e-4, 1e-6
+
+_SCENES = {}
+
+
+def scene(name):
+    if name not in _SCENES:
+        _SCENES[name] = synth.make_scene(name)
+    return _SCENES[name]
+
+
+def check_radiance(hip, ref, mask=None, rtol=RTOL, atol=ATOL):
+    if mask is not None:
+        hip, ref = hip[mask], ref[mask]
+    err = np.abs(hip.astype(np.float64) - ref.astype(np.float64))
+    tol = rtol * np.abs(ref.astype(np.float64)) + atol
+    bad = err > tol
+    rel = err / (np.abs(ref) + 1e-30)
+    if bad.any():
+        i = np.argwhere(bad)[:5]
+        detail = "; ".join(f"{tuple(int(v) for v in k)} hip={hip[tuple(k)]:.8g} ref={ref[tuple(k)]:.8g}"
+                           for k in i)
+    else:
+        detail = ""
+    assert not bad.any(), (f"{int(bad.sum())} of {bad.size} values beyond 1e-4 rel; "
+                           f"max rel {float(rel.max()):.3e}: {detail}")
+    return float(rel.max()) if rel.size else 0.0
+
+
+def hip_tiles(ctx, sc):
+    ds = upload_scene(ctx, sc)
+    cull = LightCullPass(ctx, sc.width, sc.height)
+    cull.prepare(ds)
+    cull.execute(ds)
+    lists = cull.tiles.lists.download(np.uint32).reshape(-1, 64)
+    counts = cull.tiles.counts.download(np.uint32)
+    return ds, cull, lists, counts
+
+
+@pytest.mark.parametrize("name", ["c1_720p_64", "c2_1080p_1k"])
+def test_tile_cull_bit_exact(ctx, name):
+    sc = scene(name)
+    print(f"[tile cull] {name}", flush=True)
+    _, _, lists, counts = hip_tiles(ctx, sc)
+    ol, oc = O.cull_tiles(sc)
+    assert np.array_equal(counts, oc)
+    for t in np.nonzero(oc)[0]:
+        n = oc[t]
+        assert np.array_equal(lists[t, :n], ol[t, :n]), f"tile {t}"
+
+
+def hip_clusters(ctx, sc, params, pool_cap):
+    ds = upload_scene(ctx, sc)
+    cull = LightCullPass(ctx, sc.width, sc.height, clusters=params, pool_cap=pool_cap)
+    cull.prepare(ds)
+    cull.execute(ds)
+    off, cnt, pool = cull.cgrid.download()
+    return ds, cull, off, cnt, pool, cull.overflow()
+
+
+@pytest.mark.parametrize("name,pool_cap", [("c1_720p_64", 1 << 20), ("c2_1080p_1k", 1 << 21),
+                                           ("c3_4k_16k", 1 << 25)])
+def test_cluster_cull_bit_exact(ctx, hlib, name, pool_cap):
+    sc = scene(name)
+    print(f"[cluster cull] {name}", flush=True)
+    params = synth.cluster_params(hlib, sc)
+    _, _, off, cnt, pool, flags = hip_clusters(ctx, sc, params, pool_cap)
+    ooff, ocnt, opool, oflags = O.cull_clusters(sc, params, pool_cap)
+    assert flags == oflags
+    assert np.array_equal(cnt, ocnt)
+    assert np.array_equal(off, ooff)
+    total = int(ocnt.sum())
+    # pool is dense in cluster order: compare the used prefix
+    used = int(ooff[-1] + ocnt[-1]) if len(ocnt) else 0
+    assert np.array_equal(pool[:used], opool[:used])
+    print(f"  clusters={len(cnt)} entries={total} max={int(ocnt.max())} flags={flags}")
+
+
+def test_cluster_cap_and_pool_truncation(ctx, hlib):
+    sc = scene("c2_1080p_1k")
+    params = synth.cluster_params(hlib, sc, cap=32)
+    _, _, off, cnt, pool, flags = hip_clusters(ctx, sc, params, 5000)
+    ooff, ocnt, opool, oflags = O.cull_clusters(sc, params, 5000)
+    assert flags == oflags and (flags & abi.HL_OVERFLOW_CAP) and (flags & abi.HL_OVERFLOW_POOL)
+    assert np.array_equal(cnt, ocnt) and np.array_equal(off, ooff)
+    assert np.array_equal(pool, opool[:5000])
+
+
+@pytest.mark.parametrize("name", ["c1_720p_64", "c2_1080p_1k"])
+def test_shade_tiles_parity(ctx, name):
+    sc = scene(name)
+    print(f"[shade tiles] {name}", flush=True)
+    ds, cull, lists, counts = hip_tiles(ctx, sc)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    shade.out.fill_u32(0)
+    shade.execute(ds, cull.publish())
+    out = shade.output()
+    pairs = shade.count_pairs(ds, cull.publish())
+    lists = np.ascontiguousarray(lists.reshape(-1))
+    grid = O.tile_grid_host(sc, lists, counts)
+    ref, opairs = O.shade(sc, grid)
+    sky = sc.depth >= 1.0
+    assert np.all(out[sky] == 0.0)
+    r = check_radiance(out, ref, ~sky)
+    assert pairs == opairs
+    print(f"  max rel err {r:.2e}, Q={pairs}")
+
+
+@pytest.mark.parametrize("name,rows", [("c1_720p_64", None), ("c2_1080p_1k", None),
+                                       ("c3_4k_16k", (1000, 1064))])
+def test_shade_clusters_parity(ctx, hlib, name, rows):
+    sc = scene(name)
+    print(f"[shade clusters] {name}", flush=True)
+    params = synth.cluster_params(hlib, sc)
+    ds, cull, off, cnt, pool, flags = hip_clusters(ctx, sc, params, 1 << 25)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    shade.out.fill_u32(0)
+    shade.execute(ds, cull.publish())
+    out = shade.output()
+    grid = O.cluster_grid_host(params, off, cnt, pool)
+    ref, opairs = O.shade(sc, grid, rows=rows)
+    sl = slice(None) if rows is None else slice(rows[0], rows[1])
+    sky = sc.depth[sl] >= 1.0
+    r = check_radiance(out[sl], ref[sl], ~sky)
+    if rows is None:
+        assert shade.count_pairs(ds, cull.publish()) == opairs
+    print(f"  max rel err {r:.2e}")
+
+
+def test_zero_lights(ctx):
+    sc = synth.make_scene("c1_720p_64", num_lights=0)
+    ds = upload_scene(ctx, sc)
+    cull = LightCullPass(ctx, sc.width, sc.height)
+    cull.prepare(ds)
+    cull.tiles.counts.fill_u32(7)
+    cull.execute(ds)
+    assert np.all(cull.tiles.counts.download(np.uint32) == 0)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    shade.execute(ds, cull.publish(), flags=abi.HL_SHADE_CLEAR_SKY)
+    out = shade.output()
+    lists = np.zeros(cull.tiles.ntx * cull.tiles.nty * 64, np.uint32)
+    counts = np.zeros(cull.tiles.ntx * cull.tiles.nty, np.uint32)
+    ref, pairs = O.shade(sc, O.tile_grid_host(sc, lists, counts), flags=abi.HL_SHADE_CLEAR_SKY)
+    assert pairs == 0
+    check_radiance(out, ref)
