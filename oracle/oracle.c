@@ -530,6 +530,17 @@ static void or_setup_pixel(const GpuCameraUniforms* cam, const hl_gbuffer* gb, u
     or_normalize(p->V);
 }
 
+void or_world_pos(const GpuCameraUniforms* cam, uint32_t width, uint32_t height, uint32_t x,
+                  uint32_t y, float depth, float wp[3]) {
+    const float u = ((float)x + 0.5f) / (float)width, v = ((float)y + 0.5f) / (float)height;
+    float ndc4[4] = {u * 2.0f - 1.0f, 1.0f - v * 2.0f, depth, 1.0f};
+    float wh[4];
+    or_mat4_mul(cam->inv_view_proj, ndc4, wh);
+    wp[0] = wh[0] / wh[3];
+    wp[1] = wh[1] / wh[3];
+    wp[2] = wh[2] / wh[3];
+}
+
 static int or_debug_supported(uint32_t m) {
     return m == 0 || m == 1 || m == 2 || m == 4 || m == 5 || m == 20 || m == 21;
 }

@@ -76,6 +76,11 @@ uint64_t or_check_conservative(const GpuCameraUniforms* cam, const DeferredGloba
                                const hl_light_grid* grid, uint32_t cap, uint32_t stride,
                                uint64_t* checked);
 
+/* World position of pixel (x, y) exactly as fs_main reconstructs it in FP32
+ * (deferred_lighting.wgsl:1043-1047). */
+void or_world_pos(const GpuCameraUniforms* cam, uint32_t width, uint32_t height, uint32_t x,
+                  uint32_t y, float depth, float wp[3]);
+
 /* One pixel-light term of pbr_direct_light in FP32 (exposed for known-answer tests). */
 void or_pbr_direct_light(const GpuLight* light, const float world_pos[3], const float N[3],
                          const float V[3], const float F0[3], const float albedo[3],

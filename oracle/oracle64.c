@@ -86,15 +86,13 @@ void or64_decode(const GpuCameraUniforms* cam, const hl_gbuffer* gb, or64_pixel*
             p->ao = o[0];
             p->rough = o[1];
             p->metal = o[2];
-            double u = ((double)x + 0.5) / (double)gb->width, v = ((double)y + 0.5) / (double)gb->height;
-            double ndc[4] = {u * 2.0 - 1.0, 1.0 - v * 2.0, (double)depth, 1.0};
-            double wh[4];
-            for (int r = 0; r < 4; ++r) {
-                wh[r] = 0.0;
-                for (int c = 0; c < 4; ++c) wh[r] += (double)cam->inv_view_proj[c * 4 + r] * ndc[c];
-            }
+            /* The world position is the FP32 reconstruction of the shader (its
+             * rounding near depth 1 is a property of the input, not of the
+             * lighting arithmetic this path evaluates in FP64). */
+            float wp32[3];
+            or_world_pos(cam, gb->width, gb->height, x, y, depth, wp32);
             for (int k = 0; k < 3; ++k) {
-                p->wp[k] = wh[k] / wh[3];
+                p->wp[k] = wp32[k];
                 p->V[k] = (double)cam->position_near[k] - p->wp[k];
             }
             d_normalize(p->V);

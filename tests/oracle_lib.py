@@ -33,7 +33,7 @@ _SIGS = {
     "or_check_conservative": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer),
                                            vp, u32, P(abi.hl_light_grid), u32, u32, P(C.c_uint64)]),
     "or_pbr_direct_light": (None, [vp, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
-                                   P(C.c_float), C.c_float, C.c_float, P(C.c_float)]),
+                                   C.c_float, C.c_float, P(C.c_float)]),
     "or64_decode": (None, [P(abi.GpuCameraUniforms), P(abi.hl_gbuffer), vp]),
     "or64_shade_pixel": (C.c_uint64, [P(or64_pixel), P(abi.DeferredGlobals), vp, u32, P(C.c_double)]),
     "or64_forward": (C.c_double, [vp, u32, u32, P(abi.DeferredGlobals), vp, u32, vp, vp, u32]),
