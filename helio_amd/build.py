@@ -29,6 +29,8 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-ffp-contract=fast-honor-pragmas",
+    # hardware FP64 atomic add for the adjoint's per-light accumulators
+    "-munsafe-fp-atomics",
     # packed FP32 (v_pk_*_f32) stays off: only scalar VALU FP32 is generated
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",

@@ -567,3 +567,72 @@ HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUnifor
 }
 
 }  // extern "C"
+
+namespace {
+
+hl_status check_rgba32(hl_context* c, const void* p, uint32_t pitch, uint32_t w, const char* what) {
+    if (!p) return fail(c, HL_EINVAL, "%s is null", what);
+    if (pitch < 16u * w || pitch % 16 != 0 || ((uintptr_t)p) % 16)
+        return fail(c, HL_EINVAL, "%s: pitch %u invalid for width %u (RGBA32F, 16-B aligned)", what,
+                    pitch, w);
+    return HL_OK;
+}
+
+hl_status run_adjoint(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
+                      const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t n,
+                      const hl_light_grid* grid, const float* in_d, uint32_t in_pitch,
+                      float* out_d, uint32_t out_pitch, double* loss_d, float* d_pixel_d,
+                      float* d_pos_d, float* d_int_d, bool fused) {
+    if (!c) return HL_EINVAL;
+    ShadeArgs a;
+    int set;
+    hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, n, grid, &a, &set, true);
+    if (s != HL_OK) return s;
+    if ((s = check_rgba32(c, in_d, in_pitch, gb->width, fused ? "target" : "dL_dout")) != HL_OK) return s;
+    if (fused) {
+        if ((s = check_rgba32(c, out_d, out_pitch, gb->width, "output")) != HL_OK) return s;
+        if (!loss_d) return fail(c, HL_EINVAL, "loss is null");
+    }
+    if (!d_pixel_d) return fail(c, HL_EINVAL, "d_pixel is null");
+    if (n && (!d_pos_d || !d_int_d)) return fail(c, HL_EINVAL, "light gradient outputs are null");
+    if ((s = ensure(c, c->accum, sizeof(double) * 4 * (size_t)(n ? n : 1))) != HL_OK) return s;
+    double* acc = (double*)c->accum.p;
+    HL_HIP(c, launch_zero_f64(c->stream, acc, 4 * (size_t)n));
+    if (fused) HL_HIP(c, launch_zero_f64(c->stream, loss_d, 1));
+    a.dout = in_d;
+    a.dout_pitch = in_pitch;
+    a.out = out_d;
+    a.out_pitch = out_pitch;
+    a.loss = loss_d;
+    a.d_pixel = d_pixel_d;
+    a.d_light = acc;
+    HL_HIP(c, launch_shade(c->stream, set, fused ? kModeFusedL2 : kModeBackward, a));
+    HL_HIP(c, launch_light_grads_to_f32(c->stream, acc, n, nullptr, d_pos_d, d_int_d));
+    return HL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+HL_API hl_status hl_deferred_backward(hl_context* c, const GpuCameraUniforms* camera_d,
+                                      const DeferredGlobals* g, const hl_gbuffer* gb,
+                                      const GpuLight* lights_d, uint32_t num_lights,
+                                      const hl_light_grid* grid, const float* dL_dout_d,
+                                      uint32_t dout_pitch_bytes, float* d_pixel_d,
+                                      float* d_light_pos_d, float* d_light_intensity_d) {
+    return run_adjoint(c, camera_d, g, gb, lights_d, num_lights, grid, dL_dout_d, dout_pitch_bytes,
+                       nullptr, 0, nullptr, d_pixel_d, d_light_pos_d, d_light_intensity_d, false);
+}
+
+HL_API hl_status hl_deferred_forward_backward_l2(
+    hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
+    const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights, const hl_light_grid* grid,
+    const float* target_d, uint32_t target_pitch_bytes, float* out_d, uint32_t out_pitch_bytes,
+    double* loss_d, float* d_pixel_d, float* d_light_pos_d, float* d_light_intensity_d) {
+    return run_adjoint(c, camera_d, g, gb, lights_d, num_lights, grid, target_d, target_pitch_bytes,
+                       out_d, out_pitch_bytes, loss_d, d_pixel_d, d_light_pos_d, d_light_intensity_d,
+                       true);
+}
+
+}  // extern "C"

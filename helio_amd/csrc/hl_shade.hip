@@ -2,24 +2,65 @@
 //
 // Launch shape: 256-thread blocks covering 32x8 pixels; each wavefront owns an
 // 8x8 pixel tile.  A wavefront
-//   1. loads depth and the G-buffer (coalesced rows of 32 px per block),
-//      reconstructs the world position exactly and finds each pixel's cell;
+//   1. loads depth and the G-buffer (rows of 32 px per block), reconstructs the
+//      world position exactly and finds each pixel's cell (tile or cluster);
 //   2. for each distinct cell among its pixels: reduces the AABB of those
-//      pixels' world positions, streams the cell's list in chunks of 64, tests
-//      each light sphere against the AABB, and compacts the survivors into an
-//      LDS list with ballot + prefix popcount (order preserved);
-//   3. walks the LDS list with a wave-uniform index (light records come in
-//      through scalar loads) and evaluates pbr_direct_light for its pixels.
+//      pixels' world positions (DPP), streams the cell's list in chunks of 64,
+//      tests each light sphere against the AABB and compacts the survivors --
+//      index and 64-byte light record -- into LDS with ballot + prefix popcount
+//      (list order preserved);
+//   3. walks the LDS list (uniform address, broadcast reads) and evaluates
+//      pbr_direct_light for its pixels.
 // Every pixel therefore sums exactly the lights of its own cell list that pass
 // the range test, in list order, as deferred_lighting.wgsl:1125-1145 does.
+// The adjoint re-derives the same per-wave lists instead of storing them.
 #include "hl_internal.hpp"
 
 namespace hl {
 
-constexpr int kLdsCap = 256;
+constexpr int kStage = 64;  // LDS light records per wave and chunk
 constexpr float kPi = 3.14159265359f;
 constexpr float kInvPi = 1.0f / 3.14159265359f;
 constexpr float kThresh = 0.002f;  // deferred_lighting.wgsl:781
+
+// ------------------------------------------------------------ wave ops (DPP)
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float hl_dpp(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src),
+                                                       CTRL, ROW_MASK, 0xF, false));
+}
+
+// Full-wave reductions: quad swaps, half-row and row mirrors, then the
+// row_bcast:15 / row_bcast:31 steps; lane 63 holds the result.
+__device__ __forceinline__ float dpp_min(float v) {
+    const float I = __builtin_inff();
+    v = fminf(v, hl_dpp<0xB1>(I, v));
+    v = fminf(v, hl_dpp<0x4E>(I, v));
+    v = fminf(v, hl_dpp<0x141>(I, v));
+    v = fminf(v, hl_dpp<0x140>(I, v));
+    v = fminf(v, hl_dpp<0x142, 0xA>(I, v));
+    v = fminf(v, hl_dpp<0x143, 0xC>(I, v));
+    return hl_readlane(v, 63);
+}
+__device__ __forceinline__ float dpp_max(float v) {
+    const float I = -__builtin_inff();
+    v = fmaxf(v, hl_dpp<0xB1>(I, v));
+    v = fmaxf(v, hl_dpp<0x4E>(I, v));
+    v = fmaxf(v, hl_dpp<0x141>(I, v));
+    v = fmaxf(v, hl_dpp<0x140>(I, v));
+    v = fmaxf(v, hl_dpp<0x142, 0xA>(I, v));
+    v = fmaxf(v, hl_dpp<0x143, 0xC>(I, v));
+    return hl_readlane(v, 63);
+}
+__device__ __forceinline__ float dpp_sum(float v) {
+    v = v + hl_dpp<0xB1>(0.0f, v);
+    v = v + hl_dpp<0x4E>(0.0f, v);
+    v = v + hl_dpp<0x141>(0.0f, v);
+    v = v + hl_dpp<0x140>(0.0f, v);
+    v = v + hl_dpp<0x142, 0xA>(0.0f, v);
+    v = v + hl_dpp<0x143, 0xC>(0.0f, v);
+    return hl_readlane(v, 63);
+}
 
 struct Pixel {
     float wx, wy, wz;
@@ -28,42 +69,41 @@ struct Pixel {
     float f0[3];
     float alb[3];
     float rough, metal, ao;
-    float ndv, kk, gv, one_m;
+    float ndv, kk, gv, ev, one_m;
+    bool f0_free[3];
 };
 
-// pbr_direct_light for one light, fast arithmetic with exact decisions.
-// Returns false when the term is exactly zero in the restated shader; *in_range
-// reports the range test of deferred_lighting.wgsl:1128-1131.
+// Per-(pixel, light) quantities of pbr_direct_light (deferred_lighting.wgsl:699-820).
 struct Term {
-    float lx, ly, lz;    // L
-    float rad[3];        // radiance
+    float lx, ly, lz;
+    float rad[3];
     float ndl;
     float d, inv_d, att0, fall, s, dsdcos;
-    float hx, hy, hz, inv_h;  // H (normalized) and 1/|V+L|
+    float hx, hy, hz, inv_h;
     float hdv, u, P, ndh, a2, den, Q, D, el, gl, G, den2;
     float F[3], B[3];
 };
 
+// Fast arithmetic, exact decisions.  Returns false when the term is exactly
+// zero in the restated shader; in_range reports the range test of
+// deferred_lighting.wgsl:1128-1131.
 template <bool kFull>
 __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Term& t,
                                           bool& in_range) {
     const bool dir = __builtin_isinf(L.p0.w);
     if (dir) {
         in_range = true;
-        float dx = -L.p2.x, dy = -L.p2.y, dz = -L.p2.z;
-        float inv = hl_f_rsq(dx * dx + dy * dy + dz * dz);
+        const float dx = -L.p2.x, dy = -L.p2.y, dz = -L.p2.z;
+        const float inv = hl_f_rsq(dx * dx + dy * dy + dz * dz);
         t.lx = dx * inv;
         t.ly = dy * inv;
         t.lz = dz * inv;
         t.rad[0] = L.p1.x;
         t.rad[1] = L.p1.y;
         t.rad[2] = L.p1.z;
-        t.att0 = 1.0f;
-        t.fall = 1.0f;
-        t.s = 1.0f;
-        t.dsdcos = 0.0f;
-        t.d = 0.0f;
-        t.inv_d = 0.0f;
+        t.att0 = t.fall = t.s = 1.0f;
+        t.dsdcos = t.d = t.inv_d = 0.0f;
+        if (fmaxf(fmaxf(t.rad[0], t.rad[1]), t.rad[2]) < kThresh) return false;
     } else {
         const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
         const float d2 = tx * tx + ty * ty + tz * tz;
@@ -90,8 +130,8 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
         t.dsdcos = 0.0f;
         if (L.p2.w != kNoCone) {
             // The cone factor divides by cos(inner) - cos(outer) (0.005 for a
-            // 10-degree cone), which would amplify the last-bit error of a fast
-            // L ~200x; it is evaluated exactly as deferred_lighting.wgsl:739-742.
+            // 10-degree cone), which would amplify a last-bit error of a fast L
+            // ~200x; it is evaluated exactly as deferred_lighting.wgsl:739-742.
             float cos_a;
             {
 #pragma clang fp contract(off)
@@ -120,11 +160,8 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
             if (hl_x_radiance_max(L, p.wx, p.wy, p.wz, &dummy) < kThresh) return false;
         }
     }
-    const float ndl_raw = p.nx * t.lx + p.ny * t.ly + p.nz * t.lz;
-    t.ndl = fmaxf(ndl_raw, 0.0f);
+    t.ndl = fmaxf(p.nx * t.lx + p.ny * t.ly + p.nz * t.lz, 0.0f);
     if (t.ndl == 0.0f) return false;
-    if (dir && fmaxf(fmaxf(t.rad[0], t.rad[1]), t.rad[2]) < kThresh) return false;
-    // H = normalize(V + L)
     const float hx = p.vx + t.lx, hy = p.vy + t.ly, hz = p.vz + t.lz;
     const float inv_h = hl_f_rsq(hx * hx + hy * hy + hz * hz);
     t.inv_h = inv_h;
@@ -135,14 +172,12 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     t.u = fminf(fmaxf(1.0f - t.hdv, 0.0f), 1.0f);
     const float u2 = t.u * t.u;
     t.P = u2 * u2 * t.u;
-    // distribution_ggx
     const float a = p.rough * p.rough;
     t.a2 = a * a;
     t.ndh = fmaxf(p.nx * t.hx + p.ny * t.hy + p.nz * t.hz, 0.0f);
     t.den = t.ndh * t.ndh * (t.a2 - 1.0f) + 1.0f;
     t.Q = kPi * t.den * t.den + 0.0001f;
     t.D = t.a2 * hl_f_rcp(t.Q);
-    // geometry_smith
     t.el = t.ndl * (1.0f - p.kk) + p.kk + 0.0001f;
     t.gl = t.ndl * hl_f_rcp(t.el);
     t.G = p.gv * t.gl;
@@ -156,10 +191,154 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     return true;
 }
 
+// Reverse-mode derivative of one active term given dL/dout (gl).  Per-pixel
+// gradients accumulate into dA (albedo), dR (roughness), dM (metallic); the
+// light's position and intensity gradients are returned in g (x, y, z, I).
+__device__ __forceinline__ void adjoint_term(const HotLight& L, const Pixel& p, const Term& t,
+                                             const float gl[3], float dA[3], float& dR, float& dM,
+                                             float4& g) {
+    const float m = p.metal, r = p.rough;
+    const float inv_den2 = hl_f_rcp(t.den2);
+    const float DG_den2 = t.D * t.G * inv_den2;
+    const float invQ = hl_f_rcp(t.Q);
+    // dD/dr and dG/dr (pbr_eval.wgsl:12-43)
+    const float dDdr = (invQ - t.a2 * 2.0f * kPi * t.den * t.ndh * t.ndh * invQ * invQ) * 4.0f * r * r * r;
+    const float inv_ev = hl_f_rcp(p.ev), inv_el = hl_f_rcp(t.el);
+    const float dgdk_v = -p.ndv * (1.0f - p.ndv) * inv_ev * inv_ev;
+    const float dgdk_l = -t.ndl * (1.0f - t.ndl) * inv_el * inv_el;
+    const float dGdr = (dgdk_v * t.gl + p.gv * dgdk_l) * (r + 1.0f) * 0.25f;
+    const float dSdr = (dDdr * t.G + t.D * dGdr) * inv_den2;
+    const float atten = t.att0 * t.fall * t.s;
+    float adj_NdL = 0.0f, adj_atten = 0.0f, adj_D = 0.0f, adj_G = 0.0f, adj_den2 = 0.0f, adj_P = 0.0f;
+    float dI = 0.0f;
+    const float cI = L.p3.w;
+    const float col[3] = {L.p3.x, L.p3.y, L.p3.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float W = t.rad[c] * t.ndl;
+        const float gw = gl[c] * W;
+        const float dBdF = -(1.0f - m) * p.alb[c] * kInvPi + DG_den2;
+        const float kD = (1.0f - t.F[c]) * p.one_m;
+        dA[c] += gw * (kD * kInvPi + (p.f0_free[c] ? m * (1.0f - t.P) * dBdF : 0.0f));
+        dM += gw * (-(1.0f - t.F[c]) * p.alb[c] * kInvPi +
+                    (p.f0_free[c] ? (p.alb[c] - 0.04f) * (1.0f - t.P) * dBdF : 0.0f));
+        dR += gw * dSdr * t.F[c];
+        dI += gl[c] * t.B[c] * col[c] * atten * t.ndl;
+        const float adj_B = gw;
+        adj_NdL += gl[c] * t.B[c] * t.rad[c];
+        adj_atten += gl[c] * t.B[c] * t.ndl * col[c] * cI;
+        adj_D += adj_B * t.G * t.F[c] * inv_den2;
+        adj_G += adj_B * t.D * t.F[c] * inv_den2;
+        adj_den2 -= adj_B * DG_den2 * t.F[c] * inv_den2;
+        adj_P += adj_B * dBdF * (1.0f - p.f0[c]);
+    }
+    g = make_float4(0.0f, 0.0f, 0.0f, dI);
+    if (__builtin_isinf(L.p0.w)) return;  // directional: no position
+    float ahx = 0.0f, ahy = 0.0f, ahz = 0.0f;
+    if (t.hdv > 0.0f && t.u > 0.0f) {
+        const float u2 = t.u * t.u;
+        const float a = adj_P * (-5.0f * u2 * u2);
+        ahx += a * p.vx;
+        ahy += a * p.vy;
+        ahz += a * p.vz;
+    }
+    if (t.ndh > 0.0f) {
+        const float dDdNdH = -t.a2 * invQ * invQ * (2.0f * kPi * t.den) * (2.0f * t.ndh * (t.a2 - 1.0f));
+        const float a = adj_D * dDdNdH;
+        ahx += a * p.nx;
+        ahy += a * p.ny;
+        ahz += a * p.nz;
+    }
+    adj_NdL += adj_G * p.gv * (p.kk + 0.0001f) * inv_el * inv_el;
+    adj_NdL += adj_den2 * 4.0f * p.ndv;
+    const float hd = t.hx * ahx + t.hy * ahy + t.hz * ahz;
+    float alx = (ahx - t.hx * hd) * t.inv_h + adj_NdL * p.nx;
+    float aly = (ahy - t.hy * hd) * t.inv_h + adj_NdL * p.ny;
+    float alz = (ahz - t.hz * hd) * t.inv_h + adj_NdL * p.nz;
+    const float adj_att0 = adj_atten * t.fall * t.s;
+    const float adj_fall = adj_atten * t.att0 * t.s;
+    const float adj_s = adj_atten * t.att0 * t.fall;
+    const float R = L.p0.w;
+    const float invR = hl_f_rcp(R);
+    float adj_d = adj_att0 * (-2.0f * t.d * t.att0 * t.att0);
+    if (t.fall > 0.0f) {
+        const float dr = t.d * invR;
+        adj_d += adj_fall * (-4.0f * dr * dr * dr * invR);
+    }
+    if (L.p2.w != kNoCone && t.dsdcos != 0.0f) {
+        const float a = adj_s * t.dsdcos;
+        alx -= a * L.p2.x;
+        aly -= a * L.p2.y;
+        alz -= a * L.p2.z;
+    }
+    const float ld = t.lx * alx + t.ly * aly + t.lz * alz;
+    g.x = (alx - t.lx * ld) * t.inv_d + adj_d * t.lx;
+    g.y = (aly - t.ly * ld) * t.inv_d + adj_d * t.ly;
+    g.z = (alz - t.lz * ld) * t.inv_d + adj_d * t.lz;
+}
+
+// Walks the lights of every distinct cell among the wave's valid pixels:
+// per cell, refines its list against the AABB of those pixels, compacts the
+// survivors into LDS and calls f(light, index, mine) for each, uniformly
+// across the wave.
+template <int GRID, class F>
+__device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, uint32_t* s_i,
+                                            const Pixel& p, bool valid, uint32_t cell, F&& f) {
+    const uint32_t lane = hl_lane();
+    uint64_t todo = hl_ballot(valid);
+    while (todo) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+        const uint32_t ucell = hl_readlane(cell, leader);
+        const bool mine = valid && cell == ucell;
+        todo &= ~hl_ballot(mine);
+        const float inf = __builtin_inff();
+        const float bx0 = dpp_min(mine ? p.wx : inf), bx1 = dpp_max(mine ? p.wx : -inf);
+        const float by0 = dpp_min(mine ? p.wy : inf), by1 = dpp_max(mine ? p.wy : -inf);
+        const float bz0 = dpp_min(mine ? p.wz : inf), bz1 = dpp_max(mine ? p.wz : -inf);
+        uint32_t cnt = hl_uniform(a.counts[ucell]);
+        uint32_t base = GRID == HL_GRID_TILES ? ucell * HL_MAX_LIGHTS_PER_TILE
+                                              : hl_uniform(a.offsets[ucell]);
+        if (base >= a.lists_capacity) cnt = 0;
+        else if (cnt > a.lists_capacity - base) cnt = a.lists_capacity - base;
+        const uint32_t* list = a.lists + base;
+        for (uint32_t j0 = 0; j0 < cnt; j0 += 64u) {
+            const uint32_t j = j0 + lane;
+            bool pass = false;
+            uint32_t idx = 0;
+            if (j < cnt) {
+                idx = list[j];
+                if (idx < a.num_lights) {
+                    const float4 q = a.lights[idx].p0;
+                    const float dx = fmaxf(fmaxf(bx0 - q.x, q.x - bx1), 0.0f);
+                    const float dy = fmaxf(fmaxf(by0 - q.y, q.y - by1), 0.0f);
+                    const float dz = fmaxf(fmaxf(bz0 - q.z, q.z - bz1), 0.0f);
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    pass = d2 <= q.w * q.w * 1.0001f + 1e-12f;
+                }
+            }
+            const uint64_t m = hl_ballot(pass);
+            if (pass) {
+                const uint32_t pos = hl_prefix(m);
+                s_l[pos] = a.lights[idx];
+                s_i[pos] = idx;
+            }
+            const uint32_t n = (uint32_t)__popcll(m);
+            hl_wave_sync();
+            for (uint32_t i = 0; i < n; ++i) {
+                const HotLight L = s_l[i];
+                const uint32_t li = s_i[i];
+                f(L, li, mine);
+            }
+            hl_wave_sync();
+        }
+    }
+}
+
 // ------------------------------------------------------------------ kernel
 template <int FA, int FN, int GRID, int MODE>
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    __shared__ uint32_t s_list[4][kLdsCap];
+    __shared__ HotLight s_light[4][kStage];
+    __shared__ uint32_t s_index[4][kStage];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t b = hl_xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t bx = b % a.block_tiles_x, by = b / a.block_tiles_x;
@@ -168,31 +347,54 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     const bool inside = x < a.width && y < a.height;
     const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
     const bool valid = inside && depth < 1.0f;
+    constexpr bool kAdj = MODE == kModeBackward || MODE == kModeFusedL2;
     constexpr bool kWrites = MODE == kModeForward || MODE == kModeFusedL2;
-    float4* out_px = (kWrites && inside) ? (float4*)(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x : nullptr;
-    if (kWrites && inside && !valid && (a.flags & HL_SHADE_CLEAR_SKY))
-        hl_store_nt(out_px, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    float4* out_px = (kWrites && inside)
+                         ? (float4*)(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x
+                         : nullptr;
+    const size_t plane = (size_t)a.width * a.height;
+    const size_t pix = (size_t)y * a.width + x;
+    if (inside && !valid) {
+        if (kWrites && (a.flags & HL_SHADE_CLEAR_SKY))
+            hl_store_nt(out_px, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        if (kAdj)
+            for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) a.d_pixel[k * plane + pix] = 0.0f;
+    }
     if (hl_ballot(valid) == 0) return;
 
     // ---- per-pixel set-up (deferred_lighting.wgsl:999-1086)
     Pixel p;
-    float4 A = make_float4(0, 0, 0, 0), Nr = A, O = A, E = A;
+    float4 A = make_float4(0, 0, 0, 0), Nr = make_float4(0, 1, 0, 0), O = A, E = A;
     if (valid) {
         A = hl_load_rgba<FA>(a.albedo, x, y);
         Nr = hl_load_rgba<FN>(a.normal, x, y);
         O = hl_load_rgba<FA>(a.orm, x, y);
         E = hl_load_rgba<FN>(a.emissive, x, y);
-    } else {
-        Nr = make_float4(0, 1, 0, 0);
     }
+    const GpuCameraUniforms* cam = a.cam;
     {
         // N, V and NdV are per-pixel and exact: NdV cancels at grazing view
         // (dot of two unit vectors ~1e-6), where a last-bit difference in the
         // normalization becomes a percent-level difference in G1(NdV).
+#pragma clang fp contract(off)
         const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
         p.nx = n.x;
         p.ny = n.y;
         p.nz = n.z;
+        const float u = ((float)x + 0.5f) / (float)a.width;
+        const float v = ((float)y + 0.5f) / (float)a.height;
+        const float ndx = u * 2.0f - 1.0f, ndy = 1.0f - v * 2.0f;
+        const float* m = cam->inv_view_proj;
+        const float hw = hl_x_mat_row(m, 3, ndx, ndy, depth, 1.0f);
+        p.wx = hl_x_mat_row(m, 0, ndx, ndy, depth, 1.0f) / hw;
+        p.wy = hl_x_mat_row(m, 1, ndx, ndy, depth, 1.0f) / hw;
+        p.wz = hl_x_mat_row(m, 2, ndx, ndy, depth, 1.0f) / hw;
+        const float3 vv = hl_x_normalize(cam->position_near[0] - p.wx, cam->position_near[1] - p.wy,
+                                         cam->position_near[2] - p.wz);
+        p.vx = vv.x;
+        p.vy = vv.y;
+        p.vz = vv.z;
+        p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
     }
     p.alb[0] = A.x;
     p.alb[1] = A.y;
@@ -201,53 +403,34 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     p.rough = O.y;
     p.metal = O.z;
     p.one_m = 1.0f - p.metal;
-    const GpuCameraUniforms* cam = a.cam;
-    {
-#pragma clang fp contract(off)
-        const float u = ((float)x + 0.5f) / (float)a.width;
-        const float v = ((float)y + 0.5f) / (float)a.height;
-        const float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
-        const float* m = cam->inv_view_proj;
-        const float hw = hl_x_mat_row(m, 3, nx, ny, depth, 1.0f);
-        p.wx = hl_x_mat_row(m, 0, nx, ny, depth, 1.0f) / hw;
-        p.wy = hl_x_mat_row(m, 1, nx, ny, depth, 1.0f) / hw;
-        p.wz = hl_x_mat_row(m, 2, nx, ny, depth, 1.0f) / hw;
-    }
-    if (MODE == kModeBackward || MODE == kModeFusedL2) {
+    if (kAdj) {
         // the differentiated forward resolves F0 from albedo and metallic
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            p.f0[c] = fminf(fmaxf(0.04f * (1.0f - p.metal) + p.alb[c] * p.metal, 0.0f), 0.999f);
+        for (int c = 0; c < 3; ++c) {
+            const float v = 0.04f * (1.0f - p.metal) + p.alb[c] * p.metal;
+            p.f0_free[c] = v > 0.0f && v < 0.999f;
+            p.f0[c] = fminf(fmaxf(v, 0.0f), 0.999f);
+        }
     } else {
         p.f0[0] = fminf(fmaxf(Nr.w, 0.0f), 0.999f);
         p.f0[1] = fminf(fmaxf(O.w, 0.0f), 0.999f);
         p.f0[2] = fminf(fmaxf(E.w, 0.0f), 0.999f);
     }
     {
-#pragma clang fp contract(off)
-        const float3 v = hl_x_normalize(cam->position_near[0] - p.wx,
-                                        cam->position_near[1] - p.wy,
-                                        cam->position_near[2] - p.wz);
-        p.vx = v.x;
-        p.vy = v.y;
-        p.vz = v.z;
-        p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
-    }
-    {
         const float r1 = p.rough + 1.0f;
         p.kk = r1 * r1 * 0.125f;
-        p.gv = p.ndv * hl_f_rcp(p.ndv * (1.0f - p.kk) + p.kk + 0.0001f);
+        p.ev = p.ndv * (1.0f - p.kk) + p.kk + 0.0001f;
+        p.gv = p.ndv * hl_f_rcp(p.ev);
     }
 
     // ---- debug views (deferred_lighting.wgsl:1032-1038)
     if (MODE == kModeForward && a.debug_mode != 0u) {
         if (valid) {
             float4 o;
-            if (a.debug_mode == 5u) {
+            if (a.debug_mode == 5u)
                 o = make_float4(p.nx * 0.5f + 0.5f, p.ny * 0.5f + 0.5f, p.nz * 0.5f + 0.5f, 1.0f);
-            } else {
-                o = make_float4(A.x, A.y, A.z, A.w);
-            }
+            else
+                o = A;
             hl_store_nt(out_px, o);
         }
         return;
@@ -264,7 +447,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             // guess from the view distance, then settle on the exact thresholds
             const float p22 = cam->proj[10], p32 = cam->proj[14];
             const float dist = p32 * hl_f_rcp(depth + p22);
-            float g = __builtin_amdgcn_logf(fmaxf(dist, 1e-30f)) * a.log_scale + a.log_bias;
+            const float g = __builtin_amdgcn_logf(fmaxf(dist, 1e-30f)) * a.log_scale + a.log_bias;
             int k = (int)floorf(g);
             k = k < 0 ? 0 : (k > (int)a.grid_z - 1 ? (int)a.grid_z - 1 : k);
             while (k > 0 && a.depth_split[k] > depth) --k;
@@ -274,93 +457,107 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         cell = cx + a.grid_x * (cy + a.grid_y * cz);
     }
 
+    HotLight* s_l = s_light[wave];
+    uint32_t* s_i = s_index[wave];
     float lo[3] = {0.0f, 0.0f, 0.0f};
-    uint32_t pairs = 0;
-
-    uint64_t todo = hl_ballot(valid);
-    while (todo) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
-        const uint32_t ucell = hl_readlane(cell, leader);
-        const bool mine = valid && cell == ucell;
-        todo &= ~hl_ballot(mine);
-        // AABB of this cell's pixels in the wave
-        const float inf = __builtin_inff();
-        const float bx0 = hl_wave_min(mine ? p.wx : inf), bx1 = hl_wave_max(mine ? p.wx : -inf);
-        const float by0 = hl_wave_min(mine ? p.wy : inf), by1 = hl_wave_max(mine ? p.wy : -inf);
-        const float bz0 = hl_wave_min(mine ? p.wz : inf), bz1 = hl_wave_max(mine ? p.wz : -inf);
-        // list of the cell
-        uint32_t cnt, base;
-        if (GRID == HL_GRID_TILES) {
-            cnt = a.counts[ucell];
-            base = ucell * HL_MAX_LIGHTS_PER_TILE;
-        } else {
-            cnt = a.counts[ucell];
-            base = a.offsets[ucell];
-        }
-        cnt = hl_uniform(cnt);
-        base = hl_uniform(base);
-        if (base >= a.lists_capacity) cnt = 0;
-        else if (cnt > a.lists_capacity - base) cnt = a.lists_capacity - base;
-        const uint32_t* list = a.lists + base;
-
-        uint32_t nl = 0;
-        for (uint32_t j0 = 0; j0 < cnt; j0 += 64u) {
-            const uint32_t j = j0 + lane;
-            bool pass = false;
-            uint32_t idx = 0;
-            if (j < cnt) {
-                idx = list[j];
-                if (idx < a.num_lights) {
-                    const float4 q = a.lights[idx].p0;
-                    float dx = fmaxf(fmaxf(bx0 - q.x, q.x - bx1), 0.0f);
-                    float dy = fmaxf(fmaxf(by0 - q.y, q.y - by1), 0.0f);
-                    float dz = fmaxf(fmaxf(bz0 - q.z, q.z - bz1), 0.0f);
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    pass = d2 <= q.w * q.w * 1.0001f + 1e-12f;
-                }
-            }
-            const uint64_t m = hl_ballot(pass);
-            if (pass) s_list[wave][nl + hl_prefix(m)] = idx;
-            nl += (uint32_t)__popcll(m);
-            if (nl > (uint32_t)kLdsCap - 64u || j0 + 64u >= cnt) {
-                hl_wave_sync();
-                for (uint32_t i = 0; i < nl; ++i) {
-                    const uint32_t li = hl_uniform(s_list[wave][i]);
-                    const HotLight L = a.lights[li];
-                    if (mine) {
-                        Term t;
-                        bool in_range;
-                        const bool act = eval_term<false>(L, p, t, in_range);
-                        if (MODE == kModeCount) pairs += in_range ? 1u : 0u;
-                        if (act) {
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) lo[c] += t.B[c] * t.rad[c] * t.ndl;
-                        }
-                    }
-                }
-                hl_wave_sync();
-                nl = 0;
-            }
-        }
-    }
 
     if (MODE == kModeCount) {
-        uint32_t tot = (uint32_t)hl_wave_sum((float)pairs);
-        // exact for < 2^24 per wave (at most 64 pixels * lights per cell)
+        uint32_t pairs = 0;
+        walk_lights<GRID>(a, s_l, s_i, p, valid, cell,
+                          [&](const HotLight& L, uint32_t, bool mine) {
+                              if (!mine || __builtin_isinf(L.p0.w)) {
+                                  pairs += mine ? 1u : 0u;
+                                  return;
+                              }
+                              const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
+                              const float d2 = tx * tx + ty * ty + tz * tz;
+                              const float r2 = L.p0.w * L.p0.w;
+                              bool in;
+                              if (d2 < r2 * (1.0f - 1e-5f)) in = true;
+                              else if (d2 > r2 * (1.0f + 1e-5f)) in = false;
+                              else in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > L.p0.w);
+                              pairs += in ? 1u : 0u;
+                          });
+        // exact: at most 64 pixels x lights per cell per wave, far below 2^24
+        const uint32_t tot = (uint32_t)dpp_sum((float)pairs);
         if (lane == 0 && tot) atomicAdd(a.pair_counter, (unsigned long long)tot);
         return;
     }
-    if (MODE == kModeForward && valid) {
-        // ambient + combine (deferred_lighting.wgsl:1196-1231)
-        const float ht = p.ny * 0.5f + 0.5f;
-        float o[3];
+
+    if (MODE != kModeBackward) {
+        walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
+            if (!mine) return;
+            Term t;
+            bool in_range;
+            if (eval_term<false>(L, p, t, in_range)) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float sky = a.sky[c];
-            const float hemi = (sky * 0.15f * (1.0f - ht) + sky * ht) * p.alb[c];
-            o[c] = lo[c] + hemi * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
+                for (int c = 0; c < 3; ++c) lo[c] += t.B[c] * t.rad[c] * t.ndl;
+            }
+        });
+    }
+    // ambient + combine (deferred_lighting.wgsl:1196-1231)
+    const float ht = p.ny * 0.5f + 0.5f;
+    float mixv[3], o[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float sky = a.sky[c];
+        mixv[c] = sky * 0.15f * (1.0f - ht) + sky * ht;
+        o[c] = lo[c] + mixv[c] * p.alb[c] * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
+    }
+    if (MODE == kModeForward) {
+        if (valid) hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+        return;
+    }
+
+    // ---- adjoint
+    float gl[3] = {0.0f, 0.0f, 0.0f};
+    if (MODE == kModeFusedL2) {
+        float l2 = 0.0f;
+        if (valid) {
+            hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+            const float4 tg = *((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
+            const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
+            gl[0] = 2.0f * e0;
+            gl[1] = 2.0f * e1;
+            gl[2] = 2.0f * e2;
+            l2 = e0 * e0 + e1 * e1 + e2 * e2;
         }
-        hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+        const float wl = dpp_sum(l2);
+        if (lane == 0) atomicAdd(a.loss, (double)wl);
+    } else if (valid) {
+        const float4 g = *((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
+        gl[0] = g.x;
+        gl[1] = g.y;
+        gl[2] = g.z;
+    }
+    float dA[3] = {0.0f, 0.0f, 0.0f}, dR = 0.0f, dM = 0.0f;
+    walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
+        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool act = false;
+        if (mine) {
+            Term t;
+            bool in_range;
+            act = eval_term<true>(L, p, t, in_range);
+            if (act) adjoint_term(L, p, t, gl, dA, dR, dM, g);
+        }
+        if (hl_ballot(act) == 0) return;
+        const float sx = dpp_sum(g.x), sy = dpp_sum(g.y), sz = dpp_sum(g.z), si = dpp_sum(g.w);
+        if (lane == 0) {
+            double* dl = a.d_light + 4 * (size_t)li;
+            atomicAdd(dl + 0, (double)sx);
+            atomicAdd(dl + 1, (double)sy);
+            atomicAdd(dl + 2, (double)sz);
+            atomicAdd(dl + 3, (double)si);
+        }
+    });
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dA[c] += gl[c] * mixv[c] * p.ao;
+        a.d_pixel[0 * plane + pix] = dA[0];
+        a.d_pixel[1 * plane + pix] = dA[1];
+        a.d_pixel[2 * plane + pix] = dA[2];
+        a.d_pixel[3 * plane + pix] = dR;
+        a.d_pixel[4 * plane + pix] = dM;
     }
 }
 
@@ -388,8 +585,46 @@ hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& 
     switch (mode) {
         case kModeForward: return launch_fmt<kModeForward>(s, gbuf_set, a);
         case kModeCount: return launch_fmt<kModeCount>(s, gbuf_set, a);
+        case kModeBackward: return launch_fmt<kModeBackward>(s, gbuf_set, a);
+        case kModeFusedL2: return launch_fmt<kModeFusedL2>(s, gbuf_set, a);
         default: return hipErrorInvalidValue;
     }
+}
+
+// ------------------------------------------------------ light gradient I/O
+__global__ void k_zero_f64(double* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0.0;
+}
+
+// FP64 accumulators (4 per light of the launch) -> FP32 outputs indexed by the
+// light's pool index (subset[i], or i).
+__global__ void k_light_grads_to_f32(const double* __restrict__ acc, uint32_t n,
+                                     const uint32_t* __restrict__ subset, float* __restrict__ d_pos,
+                                     float* __restrict__ d_int) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = subset ? subset[i] : i;
+    d_pos[3 * (size_t)o + 0] = (float)acc[4 * (size_t)i + 0];
+    d_pos[3 * (size_t)o + 1] = (float)acc[4 * (size_t)i + 1];
+    d_pos[3 * (size_t)o + 2] = (float)acc[4 * (size_t)i + 2];
+    d_int[o] = (float)acc[4 * (size_t)i + 3];
+}
+
+hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    k_zero_f64<<<(unsigned)blocks, 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_light_grads_to_f32(hipStream_t s, const double* acc, uint32_t n,
+                                     const uint32_t* subset, float* d_pos, float* d_int) {
+    if (n == 0) return hipSuccess;
+    k_light_grads_to_f32<<<(n + 255) / 256, 256, 0, s>>>(acc, n, subset, d_pos, d_int);
+    return hipGetLastError();
 }
 
 }  // namespace hl
