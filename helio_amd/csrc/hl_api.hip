@@ -22,7 +22,7 @@ struct hl_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf hot, cull, col_list, col_count, raw, words, accum;
+    DevBuf hot, cull, geom, bits, summary, raw, words, accum;
     // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
 };
 
@@ -55,7 +55,9 @@ hl_status hip_fail(hl_context* c, hipError_t e, const char* what) {
         if (_e != hipSuccess) return hip_fail((c), _e, #expr); \
     } while (0)
 
-hl_status ensure(hl_context* c, DevBuf& b, size_t bytes) {
+// Grows a scratch buffer (never shrinks).  zero: a new allocation is cleared
+// (for buffers the kernels keep zero between calls).
+hl_status ensure(hl_context* c, DevBuf& b, size_t bytes, bool zero = false) {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return HL_OK;
     HL_HIP(c, hipSetDevice(c->device));
@@ -73,6 +75,10 @@ hl_status ensure(hl_context* c, DevBuf& b, size_t bytes) {
                     hipGetErrorString(e));
     }
     b.bytes = want;
+    if (zero) {
+        HL_HIP(c, hipMemsetAsync(b.p, 0, want, c->stream));
+        HL_HIP(c, hipStreamSynchronize(c->stream));
+    }
     return HL_OK;
 }
 
@@ -139,8 +145,9 @@ hl_status check_cluster_params(hl_context* c, const hl_cluster_params* p) {
         return fail(c, HL_EINVAL, "cluster cell or screen size is zero");
     if ((uint64_t)p->grid_x * p->cell_w < p->width || (uint64_t)p->grid_y * p->cell_h < p->height)
         return fail(c, HL_EINVAL, "cluster grid does not cover the screen");
-    if ((uint64_t)p->grid_x * p->grid_y * p->grid_z > (1ull << 26))
-        return fail(c, HL_EINVAL, "cluster grid too large");
+    if ((uint64_t)p->grid_x * p->grid_y * p->grid_z > (1ull << 26) ||
+        (uint64_t)p->grid_x * p->grid_y > 65535u)
+        return fail(c, HL_EINVAL, "cluster grid too large (at most 65535 columns, 2^26 cells)");
     return HL_OK;
 }
 
@@ -294,7 +301,7 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
     if (!c) return HL_EINVAL;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->hot, &c->cull, &c->col_list, &c->col_count, &c->raw, &c->words, &c->accum})
+    for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->words, &c->accum})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return HL_OK;
@@ -431,9 +438,10 @@ HL_API hl_status hl_cluster_params_init(const GpuCameraUniforms* cam, uint32_t w
     // View-distance range of each slice, padded for the FP32 reconstruction
     // error of a world position from its depth: w = m23 * depth + m33 with
     // |m33| = 1/near cancels to 1/distance, so w carries an absolute error of
-    // ~1.2e-7/near, a relative error of ~1.2e-7 * distance / near (x5 margin
-    // below).  The radius padding bounds the same error as a world-space
-    // distance (relative error times the point's distance from the origin).
+    // ~1.2e-7/near, a relative error of ~1.2e-7 * distance / near (x2.5 margin
+    // below, plus 1e-5 for the other rounding steps).  The radius padding
+    // bounds the same error as a world-space distance (twice the relative
+    // error times the point's distance from the origin, plus 1 mm).
     const double cam_dist = sqrt((double)cam->position_near[0] * cam->position_near[0] +
                                  (double)cam->position_near[1] * cam->position_near[1] +
                                  (double)cam->position_near[2] * cam->position_near[2]);
@@ -442,12 +450,12 @@ HL_API hl_status hl_cluster_params_init(const GpuCameraUniforms* cam, uint32_t w
         const double d0 = out->depth_split[k], d1 = out->depth_split[k + 1];
         double z0 = k == 0 ? near_ : p32 / (d0 + p22);
         double z1 = k + 1 == gz ? far_ : p32 / (d1 + p22);
-        const double rel = 2e-3 + 6e-7 * z1 / near_;
+        const double rel = 1e-5 + 3e-7 * z1 / near_;
         z0 *= (1.0 - rel);
         z1 *= (1.0 + rel);
         out->slice_near[k] = (float)z0;
         out->slice_far[k] = (float)z1;
-        const double pad = rel * (z1 + cam_dist) + 1e-3;
+        const double pad = 2.0 * rel * (z1 + cam_dist) + 1e-3;
         out->slice_r_pad[k] = (float)pad;
         if (pad > maxpad) maxpad = pad;
     }
@@ -496,8 +504,9 @@ HL_API hl_status hl_light_cull_clusters(hl_context* c, const GpuCameraUniforms* 
     const uint32_t n = num_lights;
     if ((s = ensure(c, c->cull, sizeof(CullLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
     if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
-    if ((s = ensure(c, c->col_list, sizeof(uint32_t) * (size_t)ncol * (n ? n : 1))) != HL_OK) return s;
-    if ((s = ensure(c, c->col_count, sizeof(uint32_t) * ncol)) != HL_OK) return s;
+    if ((s = ensure(c, c->geom, cluster_geom_bytes(ncol, (uint32_t)ncell))) != HL_OK) return s;
+    if ((s = ensure(c, c->bits, 4 * cluster_bits_words(ncell, n), true)) != HL_OK) return s;
+    if ((s = ensure(c, c->summary, 4 * cluster_summary_words(ncell, n), true)) != HL_OK) return s;
     if ((s = ensure(c, c->raw, sizeof(uint32_t) * ncell)) != HL_OK) return s;
     HL_HIP(c, hipSetDevice(c->device));
     uint32_t* flags = overflow_d ? overflow_d : ctx_words(c);
@@ -506,9 +515,9 @@ HL_API hl_status hl_light_cull_clusters(hl_context* c, const GpuCameraUniforms* 
     HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, n, camera_d, (HotLight*)c->hot.p,
                           (CullLight*)c->cull.p, ctx_words(c) + 1));
     ClusterArgs a = make_cluster_args(p, n);
-    HL_HIP(c, launch_cluster_cull(c->stream, a, camera_d, (const CullLight*)c->cull.p,
-                                  (uint32_t*)c->col_list.p, (uint32_t*)c->col_count.p,
-                                  (uint32_t*)c->raw.p, offsets_d, counts_d, pool_d, pool_cap, flags));
+    ClusterScratch w{c->geom.p, (uint32_t*)c->bits.p, (uint32_t*)c->summary.p, (uint32_t*)c->raw.p};
+    HL_HIP(c, launch_cluster_cull(c->stream, a, camera_d, (const CullLight*)c->cull.p, w, offsets_d,
+                                  counts_d, pool_d, pool_cap, flags));
     if (overflow_d) HL_HIP(c, hipMemcpyAsync(ctx_words(c), overflow_d, 4, hipMemcpyDeviceToDevice, c->stream));
     return HL_OK;
 }

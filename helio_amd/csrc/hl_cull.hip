@@ -1,10 +1,21 @@
 // hl_cull.hip -- light preparation and light assignment kernels for gfx950.
 //
-//  k_prep_lights    GpuLight[] -> HotLight[] (shading) + CullLight[] (culling)
-//  k_cull_tiles     reference 16-px tile lists (light_cull.wgsl:123-180), one
-//                   wavefront per tile, ballot + prefix-popcount compaction
-//  k_cluster_*      log-depth froxel lists: per-column pre-cull and per-slice
-//                   counts, one exclusive scan, then the fill pass
+//  k_prep_lights      GpuLight[] -> HotLight[] (shading) + CullLight[] (culling)
+//  k_cull_tiles       reference 16-px tile lists (light_cull.wgsl:123-180), one
+//                     wavefront per tile, ballot + prefix-popcount compaction
+//  k_cluster_geom     per column: NDC rectangle and lateral planes; per cell:
+//                     the view-space AABB
+//  k_cluster_mark     light-parallel: each light bounds the cells it can reach
+//                     (projected bounding box x slice range); a wavefront tests
+//                     those candidates 64 at a time with the exact membership
+//                     rule and sets the member bits in a two-level bitmask
+//                     (bits + a summary bit per non-empty word)
+//  k_cluster_count    per cluster: popcount over its non-empty words
+//  k_cluster_scan     exclusive scan of the capped counts (single block)
+//  k_cluster_fill     per cluster: ascending indices from the bitmask, which it
+//                     clears again for the next call
+// Membership is decided by the same exact FP32 tests whatever the order of
+// evaluation, so the lists equal the per-cluster brute force bit for bit.
 #include "hl_internal.hpp"
 
 namespace hl {
@@ -26,16 +37,18 @@ __global__ __launch_bounds__(256) void k_prep_lights(const GpuLight* __restrict_
     const uint32_t type = l->light_type;
     const float inner = l->inner_angle;
     const float inf = __builtin_inff();
-    HotLight h;
-    float range = type == HL_LIGHT_DIRECTIONAL ? inf : pr.w;
-    h.p0 = make_float4(pr.x, pr.y, pr.z, range);
-    {
+    const float range = type == HL_LIGHT_DIRECTIONAL ? inf : pr.w;
+    if (hot) {
+        HotLight h;
+        h.p0 = make_float4(pr.x, pr.y, pr.z, range);
+        {
 #pragma clang fp contract(off)
-        h.p1 = make_float4(ci.x * ci.w, ci.y * ci.w, ci.z * ci.w, inner);
+            h.p1 = make_float4(ci.x * ci.w, ci.y * ci.w, ci.z * ci.w, inner);
+        }
+        h.p2 = make_float4(dro.x, dro.y, dro.z, type == HL_LIGHT_SPOT ? dro.w : kNoCone);
+        h.p3 = ci;
+        hot[i] = h;
     }
-    h.p2 = make_float4(dro.x, dro.y, dro.z, type == HL_LIGHT_SPOT ? dro.w : kNoCone);
-    h.p3 = ci;
-    hot[i] = h;
     if (cull) {
         const float* v = cam->view;
         CullLight c;
@@ -108,195 +121,328 @@ __global__ __launch_bounds__(256) void k_cull_tiles(const GpuCameraUniforms* __r
 struct ColumnGeom {
     float3 pl[4];
     float l, r, b, t;
+    uint32_t on;
+    uint32_t pad[3];
 };
 
-// The column's screen rectangle in NDC and its lateral planes (same
-// construction as the tile cull).  false when the column lies off screen.
-__device__ __forceinline__ bool column_geom(const GpuCameraUniforms* cam, const ClusterArgs& a,
-                                            uint32_t cx, uint32_t cy, ColumnGeom* g) {
+struct CellBox {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ bool column_rect(const ClusterArgs& a, uint32_t cx, uint32_t cy,
+                                            float* l, float* r, float* b, float* t) {
 #pragma clang fp contract(off)
-    uint32_t px0 = cx * a.cell_w, py0 = cy * a.cell_h;
+    const uint32_t px0 = cx * a.cell_w, py0 = cy * a.cell_h;
     if (px0 >= a.width || py0 >= a.height) return false;
-    uint32_t px1 = min(px0 + a.cell_w, a.width), py1 = min(py0 + a.cell_h, a.height);
+    const uint32_t px1 = min(px0 + a.cell_w, a.width), py1 = min(py0 + a.cell_h, a.height);
     const float fw = (float)a.width, fh = (float)a.height;
-    g->l = ((float)px0 / fw) * 2.0f - 1.0f;
-    g->r = ((float)px1 / fw) * 2.0f - 1.0f;
-    g->t = 1.0f - ((float)py0 / fh) * 2.0f;
-    g->b = 1.0f - ((float)py1 / fh) * 2.0f;
-    const float p00 = cam->proj[0], p11 = cam->proj[5];
-    g->pl[0] = hl_x_plane(p00, p11, g->l, g->b, g->l, g->t);
-    g->pl[1] = hl_x_plane(p00, p11, g->r, g->t, g->r, g->b);
-    g->pl[2] = hl_x_plane(p00, p11, g->l, g->t, g->r, g->t);
-    g->pl[3] = hl_x_plane(p00, p11, g->r, g->b, g->l, g->b);
+    *l = ((float)px0 / fw) * 2.0f - 1.0f;
+    *r = ((float)px1 / fw) * 2.0f - 1.0f;
+    *t = 1.0f - ((float)py0 / fh) * 2.0f;
+    *b = 1.0f - ((float)py1 / fh) * 2.0f;
     return true;
 }
 
-__device__ __forceinline__ void slice_aabb(const GpuCameraUniforms* cam, const ColumnGeom& g,
-                                           float zn, float zf, float3* lo, float3* hi) {
+// One thread per cell: the cell's view-space AABB (the column's NDC corners
+// unprojected at the slice's padded near and far distances); the slice-0
+// thread of each column also writes the column's lateral planes (the tile
+// cull's construction on the column's pixel rectangle).
+__global__ __launch_bounds__(256) void k_cluster_geom(ClusterArgs a,
+                                                      const GpuCameraUniforms* __restrict__ cam,
+                                                      ColumnGeom* __restrict__ geom,
+                                                      CellBox* __restrict__ boxes) {
 #pragma clang fp contract(off)
+    const uint32_t ncol = a.grid_x * a.grid_y;
+    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncol * a.grid_z) return;
+    const uint32_t col = cell % ncol, k = cell / ncol;
+    const uint32_t cx = col % a.grid_x, cy = col / a.grid_x;
+    float l = 0.0f, r = 0.0f, b = 0.0f, t = 0.0f;
+    const bool on = column_rect(a, cx, cy, &l, &r, &b, &t);
     const float p00 = cam->proj[0], p11 = cam->proj[5];
-    float x0 = g.l * zn / p00, x1 = g.r * zn / p00, x2 = g.l * zf / p00, x3 = g.r * zf / p00;
-    float y0 = g.b * zn / p11, y1 = g.t * zn / p11, y2 = g.b * zf / p11, y3 = g.t * zf / p11;
-    lo->x = fminf(fminf(fminf(x0, x1), x2), x3);
-    hi->x = fmaxf(fmaxf(fmaxf(x0, x1), x2), x3);
-    lo->y = fminf(fminf(fminf(y0, y1), y2), y3);
-    hi->y = fmaxf(fmaxf(fmaxf(y0, y1), y2), y3);
-    lo->z = -zf;
-    hi->z = -zn;
+    CellBox bx;
+    if (on) {
+        const float zn = a.slice_near[k], zf = a.slice_far[k];
+        const float x0 = l * zn / p00, x1 = r * zn / p00, x2 = l * zf / p00, x3 = r * zf / p00;
+        const float y0 = b * zn / p11, y1 = t * zn / p11, y2 = b * zf / p11, y3 = t * zf / p11;
+        bx.lo[0] = fminf(fminf(fminf(x0, x1), x2), x3);
+        bx.hi[0] = fmaxf(fmaxf(fmaxf(x0, x1), x2), x3);
+        bx.lo[1] = fminf(fminf(fminf(y0, y1), y2), y3);
+        bx.hi[1] = fmaxf(fmaxf(fmaxf(y0, y1), y2), y3);
+        bx.lo[2] = -zf;
+        bx.hi[2] = -zn;
+    } else {
+        bx.lo[0] = bx.lo[1] = bx.lo[2] = bx.hi[0] = bx.hi[1] = bx.hi[2] = 0.0f;
+    }
+    boxes[cell] = bx;
+    if (k != 0) return;
+    ColumnGeom g;
+    g.on = on ? 1u : 0u;
+    g.pad[0] = g.pad[1] = g.pad[2] = 0;
+    g.l = l;
+    g.r = r;
+    g.b = b;
+    g.t = t;
+    if (on) {
+        g.pl[0] = hl_x_plane(p00, p11, l, b, l, t);
+        g.pl[1] = hl_x_plane(p00, p11, r, t, r, b);
+        g.pl[2] = hl_x_plane(p00, p11, l, t, r, t);
+        g.pl[3] = hl_x_plane(p00, p11, r, b, l, b);
+    } else {
+        for (int i = 0; i < 4; ++i) g.pl[i] = make_float3(0.0f, 0.0f, 0.0f);
+    }
+    geom[col] = g;
 }
 
-__device__ __forceinline__ bool slice_member(const CullLight& c, float pad, float3 lo, float3 hi) {
+// first k in [0, n) with v[k] >= x (v ascending); n if none
+__device__ __forceinline__ int lower_bound(const float* v, int n, float x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Membership of light (c) in cell (col, k): the definition of the clustered
+// lists -- lateral planes of the column with the column padding, the cell's
+// AABB with the slice padding, and the cone for spot lights.
+__device__ __forceinline__ bool cell_member(const ClusterArgs& a, const CullLight& c,
+                                            const ColumnGeom& g, const CellBox& bx, uint32_t k) {
 #pragma clang fp contract(off)
-    if (!hl_x_sphere_aabb(c.vpos, c.vpos.w + pad, lo, hi)) return false;
+    const float pad = a.slice_r_pad[k];
+    const float rc = c.vpos.w + pad;
+    if (!(hl_x_plane_pass(g.pl[0], c.vpos, rc) && hl_x_plane_pass(g.pl[1], c.vpos, rc) &&
+          hl_x_plane_pass(g.pl[2], c.vpos, rc) && hl_x_plane_pass(g.pl[3], c.vpos, rc)))
+        return false;
+    const float3 lo = make_float3(bx.lo[0], bx.lo[1], bx.lo[2]);
+    const float3 hi = make_float3(bx.hi[0], bx.hi[1], bx.hi[2]);
+    if (!hl_x_sphere_aabb(c.vpos, rc, lo, hi)) return false;
     if (c.cone.w != kNoCone && !hl_x_cone_aabb(c.vpos, c.cone, pad, lo, hi)) return false;
     return true;
 }
 
-// Block per column: build the column list (lateral-plane pre-cull, ascending)
-// then count the members of every slice.
-__global__ __launch_bounds__(256) void k_cluster_columns(ClusterArgs a,
-                                                         const GpuCameraUniforms* __restrict__ cam,
-                                                         const CullLight* __restrict__ cl,
-                                                         uint32_t* __restrict__ col_list,
-                                                         uint32_t* __restrict__ col_count,
-                                                         uint32_t* __restrict__ raw) {
-    __shared__ uint32_t s_cnt[4];
-    const uint32_t col = blockIdx.x;
-    const uint32_t ncol = a.grid_x * a.grid_y;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    ColumnGeom g;
-    const bool on = column_geom(cam, a, col % a.grid_x, col / a.grid_x, &g);
-    uint32_t* list = col_list + (size_t)col * a.num_lights;
-    uint32_t total = 0;
-    if (on) {
-        for (uint32_t base = 0; base < a.num_lights; base += 256u) {
-            const uint32_t i = base + threadIdx.x;
-            bool pass = false;
-            if (i < a.num_lights) {
-                const float4 c = cl[i].vpos;
-                const float r = c.w + a.r_pad_column;
-                pass = hl_x_plane_pass(g.pl[0], c, r) && hl_x_plane_pass(g.pl[1], c, r) &&
-                       hl_x_plane_pass(g.pl[2], c, r) && hl_x_plane_pass(g.pl[3], c, r);
-            }
-            const uint64_t m = hl_ballot(pass);
-            if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(m);
-            __syncthreads();
-            uint32_t off = total;
-            for (uint32_t w = 0; w < wave; ++w) off += s_cnt[w];
-            if (pass) list[off + hl_prefix(m)] = i;
-            total += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-            __syncthreads();
+// One wavefront per light.  The light bounds the cells it can reach -- the
+// slices its view distance +- (radius + padding) spans and the columns whose
+// lateral planes it can pass at its own depth, each widened by one -- and the
+// wave tests those candidate cells 64 at a time with the exact membership
+// rule, setting the member bits in the two-level bitmask.
+__global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
+                                                      const GpuCameraUniforms* __restrict__ cam,
+                                                      const CullLight* __restrict__ cl,
+                                                      const ColumnGeom* __restrict__ geom,
+                                                      const CellBox* __restrict__ boxes,
+                                                      uint32_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ summary, uint32_t W,
+                                                      uint32_t SW) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t li = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (li >= a.num_lights) return;
+    const int gx = (int)a.grid_x, gy = (int)a.grid_y, gz = (int)a.grid_z;
+    CullLight L = cl[li];
+    L.vpos = make_float4(hl_uniform(L.vpos.x), hl_uniform(L.vpos.y), hl_uniform(L.vpos.z),
+                         hl_uniform(L.vpos.w));
+    L.cone = make_float4(hl_uniform(L.cone.x), hl_uniform(L.cone.y), hl_uniform(L.cone.z),
+                         hl_uniform(L.cone.w));
+    int x0 = 0, x1 = gx - 1, y0 = 0, y1 = gy - 1, s0 = 0, s1 = gz - 1;
+    if (!__builtin_isinf(L.vpos.w)) {
+        const float zc = -L.vpos.z;
+        // slices under the largest padding, then again under the largest
+        // padding of those slices (paddings grow with distance)
+        float rr = L.vpos.w + a.r_pad_column;
+        s1 = clampi(lower_bound(a.slice_near, gz, zc + rr) + 1, 0, gz - 1);
+        rr = L.vpos.w + a.slice_r_pad[s1];
+        s0 = lower_bound(a.slice_far, gz, zc - rr) - 1;
+        s1 = lower_bound(a.slice_near, gz, zc + rr) + 1;
+        if (zc > 1e-4f) {
+            // A lateral plane through the origin at angle alpha from the view
+            // axis passes a centre up to rr / cos(alpha) beyond the column's
+            // edge at the centre's own depth; 1/cos(alpha) <= sqrt(1 + 1/p00^2).
+            const float p00 = cam->proj[0], p11 = cam->proj[5];
+            const float fx = sqrtf(1.0f + 1.0f / (p00 * p00));
+            const float fy = sqrtf(1.0f + 1.0f / (p11 * p11));
+            const float iz = 1.0f / zc;
+            const float nx0 = p00 * (L.vpos.x - rr * fx) * iz, nx1 = p00 * (L.vpos.x + rr * fx) * iz;
+            const float ny0 = p11 * (L.vpos.y - rr * fy) * iz, ny1 = p11 * (L.vpos.y + rr * fy) * iz;
+            const float fw = (float)a.width, fh = (float)a.height;
+            const float px0 = (nx0 + 1.0f) * 0.5f * fw, px1 = (nx1 + 1.0f) * 0.5f * fw;
+            const float py0 = (1.0f - ny1) * 0.5f * fh, py1 = (1.0f - ny0) * 0.5f * fh;
+            const float cw = (float)a.cell_w, ch = (float)a.cell_h;
+            x0 = (int)fmaxf(floorf(px0 / cw) - 1.0f, -1.0f);
+            x1 = (int)fminf(floorf(px1 / cw) + 1.0f, (float)gx);
+            y0 = (int)fmaxf(floorf(py0 / ch) - 1.0f, -1.0f);
+            y1 = (int)fminf(floorf(py1 / ch) + 1.0f, (float)gy);
         }
     }
-    if (threadIdx.x == 0) col_count[col] = total;
-    __threadfence_block();
-    __syncthreads();
-    for (uint32_t k = wave; k < a.grid_z; k += 4u) {
-        uint32_t hits = 0;
-        if (on) {
-            float3 lo, hi;
-            slice_aabb(cam, g, a.slice_near[k], a.slice_far[k], &lo, &hi);
-            const float pad = a.slice_r_pad[k];
-            for (uint32_t j = 0; j < total; j += 64u) {
-                bool mem = false;
-                if (j + lane < total) mem = slice_member(cl[list[j + lane]], pad, lo, hi);
-                hits += (uint32_t)__popcll(hl_ballot(mem));
-            }
-        }
-        if (lane == 0) raw[col + ncol * k] = hits;
+    x0 = clampi(x0, 0, gx - 1);
+    x1 = clampi(x1, -1, gx - 1);
+    y0 = clampi(y0, 0, gy - 1);
+    y1 = clampi(y1, -1, gy - 1);
+    s0 = clampi(s0, 0, gz - 1);
+    s1 = clampi(s1, -1, gz - 1);
+    if (x1 < x0 || y1 < y0 || s1 < s0) return;
+    const uint32_t nx = (uint32_t)(x1 - x0 + 1), ny = (uint32_t)(y1 - y0 + 1),
+                   nz = (uint32_t)(s1 - s0 + 1);
+    const uint32_t work = nx * ny * nz;
+    const uint32_t ncol = a.grid_x * a.grid_y;
+    const uint32_t bit = 1u << (li & 31u);
+    const uint32_t word = li >> 5;
+    for (uint32_t t0 = 0; t0 < work; t0 += 64u) {
+        const uint32_t t = t0 + lane;
+        if (t >= work) break;
+        const uint32_t kz = t % nz;
+        const uint32_t rest = t / nz;
+        const uint32_t col = ((uint32_t)x0 + rest % nx) + a.grid_x * ((uint32_t)y0 + rest / nx);
+        const uint32_t k = (uint32_t)s0 + kz;
+        const ColumnGeom g = geom[col];
+        if (!g.on) continue;
+        const size_t cell = (size_t)col + (size_t)ncol * k;
+        if (!cell_member(a, L, g, boxes[cell], k)) continue;
+        const uint32_t old = atomicOr(&bits[cell * W + word], bit);
+        if (old == 0u) atomicOr(&summary[cell * SW + (word >> 5)], 1u << (word & 31u));
     }
 }
 
-// Single block: capped counts -> offsets (exclusive scan), pool truncation, flags.
+// one wavefront per cluster: number of members (before the cap)
+__global__ __launch_bounds__(256) void k_cluster_count(uint32_t ncell, uint32_t W, uint32_t SW,
+                                                       const uint32_t* __restrict__ bits,
+                                                       const uint32_t* __restrict__ summary,
+                                                       uint32_t* __restrict__ raw) {
+    const uint32_t cell = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (cell >= ncell) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t total = 0;
+    for (uint32_t s0 = 0; s0 < SW; s0 += 64u) {
+        const uint32_t j = s0 + lane;
+        uint32_t sw = j < SW ? summary[(size_t)cell * SW + j] : 0u;
+        while (sw) {
+            const uint32_t b = (uint32_t)__builtin_ctz(sw);
+            sw &= sw - 1u;
+            total += (uint32_t)__popc(bits[(size_t)cell * W + 32u * j + b]);
+        }
+    }
+    const uint32_t t = (uint32_t)dpp_sum((float)total);  // exact below 2^24 lights
+    if (lane == 0) raw[cell] = t;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = hl_lane();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+// Single block of 1024: capped counts -> offsets (exclusive scan in cluster
+// order), pool truncation, overflow flags.  Tiles of 4096 counts, read
+// coalesced, four per thread.
 __global__ __launch_bounds__(1024) void k_cluster_scan(const uint32_t* __restrict__ raw,
                                                        uint32_t ncell, uint32_t cap,
                                                        uint32_t pool_cap,
                                                        uint32_t* __restrict__ offsets,
                                                        uint32_t* __restrict__ counts,
                                                        uint32_t* __restrict__ flags_out) {
-    __shared__ unsigned long long s_sum[1024];
+    __shared__ unsigned long long s_wave[16];
+    __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_flags;
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (ncell + 1023u) / 1024u;
-    const uint32_t c0 = t * per, c1 = min(c0 + per, ncell);
-    if (t == 0) s_flags = 0;
-    unsigned long long local = 0;
-    uint32_t fl = 0;
-    for (uint32_t c = c0; c < c1; ++c) {
-        uint32_t n = raw[c];
-        if (cap && n > cap) {
-            n = cap;
-            fl |= HL_OVERFLOW_CAP;
-        }
-        local += n;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    if (t == 0) {
+        s_carry = 0;
+        s_flags = 0;
     }
-    s_sum[t] = local;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums
-    for (uint32_t o = 1; o < 1024u; o <<= 1) {
-        unsigned long long v = t >= o ? s_sum[t - o] : 0ull;
-        __syncthreads();
-        s_sum[t] += v;
-        __syncthreads();
-    }
-    unsigned long long off = s_sum[t] - local;
-    for (uint32_t c = c0; c < c1; ++c) {
-        uint32_t n = raw[c];
-        if (cap && n > cap) n = cap;
-        uint32_t o = off > pool_cap ? pool_cap : (uint32_t)off;
-        uint32_t room = pool_cap - o;
-        uint32_t nn = n;
-        if (nn > room) {
-            nn = room;
-            fl |= HL_OVERFLOW_POOL;
+    uint32_t fl = 0;
+    for (uint32_t start = 0; start < ncell; start += 4096u) {
+        uint32_t v[4];
+        uint32_t local = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t c = start + 4u * t + (uint32_t)q;
+            uint32_t n = c < ncell ? raw[c] : 0u;
+            if (cap && n > cap) {
+                n = cap;
+                fl |= HL_OVERFLOW_CAP;
+            }
+            v[q] = n;
+            local += n;
         }
-        offsets[c] = o;
-        counts[c] = nn;
-        off += n;
+        // local <= 4 * 2^24: the wave scan stays in 32 bits; tile sums go to 64
+        const uint32_t incl = wave_incl_scan(local);
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        unsigned long long before = s_carry;
+        for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+        unsigned long long off = before + (incl - local);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t c = start + 4u * t + (uint32_t)q;
+            if (c < ncell) {
+                const uint32_t o = off > pool_cap ? pool_cap : (uint32_t)off;
+                const uint32_t room = pool_cap - o;
+                uint32_t n = v[q];
+                if (n > room) {
+                    n = room;
+                    fl |= HL_OVERFLOW_POOL;
+                }
+                offsets[c] = o;
+                counts[c] = n;
+            }
+            off += v[q];
+        }
+        __syncthreads();
+        if (t == 0) {
+            unsigned long long tot = 0;
+            for (uint32_t w = 0; w < 16u; ++w) tot += s_wave[w];
+            s_carry += tot;
+        }
+        __syncthreads();
     }
     if (fl) atomicOr(&s_flags, fl);
     __syncthreads();
     if (t == 0) flags_out[0] |= s_flags;
 }
 
-// Block per column, wave per slice: write the members of each slice, ascending,
-// stopping at the (capped, pool-truncated) count.
-__global__ __launch_bounds__(256) void k_cluster_fill(ClusterArgs a,
-                                                      const GpuCameraUniforms* __restrict__ cam,
-                                                      const CullLight* __restrict__ cl,
-                                                      const uint32_t* __restrict__ col_list,
-                                                      const uint32_t* __restrict__ col_count,
+// one wavefront per cluster: write the members in ascending order up to the
+// (capped, pool-truncated) count, and clear the cluster's bitmask words.
+__global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t W, uint32_t SW,
+                                                      uint32_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ summary,
                                                       const uint32_t* __restrict__ offsets,
                                                       const uint32_t* __restrict__ counts,
                                                       uint32_t* __restrict__ pool) {
-    const uint32_t col = blockIdx.x;
-    const uint32_t ncol = a.grid_x * a.grid_y;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    ColumnGeom g;
-    if (!column_geom(cam, a, col % a.grid_x, col / a.grid_x, &g)) return;
-    const uint32_t* list = col_list + (size_t)col * a.num_lights;
-    const uint32_t total = col_count[col];
-    for (uint32_t k = wave; k < a.grid_z; k += 4u) {
-        const uint32_t c = col + ncol * k;
-        const uint32_t cnt = counts[c];
-        if (cnt == 0) continue;
-        uint32_t* dst = pool + offsets[c];
-        float3 lo, hi;
-        slice_aabb(cam, g, a.slice_near[k], a.slice_far[k], &lo, &hi);
-        const float pad = a.slice_r_pad[k];
-        uint32_t written = 0;
-        for (uint32_t j = 0; j < total && written < cnt; j += 64u) {
-            bool mem = false;
-            uint32_t idx = 0;
-            if (j + lane < total) {
-                idx = list[j + lane];
-                mem = slice_member(cl[idx], pad, lo, hi);
+    const uint32_t cell = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (cell >= ncell) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t cnt = counts[cell];
+    uint32_t* dst = pool + offsets[cell];
+    uint32_t* cb = bits + (size_t)cell * W;
+    uint32_t written = 0;
+    for (uint32_t s0 = 0; s0 < SW; s0 += 64u) {
+        const uint32_t j = s0 + lane;
+        const uint32_t sw0 = j < SW ? summary[(size_t)cell * SW + j] : 0u;
+        uint32_t lt = 0;
+        for (uint32_t sw = sw0; sw; sw &= sw - 1u)
+            lt += (uint32_t)__popc(cb[32u * j + (uint32_t)__builtin_ctz(sw)]);
+        const uint32_t incl = wave_incl_scan(lt);
+        uint32_t pos = written + incl - lt;
+        for (uint32_t sw = sw0; sw; sw &= sw - 1u) {
+            const uint32_t w = 32u * j + (uint32_t)__builtin_ctz(sw);
+            uint32_t word = cb[w];
+            cb[w] = 0u;
+            while (word) {
+                const uint32_t b = (uint32_t)__builtin_ctz(word);
+                word &= word - 1u;
+                if (pos < cnt) dst[pos] = 32u * w + b;
+                ++pos;
             }
-            const uint64_t m = hl_ballot(mem);
-            const uint32_t pos = written + hl_prefix(m);
-            if (mem && pos < cnt) dst[pos] = idx;
-            written += (uint32_t)__popcll(m);
         }
+        if (sw0) summary[(size_t)cell * SW + j] = 0u;
+        written += hl_readlane(incl, 63);
     }
 }
 
@@ -331,19 +477,37 @@ hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const 
     return hipGetLastError();
 }
 
+// geometry scratch: ColumnGeom per column, then (256-B aligned) CellBox per cell
+size_t cluster_geom_offset_boxes(uint32_t ncol) {
+    return (sizeof(ColumnGeom) * (size_t)ncol + 255) & ~(size_t)255;
+}
+size_t cluster_geom_bytes(uint32_t ncol, uint32_t ncell) {
+    return cluster_geom_offset_boxes(ncol) + sizeof(CellBox) * (size_t)ncell;
+}
+
 hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCameraUniforms* cam,
-                               const CullLight* cl, uint32_t* col_list, uint32_t* col_count,
-                               uint32_t* raw, uint32_t* offsets, uint32_t* counts, uint32_t* pool,
-                               uint32_t pool_cap, uint32_t* flags) {
+                               const CullLight* cl, const ClusterScratch& w, uint32_t* offsets,
+                               uint32_t* counts, uint32_t* pool, uint32_t pool_cap,
+                               uint32_t* flags) {
     const uint32_t ncol = a.grid_x * a.grid_y;
     const uint32_t ncell = ncol * a.grid_z;
-    k_cluster_columns<<<ncol, 256, 0, s>>>(a, cam, cl, col_list, col_count, raw);
+    const uint32_t W = (a.num_lights + 31u) / 32u, SW = (W + 31u) / 32u;
+    ColumnGeom* geom = (ColumnGeom*)w.geom;
+    CellBox* boxes = (CellBox*)((uint8_t*)w.geom + cluster_geom_offset_boxes(ncol));
+    k_cluster_geom<<<(ncell + 255) / 256, 256, 0, s>>>(a, cam, geom, boxes);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_cluster_scan<<<1, 1024, 0, s>>>(raw, ncell, a.cap, pool_cap, offsets, counts, flags);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_cluster_fill<<<ncol, 256, 0, s>>>(a, cam, cl, col_list, col_count, offsets, counts, pool);
+    if (a.num_lights) {
+        k_cluster_mark<<<(a.num_lights + 3) / 4, 256, 0, s>>>(a, cam, cl, geom, boxes, w.bits,
+                                                                  w.summary, W, SW);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const uint32_t blocks = (ncell + 3) / 4;
+    k_cluster_count<<<blocks, 256, 0, s>>>(ncell, W, SW, w.bits, w.summary, w.raw);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_cluster_scan<<<1, 1024, 0, s>>>(w.raw, ncell, a.cap, pool_cap, offsets, counts, flags);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_cluster_fill<<<blocks, 256, 0, s>>>(ncell, W, SW, w.bits, w.summary, offsets, counts, pool);
     return hipGetLastError();
 }
 

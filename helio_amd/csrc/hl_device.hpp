@@ -296,6 +296,46 @@ __device__ __forceinline__ float hl_wave_sum(float v) {
     return v;
 }
 
+// ---------------------------------------------------- DPP wave reductions
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float hl_dpp(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src),
+                                                       CTRL, ROW_MASK, 0xF, false));
+}
+
+// Full-wave reductions: quad swaps, half-row and row mirrors, then the
+// row_bcast:15 / row_bcast:31 steps; lane 63 holds the result, which is read
+// back as a wave-uniform value.  All 64 lanes must be active.
+__device__ __forceinline__ float dpp_min(float v) {
+    const float I = __builtin_inff();
+    v = fminf(v, hl_dpp<0xB1>(I, v));
+    v = fminf(v, hl_dpp<0x4E>(I, v));
+    v = fminf(v, hl_dpp<0x141>(I, v));
+    v = fminf(v, hl_dpp<0x140>(I, v));
+    v = fminf(v, hl_dpp<0x142, 0xA>(I, v));
+    v = fminf(v, hl_dpp<0x143, 0xC>(I, v));
+    return hl_readlane(v, 63);
+}
+__device__ __forceinline__ float dpp_max(float v) {
+    const float I = -__builtin_inff();
+    v = fmaxf(v, hl_dpp<0xB1>(I, v));
+    v = fmaxf(v, hl_dpp<0x4E>(I, v));
+    v = fmaxf(v, hl_dpp<0x141>(I, v));
+    v = fmaxf(v, hl_dpp<0x140>(I, v));
+    v = fmaxf(v, hl_dpp<0x142, 0xA>(I, v));
+    v = fmaxf(v, hl_dpp<0x143, 0xC>(I, v));
+    return hl_readlane(v, 63);
+}
+__device__ __forceinline__ float dpp_sum(float v) {
+    v = v + hl_dpp<0xB1>(0.0f, v);
+    v = v + hl_dpp<0x4E>(0.0f, v);
+    v = v + hl_dpp<0x141>(0.0f, v);
+    v = v + hl_dpp<0x140>(0.0f, v);
+    v = v + hl_dpp<0x142, 0xA>(0.0f, v);
+    v = v + hl_dpp<0x143, 0xC>(0.0f, v);
+    return hl_readlane(v, 63);
+}
+
 // Wave-local ordering point between LDS writes and reads of other lanes.
 __device__ __forceinline__ void hl_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -26,10 +26,27 @@ hipError_t launch_prep(hipStream_t s, const GpuLight* lights, const uint32_t* su
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
 hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const LightCullParams& p,
                              const CullLight* cl, uint32_t* lists, uint32_t* counts);
+// Scratch of the clustered cull.  bits (ncell * ceil(L/32) words) and summary
+// (ncell * ceil(L/1024) words) must be zero before the first call; every call
+// leaves them zero again.
+struct ClusterScratch {
+    void* geom;          // cluster_geom_bytes(columns, cells)
+    uint32_t* bits;
+    uint32_t* summary;
+    uint32_t* raw;       // ncell
+};
+size_t cluster_geom_bytes(uint32_t ncol, uint32_t ncell);
+size_t cluster_geom_offset_boxes(uint32_t ncol);
+inline size_t cluster_bits_words(uint32_t ncell, uint32_t num_lights) {
+    return (size_t)ncell * ((num_lights + 31u) / 32u);
+}
+inline size_t cluster_summary_words(uint32_t ncell, uint32_t num_lights) {
+    return (size_t)ncell * (((num_lights + 31u) / 32u + 31u) / 32u);
+}
 hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCameraUniforms* cam,
-                               const CullLight* cl, uint32_t* col_list, uint32_t* col_count,
-                               uint32_t* raw, uint32_t* offsets, uint32_t* counts, uint32_t* pool,
-                               uint32_t pool_cap, uint32_t* flags);
+                               const CullLight* cl, const ClusterScratch& w, uint32_t* offsets,
+                               uint32_t* counts, uint32_t* pool, uint32_t pool_cap,
+                               uint32_t* flags);
 
 // launchers (hl_shade.hip)
 enum ShadeMode : int { kModeForward = 0, kModeCount = 1, kModeBackward = 2, kModeFusedL2 = 3 };

@@ -314,11 +314,11 @@ static int or_cluster_member(const GpuCameraUniforms* cam, const hl_cluster_para
                              const GpuLight* l, const float vpos[3], const float planes[4][4],
                              const float ndc[4], uint32_t k) {
     if (l->light_type == 0u) return 1;
-    const float r = l->position_range[3];
-    if (!or_sphere_inside(vpos, r + cp->r_pad_column, planes)) return 0;
+    const float r = l->position_range[3] + cp->slice_r_pad[k];
+    if (!or_sphere_inside(vpos, r, planes)) return 0;
     float lo[3], hi[3];
     or_cluster_aabb(cam, cp, ndc, k, lo, hi);
-    if (!or_sphere_aabb(vpos, r + cp->slice_r_pad[k], lo, hi)) return 0;
+    if (!or_sphere_aabb(vpos, r, lo, hi)) return 0;
     if (l->light_type == 2u) {
         float a[3];
         or_light_view_dir(cam, l, a);
