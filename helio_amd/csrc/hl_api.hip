@@ -12,27 +12,12 @@
 
 using namespace hl;
 
-// ------------------------------------------------------------------ context
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-struct hl_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    DevBuf hot, cull, geom, bits, summary, raw, words, accum;
-    // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
-};
-
 struct hl_event {
     hipEvent_t ev;
 };
 
-namespace {
+namespace hl {
 
-hl_status fail(hl_context* c, hl_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 hl_status fail(hl_context* c, hl_status s, const char* fmt, ...) {
     if (c) {
         char buf[512];
@@ -49,15 +34,7 @@ hl_status hip_fail(hl_context* c, hipError_t e, const char* what) {
     return fail(c, HL_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define HL_HIP(c, expr)                                       \
-    do {                                                      \
-        hipError_t _e = (expr);                               \
-        if (_e != hipSuccess) return hip_fail((c), _e, #expr); \
-    } while (0)
-
-// Grows a scratch buffer (never shrinks).  zero: a new allocation is cleared
-// (for buffers the kernels keep zero between calls).
-hl_status ensure(hl_context* c, DevBuf& b, size_t bytes, bool zero = false) {
+hl_status ensure(hl_context* c, DevBuf& b, size_t bytes, bool zero) {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return HL_OK;
     HL_HIP(c, hipSetDevice(c->device));
@@ -172,14 +149,14 @@ ClusterArgs make_cluster_args(const hl_cluster_params* p, uint32_t num_lights) {
 
 uint32_t* ctx_words(hl_context* c) { return (uint32_t*)c->words.p; }
 
-// Shared set-up of the shading entry points.
-hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
-                        const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights,
-                        const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0) {
+// Validation and shading arguments of one view (no device work; a->lights is
+// left for the caller).
+hl_status build_shade_args(hl_context* c, const GpuCameraUniforms* camera_d,
+                           const DeferredGlobals* g, const hl_gbuffer* gb, uint32_t num_lights,
+                           const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0) {
     if (!c) return HL_EINVAL;
     if (!camera_d) return fail(c, HL_EINVAL, "camera is null");
     if (!g) return fail(c, HL_EINVAL, "globals are null");
-    if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
     hl_status s = check_gbuffer(c, gb, set);
     if (s != HL_OK) return s;
     if (!debug_supported(g->debug_mode))
@@ -245,8 +222,22 @@ hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const 
         a->sky[k] = g->ambient_color[k] * g->ambient_intensity;
     }
     a->debug_mode = g->debug_mode;
-    a->block_tiles_x = (W + 31) / 32;
-    a->num_blocks = a->block_tiles_x * ((H + 7) / 8);
+    set_tile_shape(*a, 3);  // 8x8 pixels per wavefront
+    return HL_OK;
+}
+
+}  // namespace hl
+
+namespace {
+
+// Shared set-up of the shading entry points: arguments, then the light records.
+hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
+                        const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights,
+                        const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0) {
+    if (!c) return HL_EINVAL;
+    if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
+    hl_status s = build_shade_args(c, camera_d, g, gb, num_lights, grid, a, set, need_debug0);
+    if (s != HL_OK) return s;
     if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(num_lights ? num_lights : 1))) != HL_OK) return s;
     if ((s = ensure(c, c->words, 64)) != HL_OK) return s;
     a->lights = (const HotLight*)c->hot.p;

@@ -21,14 +21,11 @@
 namespace hl {
 
 // ------------------------------------------------------------------- prep
-__global__ __launch_bounds__(256) void k_prep_lights(const GpuLight* __restrict__ lights,
-                                                     const uint32_t* __restrict__ subset,
-                                                     uint32_t n,
-                                                     const GpuCameraUniforms* __restrict__ cam,
-                                                     HotLight* __restrict__ hot,
-                                                     CullLight* __restrict__ cull,
-                                                     uint32_t* __restrict__ flags) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
+                                         const uint32_t* __restrict__ subset, uint32_t n,
+                                         const GpuCameraUniforms* __restrict__ cam,
+                                         HotLight* __restrict__ hot, CullLight* __restrict__ cull,
+                                         uint32_t* __restrict__ flags, uint32_t i) {
     if (i >= n) return;
     const GpuLight* l = &lights[subset ? subset[i] : i];
     const float4 pr = *reinterpret_cast<const float4*>(l->position_range);
@@ -69,6 +66,16 @@ __global__ __launch_bounds__(256) void k_prep_lights(const GpuLight* __restrict_
         if (l->ies_profile_index >= 0 || l->light_function_index >= 0) f |= kFlagTexturedIgnored;
         if (f) atomicOr(flags, f);
     }
+}
+
+__global__ __launch_bounds__(256) void k_prep_lights(const GpuLight* __restrict__ lights,
+                                                     const uint32_t* __restrict__ subset,
+                                                     uint32_t n,
+                                                     const GpuCameraUniforms* __restrict__ cam,
+                                                     HotLight* __restrict__ hot,
+                                                     CullLight* __restrict__ cull,
+                                                     uint32_t* __restrict__ flags) {
+    prep_one(lights, subset, n, cam, hot, cull, flags, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // -------------------------------------------------------------- tile cull
@@ -129,6 +136,11 @@ struct CellBox {
     float lo[3], hi[3];
 };
 
+// geometry scratch: ColumnGeom per column, then (256-B aligned) CellBox per cell
+__host__ __device__ inline size_t cluster_geom_offset_boxes_dev(uint32_t ncol) {
+    return (sizeof(ColumnGeom) * (size_t)ncol + 255) & ~(size_t)255;
+}
+
 __device__ __forceinline__ bool column_rect(const ClusterArgs& a, uint32_t cx, uint32_t cy,
                                             float* l, float* r, float* b, float* t) {
 #pragma clang fp contract(off)
@@ -147,13 +159,12 @@ __device__ __forceinline__ bool column_rect(const ClusterArgs& a, uint32_t cx, u
 // unprojected at the slice's padded near and far distances); the slice-0
 // thread of each column also writes the column's lateral planes (the tile
 // cull's construction on the column's pixel rectangle).
-__global__ __launch_bounds__(256) void k_cluster_geom(ClusterArgs a,
-                                                      const GpuCameraUniforms* __restrict__ cam,
-                                                      ColumnGeom* __restrict__ geom,
-                                                      CellBox* __restrict__ boxes) {
+__device__ __forceinline__ void geom_cell(const ClusterArgs& a,
+                                          const GpuCameraUniforms* __restrict__ cam,
+                                          ColumnGeom* __restrict__ geom,
+                                          CellBox* __restrict__ boxes, uint32_t cell) {
 #pragma clang fp contract(off)
     const uint32_t ncol = a.grid_x * a.grid_y;
-    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= ncol * a.grid_z) return;
     const uint32_t col = cell % ncol, k = cell / ncol;
     const uint32_t cx = col % a.grid_x, cy = col / a.grid_x;
@@ -230,16 +241,15 @@ __device__ __forceinline__ bool cell_member(const ClusterArgs& a, const CullLigh
 // lateral planes it can pass at its own depth, each widened by one -- and the
 // wave tests those candidate cells 64 at a time with the exact membership
 // rule, setting the member bits in the two-level bitmask.
-__global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
-                                                      const GpuCameraUniforms* __restrict__ cam,
-                                                      const CullLight* __restrict__ cl,
-                                                      const ColumnGeom* __restrict__ geom,
-                                                      const CellBox* __restrict__ boxes,
-                                                      uint32_t* __restrict__ bits,
-                                                      uint32_t* __restrict__ summary, uint32_t W,
-                                                      uint32_t SW) {
+__device__ __forceinline__ void mark_light(const ClusterArgs& a,
+                                           const GpuCameraUniforms* __restrict__ cam,
+                                           const CullLight* __restrict__ cl,
+                                           const ColumnGeom* __restrict__ geom,
+                                           const CellBox* __restrict__ boxes,
+                                           uint32_t* __restrict__ bits,
+                                           uint32_t* __restrict__ summary, uint32_t W, uint32_t SW,
+                                           uint32_t li) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t li = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (li >= a.num_lights) return;
     const int gx = (int)a.grid_x, gy = (int)a.grid_y, gz = (int)a.grid_z;
     CullLight L = cl[li];
@@ -307,11 +317,10 @@ __global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
 }
 
 // one wavefront per cluster: number of members (before the cap)
-__global__ __launch_bounds__(256) void k_cluster_count(uint32_t ncell, uint32_t W, uint32_t SW,
-                                                       const uint32_t* __restrict__ bits,
-                                                       const uint32_t* __restrict__ summary,
-                                                       uint32_t* __restrict__ raw) {
-    const uint32_t cell = blockIdx.x * 4u + (threadIdx.x >> 6);
+__device__ __forceinline__ void count_cell(uint32_t ncell, uint32_t W, uint32_t SW,
+                                           const uint32_t* __restrict__ bits,
+                                           const uint32_t* __restrict__ summary,
+                                           uint32_t* __restrict__ raw, uint32_t cell) {
     if (cell >= ncell) return;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t total = 0;
@@ -341,15 +350,20 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // Single block of 1024: capped counts -> offsets (exclusive scan in cluster
 // order), pool truncation, overflow flags.  Tiles of 4096 counts, read
 // coalesced, four per thread.
-__global__ __launch_bounds__(1024) void k_cluster_scan(const uint32_t* __restrict__ raw,
-                                                       uint32_t ncell, uint32_t cap,
-                                                       uint32_t pool_cap,
-                                                       uint32_t* __restrict__ offsets,
-                                                       uint32_t* __restrict__ counts,
-                                                       uint32_t* __restrict__ flags_out) {
-    __shared__ unsigned long long s_wave[16];
-    __shared__ unsigned long long s_carry;
-    __shared__ uint32_t s_flags;
+struct ScanSmem {
+    unsigned long long wave[16];
+    unsigned long long carry;
+    uint32_t flags;
+};
+
+__device__ __forceinline__ void scan_cells(const uint32_t* __restrict__ raw, uint32_t ncell,
+                                           uint32_t cap, uint32_t pool_cap,
+                                           uint32_t* __restrict__ offsets,
+                                           uint32_t* __restrict__ counts,
+                                           uint32_t* __restrict__ flags_out, ScanSmem& sm) {
+    unsigned long long* s_wave = sm.wave;
+    unsigned long long& s_carry = sm.carry;
+    uint32_t& s_flags = sm.flags;
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     if (t == 0) {
         s_carry = 0;
@@ -409,13 +423,12 @@ __global__ __launch_bounds__(1024) void k_cluster_scan(const uint32_t* __restric
 
 // one wavefront per cluster: write the members in ascending order up to the
 // (capped, pool-truncated) count, and clear the cluster's bitmask words.
-__global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t W, uint32_t SW,
-                                                      uint32_t* __restrict__ bits,
-                                                      uint32_t* __restrict__ summary,
-                                                      const uint32_t* __restrict__ offsets,
-                                                      const uint32_t* __restrict__ counts,
-                                                      uint32_t* __restrict__ pool) {
-    const uint32_t cell = blockIdx.x * 4u + (threadIdx.x >> 6);
+__device__ __forceinline__ void fill_cell(uint32_t ncell, uint32_t W, uint32_t SW,
+                                          uint32_t* __restrict__ bits,
+                                          uint32_t* __restrict__ summary,
+                                          const uint32_t* __restrict__ offsets,
+                                          const uint32_t* __restrict__ counts,
+                                          uint32_t* __restrict__ pool, uint32_t cell) {
     if (cell >= ncell) return;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t cnt = counts[cell];
@@ -452,6 +465,149 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// ------------------------------------------------------ one view per launch
+__global__ __launch_bounds__(256) void k_cluster_geom(ClusterArgs a,
+                                                      const GpuCameraUniforms* __restrict__ cam,
+                                                      ColumnGeom* __restrict__ geom,
+                                                      CellBox* __restrict__ boxes) {
+    geom_cell(a, cam, geom, boxes, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
+                                                      const GpuCameraUniforms* __restrict__ cam,
+                                                      const CullLight* __restrict__ cl,
+                                                      const ColumnGeom* __restrict__ geom,
+                                                      const CellBox* __restrict__ boxes,
+                                                      uint32_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ summary, uint32_t W,
+                                                      uint32_t SW) {
+    mark_light(a, cam, cl, geom, boxes, bits, summary, W, SW, blockIdx.x * 4u + (threadIdx.x >> 6));
+}
+
+__global__ __launch_bounds__(256) void k_cluster_count(uint32_t ncell, uint32_t W, uint32_t SW,
+                                                       const uint32_t* __restrict__ bits,
+                                                       const uint32_t* __restrict__ summary,
+                                                       uint32_t* __restrict__ raw) {
+    count_cell(ncell, W, SW, bits, summary, raw, blockIdx.x * 4u + (threadIdx.x >> 6));
+}
+
+__global__ __launch_bounds__(1024) void k_cluster_scan(const uint32_t* __restrict__ raw,
+                                                       uint32_t ncell, uint32_t cap,
+                                                       uint32_t pool_cap,
+                                                       uint32_t* __restrict__ offsets,
+                                                       uint32_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ flags_out) {
+    __shared__ ScanSmem sm;
+    scan_cells(raw, ncell, cap, pool_cap, offsets, counts, flags_out, sm);
+}
+
+__global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t W, uint32_t SW,
+                                                      uint32_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ summary,
+                                                      const uint32_t* __restrict__ offsets,
+                                                      const uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ pool) {
+    fill_cell(ncell, W, SW, bits, summary, offsets, counts, pool, blockIdx.x * 4u + (threadIdx.x >> 6));
+}
+
+// ------------------------------------------------- all views in one launch
+// base[] holds nviews + 1 prefix offsets; returns the view owning index g.
+__device__ __forceinline__ uint32_t find_view(const uint32_t* __restrict__ base, uint32_t nviews,
+                                              uint32_t g) {
+    uint32_t lo = 0, hi = nviews;  // base[lo] <= g < base[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ CellBox* view_boxes(const BatchCullView& v) {
+    return (CellBox*)((uint8_t*)v.geom + cluster_geom_offset_boxes_dev(v.ca.grid_x * v.ca.grid_y));
+}
+
+__global__ __launch_bounds__(256) void k_batch_prep(const GpuLight* __restrict__ pool,
+                                                    const BatchCullView* __restrict__ views,
+                                                    const uint32_t* __restrict__ light_base,
+                                                    uint32_t nviews) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= light_base[nviews]) return;
+    const uint32_t vi = find_view(light_base, nviews, g);
+    const BatchCullView& v = views[vi];
+    prep_one(pool, v.subset, v.ca.num_lights, v.cam, v.hot, v.cull, nullptr, g - light_base[vi]);
+}
+
+__global__ __launch_bounds__(256) void k_batch_geom(const BatchCullView* __restrict__ views,
+                                                    const uint32_t* __restrict__ cell_base,
+                                                    uint32_t nviews) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= cell_base[nviews]) return;
+    const uint32_t vi = find_view(cell_base, nviews, g);
+    const BatchCullView& v = views[vi];
+    geom_cell(v.ca, v.cam, (ColumnGeom*)v.geom, view_boxes(v), g - cell_base[vi]);
+}
+
+__global__ __launch_bounds__(256) void k_batch_mark(const BatchCullView* __restrict__ views,
+                                                    const uint32_t* __restrict__ light_base,
+                                                    uint32_t nviews) {
+    const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (g >= light_base[nviews]) return;
+    const uint32_t vi = find_view(light_base, nviews, g);
+    const BatchCullView& v = views[vi];
+    mark_light(v.ca, v.cam, v.cull, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary,
+               v.W, v.SW, g - light_base[vi]);
+}
+
+__global__ __launch_bounds__(256) void k_batch_count(const BatchCullView* __restrict__ views,
+                                                     const uint32_t* __restrict__ cell_base,
+                                                     uint32_t nviews) {
+    const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (g >= cell_base[nviews]) return;
+    const uint32_t vi = find_view(cell_base, nviews, g);
+    const BatchCullView& v = views[vi];
+    count_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.raw, g - cell_base[vi]);
+}
+
+// one block per view
+__global__ __launch_bounds__(1024) void k_batch_scan(const BatchCullView* __restrict__ views) {
+    __shared__ ScanSmem sm;
+    const BatchCullView& v = views[blockIdx.x];
+    scan_cells(v.raw, v.ncell, v.ca.cap, v.pool_cap, v.offsets, v.counts, v.flags, sm);
+}
+
+__global__ __launch_bounds__(256) void k_batch_fill(const BatchCullView* __restrict__ views,
+                                                    const uint32_t* __restrict__ cell_base,
+                                                    uint32_t nviews) {
+    const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (g >= cell_base[nviews]) return;
+    const uint32_t vi = find_view(cell_base, nviews, g);
+    const BatchCullView& v = views[vi];
+    fill_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.offsets, v.counts, v.pool, g - cell_base[vi]);
+}
+
+hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCullView* views,
+                             const uint32_t* light_base, const uint32_t* cell_base, uint32_t nviews,
+                             uint32_t total_lights, uint32_t total_cells) {
+    hipError_t e;
+    if (total_lights) {
+        k_batch_prep<<<(total_lights + 255) / 256, 256, 0, s>>>(pool, views, light_base, nviews);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    k_batch_geom<<<(total_cells + 255) / 256, 256, 0, s>>>(views, cell_base, nviews);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (total_lights) {
+        k_batch_mark<<<(total_lights + 3) / 4, 256, 0, s>>>(views, light_base, nviews);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    k_batch_count<<<(total_cells + 3) / 4, 256, 0, s>>>(views, cell_base, nviews);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_batch_scan<<<nviews, 1024, 0, s>>>(views);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_batch_fill<<<(total_cells + 3) / 4, 256, 0, s>>>(views, cell_base, nviews);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- launchers
 hipError_t launch_prep(hipStream_t s, const GpuLight* lights, const uint32_t* subset, uint32_t n,
                        const GpuCameraUniforms* cam, HotLight* hot, CullLight* cull,
@@ -477,10 +633,7 @@ hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const 
     return hipGetLastError();
 }
 
-// geometry scratch: ColumnGeom per column, then (256-B aligned) CellBox per cell
-size_t cluster_geom_offset_boxes(uint32_t ncol) {
-    return (sizeof(ColumnGeom) * (size_t)ncol + 255) & ~(size_t)255;
-}
+size_t cluster_geom_offset_boxes(uint32_t ncol) { return cluster_geom_offset_boxes_dev(ncol); }
 size_t cluster_geom_bytes(uint32_t ncol, uint32_t ncell) {
     return cluster_geom_offset_boxes(ncol) + sizeof(CellBox) * (size_t)ncell;
 }

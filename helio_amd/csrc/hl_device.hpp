@@ -82,8 +82,17 @@ struct ShadeArgs {
     float* d_pixel;          // 5 planes
     double* d_light;         // 4 per light: dpos.xyz, dI (FP64 accumulation)
     double* loss;            // fused L2 loss accumulator
-    uint32_t block_tiles_x;  // blocks per row of the launch grid
-    uint32_t num_blocks;
+    uint32_t block_tiles_x;  // block tiles per row
+    uint32_t num_blocks;     // block tiles of the view
+    uint32_t tw_log2;        // wavefront tile: (1 << tw_log2) x (64 >> tw_log2) pixels
+    uint32_t _pad0;
+};
+
+// One view of a batched persistent launch (device memory).
+struct BatchShadeView {
+    ShadeArgs sa;
+    uint32_t mode;  // kModeForward or kModeFusedL2
+    uint32_t _pad[3];
 };
 
 // ------------------------------------------------------------- exact math

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "hl_device.hpp"
 
 namespace hl {
@@ -18,6 +20,43 @@ struct ClusterArgs {
     float slice_far[HL_MAX_SLICES];
     float slice_r_pad[HL_MAX_SLICES];
 };
+
+}  // namespace hl
+
+// ------------------------------------------------------------------ context
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct hl_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf hot, cull, geom, bits, summary, raw, words, accum;
+    // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
+};
+
+namespace hl {
+
+hl_status fail(hl_context* c, hl_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+hl_status hip_fail(hl_context* c, hipError_t e, const char* what);
+// Grows a scratch buffer (never shrinks).  zero: a new allocation is cleared
+// (for buffers the kernels keep zero between calls).
+hl_status ensure(hl_context* c, DevBuf& b, size_t bytes, bool zero = false);
+hl_status check_gbuffer(hl_context* c, const hl_gbuffer* g, int* set);
+hl_status check_cluster_params(hl_context* c, const hl_cluster_params* p);
+ClusterArgs make_cluster_args(const hl_cluster_params* p, uint32_t num_lights);
+uint32_t* ctx_words(hl_context* c);
+hl_status build_shade_args(hl_context* c, const GpuCameraUniforms* camera_d,
+                           const DeferredGlobals* g, const hl_gbuffer* gb, uint32_t num_lights,
+                           const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0);
+
+#define HL_HIP(c, expr)                                        \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
+        if (_e != hipSuccess) return hl::hip_fail((c), _e, #expr); \
+    } while (0)
 
 // launchers (hl_cull.hip)
 hipError_t launch_prep(hipStream_t s, const GpuLight* lights, const uint32_t* subset, uint32_t n,
@@ -43,6 +82,28 @@ inline size_t cluster_bits_words(uint32_t ncell, uint32_t num_lights) {
 inline size_t cluster_summary_words(uint32_t ncell, uint32_t num_lights) {
     return (size_t)ncell * (((num_lights + 31u) / 32u + 31u) / 32u);
 }
+// One view of a batched cull (device memory): its cluster arguments, camera,
+// light subset, and its own per-view buffers.
+struct BatchCullView {
+    ClusterArgs ca;
+    const GpuCameraUniforms* cam;
+    const uint32_t* subset;  // pool indices, ascending (nullptr: the first ca.num_lights)
+    HotLight* hot;
+    CullLight* cull;
+    void* geom;              // cluster_geom_bytes(columns, cells)
+    uint32_t* bits;
+    uint32_t* summary;
+    uint32_t* raw;
+    uint32_t* offsets;
+    uint32_t* counts;
+    uint32_t* pool;
+    uint32_t* flags;
+    uint32_t pool_cap, W, SW, ncell;
+};
+hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCullView* views,
+                             const uint32_t* light_base, const uint32_t* cell_base, uint32_t nviews,
+                             uint32_t total_lights, uint32_t total_cells);
+
 hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCameraUniforms* cam,
                                const CullLight* cl, const ClusterScratch& w, uint32_t* offsets,
                                uint32_t* counts, uint32_t* pool, uint32_t pool_cap,
@@ -51,6 +112,15 @@ hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCam
 // launchers (hl_shade.hip)
 enum ShadeMode : int { kModeForward = 0, kModeCount = 1, kModeBackward = 2, kModeFusedL2 = 3 };
 hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& a);
+// block tiles of a view for a wavefront tile width (1 << tw_log2)
+inline void set_tile_shape(ShadeArgs& a, uint32_t tw_log2) {
+    const uint32_t bw = 4u << tw_log2, bh = 64u >> tw_log2;
+    a.tw_log2 = tw_log2;
+    a.block_tiles_x = (a.width + bw - 1) / bw;
+    a.num_blocks = a.block_tiles_x * ((a.height + bh - 1) / bh);
+}
+hipError_t launch_shade_batch(hipStream_t s, const BatchShadeView* views, const uint32_t* items,
+                              uint32_t num_items, uint32_t* queue, uint32_t grid_blocks);
 hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n);
 hipError_t launch_light_grads_to_f32(hipStream_t s, const double* acc, uint32_t n,
                                      const uint32_t* subset, float* d_pos, float* d_int);

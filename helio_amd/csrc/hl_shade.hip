@@ -295,16 +295,22 @@ __device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, u
     }
 }
 
-// ------------------------------------------------------------------ kernel
+// -------------------------------------------------------------- block tile
+// Shades block tile b of a view: 4 wavefronts side by side, each owning a
+// TW x (64/TW) pixel tile (TW = 1 << a.tw_log2: 8x8, 16x4 or 32x2).
+struct ShadeSmem {
+    HotLight light[4][kStage];
+    uint32_t index[4][kStage];
+};
+
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    __shared__ HotLight s_light[4][kStage];
-    __shared__ uint32_t s_index[4][kStage];
+__device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t b = hl_xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t twl = a.tw_log2, tw = 1u << twl;
+    const uint32_t th = 64u >> twl;
     const uint32_t bx = b % a.block_tiles_x, by = b / a.block_tiles_x;
-    const uint32_t x = bx * 32u + wave * 8u + (lane & 7u);
-    const uint32_t y = by * 8u + (lane >> 3);
+    const uint32_t x = bx * (4u * tw) + wave * tw + (lane & (tw - 1u));
+    const uint32_t y = by * th + (lane >> twl);
     const bool inside = x < a.width && y < a.height;
     const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
     const bool valid = inside && depth < 1.0f;
@@ -418,8 +424,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         cell = cx + a.grid_x * (cy + a.grid_y * cz);
     }
 
-    HotLight* s_l = s_light[wave];
-    uint32_t* s_i = s_index[wave];
+    HotLight* s_l = sm.light[wave];
+    uint32_t* s_i = sm.index[wave];
     float lo[3] = {0.0f, 0.0f, 0.0f};
 
     if (MODE == kModeCount) {
@@ -520,6 +526,54 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         a.d_pixel[3 * plane + pix] = dR;
         a.d_pixel[4 * plane + pix] = dM;
     }
+}
+
+// ------------------------------------------------------------------ kernels
+// One view: one block per block tile, blocks remapped so that consecutive
+// tiles share an XCD (and its L2: neighbouring tiles share cells and lights).
+template <int FA, int FN, int GRID, int MODE>
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
+    __shared__ ShadeSmem sm;
+    shade_tile<FA, FN, GRID, MODE>(a, hl_xcd_remap(blockIdx.x, gridDim.x), sm);
+}
+
+// Many views, one persistent launch: each block repeatedly claims the next
+// (view, block tile) item from an atomic queue until the queue is exhausted
+// (every block exits once its claim is past the end).  Items are ordered
+// heaviest first by the host.  Views run forward or forward + L2 adjoint.
+__global__ __launch_bounds__(256) void k_shade_batch(const BatchShadeView* __restrict__ views,
+                                                     const uint32_t* __restrict__ items,
+                                                     uint32_t num_items,
+                                                     uint32_t* __restrict__ queue) {
+    __shared__ ShadeSmem sm;
+    __shared__ uint32_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= num_items) break;
+        const uint32_t packed = items[it];
+        const BatchShadeView& v = views[packed >> 24];
+        const uint32_t blk = packed & 0xFFFFFFu;
+        if (v.mode == kModeFusedL2)
+            shade_tile<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT, HL_GRID_CLUSTERS, kModeFusedL2>(v.sa, blk, sm);
+        else
+            shade_tile<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT, HL_GRID_CLUSTERS, kModeForward>(v.sa, blk, sm);
+    }
+}
+
+__global__ void k_reset_queue(uint32_t* __restrict__ q, uint32_t n) {
+    if (threadIdx.x < n) q[threadIdx.x] = 0u;
+}
+
+hipError_t launch_shade_batch(hipStream_t s, const BatchShadeView* views, const uint32_t* items,
+                              uint32_t num_items, uint32_t* queue, uint32_t grid_blocks) {
+    k_reset_queue<<<1, 64, 0, s>>>(queue, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || num_items == 0) return e;
+    k_shade_batch<<<grid_blocks, 256, 0, s>>>(views, items, num_items, queue);
+    return hipGetLastError();
 }
 
 template <int FA, int FN, int MODE>

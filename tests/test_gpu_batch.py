@@ -1,0 +1,169 @@
+"""Batched multi-view plan on the GPU (48 probe faces + 2 XR eyes, each view
+shrunk 8x per side; 8192-light pool; log-uniform [32, 4096] subsets; L2
+adjoint on the two eyes).
+
+Bars:
+  * every view's radiance equals the single-view path run with the same
+    cluster grid bit for bit (same kernels, same lists, same order);
+  * the eyes' gradients equal the single-view fused adjoint within 1e-5
+    relative + 1e-7 of the largest magnitude (FP64 atomics reorder sums);
+  * each replay of the captured HIP graph equals the eager frame, with eager
+    work interleaved between replays;
+  * the autotuner returns a valid configuration and its cache round-trips.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from helio_amd import abi, synth
+from helio_amd.batch import BatchPlan, make_multiview
+from helio_amd.device import ClusterGrid, LightCullPass
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mv(ctx):
+    pool, pool_size, views, hosts, pool_lights = make_multiview(ctx, small=True)
+    # targets of the eyes: 0.9 x a forward render
+    for v in views:
+        if v.adjoint:
+            t = np.zeros((v.scene.host.height, v.scene.host.width, 4), np.float32)
+            t[..., :3] = 0.25
+            v.target.upload(t)
+    return pool, pool_size, views, hosts, pool_lights
+
+
+def single_view(ctx, hlib, v, cfg):
+    """The single-view path on the plan's cluster grid."""
+    sc = v.scene.host
+    sub_lights = sc.lights
+    p = synth.cluster_params(hlib, sc, grid=(cfg.grid_x, cfg.grid_y, cfg.grid_z), cap=0)
+    lights = ctx.upload(sub_lights) if len(sub_lights) else ctx.alloc(128)
+    cull = LightCullPass(ctx, sc.width, sc.height, clusters=p, pool_cap=1 << 22)
+    cam = v.scene.camera
+    ctx.lib.call("hl_light_cull_clusters", ctx.h, cam.ptr, C.byref(p), lights.ptr, len(sub_lights),
+                 cull.cgrid.offsets.ptr, cull.cgrid.counts.ptr, cull.cgrid.pool.ptr, cull.cgrid.pool_cap,
+                 cull.cgrid.overflow.ptr)
+    grid = cull.publish()
+    W, H = sc.width, sc.height
+    out = ctx.alloc(W * H * 16)
+    res = {}
+    if not v.adjoint:
+        ctx.lib.call("hl_deferred_forward", ctx.h, cam.ptr, C.byref(sc.globals), C.byref(v.scene.gbuffer),
+                     lights.ptr, len(sub_lights), C.byref(grid), out.ptr, W * 16, 0)
+    else:
+        n = len(sub_lights)
+        loss = ctx.alloc(8)
+        dpx = ctx.alloc(5 * W * H * 4)
+        dpos = ctx.alloc(3 * max(n, 1) * 4)
+        dint = ctx.alloc(max(n, 1) * 4)
+        ctx.lib.call("hl_deferred_forward_backward_l2", ctx.h, cam.ptr, C.byref(sc.globals),
+                     C.byref(v.scene.gbuffer), lights.ptr, n, C.byref(grid), v.target.ptr, W * 16, out.ptr,
+                     W * 16, loss.ptr, dpx.ptr, dpos.ptr, dint.ptr)
+        res.update(loss=float(loss.download(np.float64)[0]), dpx=dpx.download(np.float32),
+                   dpos=dpos.download(np.float32)[: 3 * n].reshape(n, 3), dint=dint.download(np.float32)[:n])
+    res["out"] = out.download(np.float32, (H, W, 4))
+    return res
+
+
+def read_view(v, pool_size, subset):
+    sc = v.scene.host
+    r = {"out": v.out.download(np.float32, (sc.height, sc.width, 4))}
+    if v.adjoint:
+        r["loss"] = float(v.loss.download(np.float64)[0])
+        r["dpx"] = v.d_pixel.download(np.float32)
+        r["dpos"] = v.d_pos.download(np.float32).reshape(pool_size, 3)[subset]
+        r["dint"] = v.d_int.download(np.float32)[subset]
+    return r
+
+
+def close(a, b, rel=1e-5):
+    scale = np.abs(b).max() if b.size else 0.0
+    return np.all(np.abs(a - b) <= rel * np.abs(b) + 1e-7 * scale)
+
+
+def test_batch_equals_single_view(ctx, hlib, mv):
+    pool, pool_size, views, hosts, _ = mv
+    print("[batch] 50 views eager vs single-view", flush=True)
+    plan = BatchPlan(ctx, pool, pool_size, views)
+    try:
+        for v in views:
+            v.out.fill_u32(0)
+        plan.launch()
+        ctx.synchronize()
+        got = [read_view(v, pool_size, sub) for v, (_, sub) in zip(views, hosts)]
+        for i, v in enumerate(views):
+            cfg = plan.config(i)
+            ref = single_view(ctx, hlib, v, cfg)
+            sky = v.scene.host.depth >= 1.0
+            assert np.array_equal(got[i]["out"][~sky], ref["out"][~sky]), f"view {i}"
+            if v.adjoint:
+                assert abs(got[i]["loss"] - ref["loss"]) <= 1e-6 * abs(ref["loss"])
+                assert close(got[i]["dpx"], ref["dpx"]), f"view {i} pixel grads"
+                assert close(got[i]["dpos"], ref["dpos"]), f"view {i} light pos grads"
+                assert close(got[i]["dint"], ref["dint"]), f"view {i} intensity grads"
+    finally:
+        plan.destroy()
+
+
+def test_graph_replays_match_eager(ctx, mv):
+    pool, pool_size, views, hosts, _ = mv
+    print("[batch] graph replays", flush=True)
+    plan = BatchPlan(ctx, pool, pool_size, views)
+    try:
+        plan.launch()
+        ctx.synchronize()
+        eager = [read_view(v, pool_size, sub) for v, (_, sub) in zip(views, hosts)]
+        plan.capture()
+        for rep in range(3):
+            for v in views:  # eager work between replays
+                v.out.fill_u32(0x7fc00000)
+            plan.launch()
+            ctx.synchronize()
+            for i, (v, (_, sub)) in enumerate(zip(views, hosts)):
+                r = read_view(v, pool_size, sub)
+                lit = v.scene.host.depth < 1.0  # sky pixels are left untouched by contract
+                assert np.array_equal(r["out"][lit], eager[i]["out"][lit]), (rep, i)
+                if v.adjoint:
+                    assert close(r["dpx"], eager[i]["dpx"]) and close(r["dpos"], eager[i]["dpos"])
+                    assert abs(r["loss"] - eager[i]["loss"]) <= 1e-6 * abs(eager[i]["loss"])
+    finally:
+        plan.destroy()
+
+
+def test_autotuner_and_cache(ctx, mv, tmp_path):
+    pool, pool_size, views, hosts, _ = mv
+    print("[batch] autotune", flush=True)
+    sel = views[::10] + [views[-1]]
+    cache = str(tmp_path / "tune.txt")
+    plan = BatchPlan(ctx, pool, pool_size, sel, autotune=True, max_trials=4, cache_path=cache)
+    try:
+        cfgs = [plan.config(i) for i in range(len(sel))]
+        for c in cfgs:
+            assert (c.wave_tile_w, c.wave_tile_h) in ((8, 8), (16, 4), (32, 2))
+            assert 1 <= c.grid_z <= 64 and c.est_ms > 0
+    finally:
+        plan.destroy()
+    text = open(cache).read()
+    assert text.count("\n") >= 1
+    plan2 = BatchPlan(ctx, pool, pool_size, sel, autotune=True, max_trials=4, cache_path=cache)
+    try:
+        for i, c in enumerate(cfgs):
+            c2 = plan2.config(i)
+            assert (c2.wave_tile_w, c2.grid_x, c2.grid_y, c2.grid_z) == (c.wave_tile_w, c.grid_x, c.grid_y, c.grid_z)
+    finally:
+        plan2.destroy()
+
+
+def test_plan_rejects_bad_subset(ctx, mv):
+    pool, pool_size, views, hosts, _ = mv
+    v = views[0]
+    bad = ctx.upload(np.array([5, 3, 9000], np.uint32))
+    from helio_amd.batch import BatchView
+    bv = BatchView(scene=v.scene, subset=bad, num_lights=3, out=v.out)
+    with pytest.raises(abi.HelioError) as e:
+        BatchPlan(ctx, pool, pool_size, [bv])
+    assert e.value.status == abi.HL_EINVAL
