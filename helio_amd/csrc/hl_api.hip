@@ -292,7 +292,8 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
     if (!c) return HL_EINVAL;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->words, &c->accum})
+    for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->markq, &c->words,
+                      &c->accum})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return HL_OK;
@@ -494,22 +495,23 @@ HL_API hl_status hl_light_cull_clusters(hl_context* c, const GpuCameraUniforms* 
     const size_t ncell = (size_t)ncol * p->grid_z;
     const uint32_t n = num_lights;
     if ((s = ensure(c, c->cull, sizeof(CullLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
-    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
     if ((s = ensure(c, c->geom, cluster_geom_bytes(ncol, (uint32_t)ncell))) != HL_OK) return s;
     if ((s = ensure(c, c->bits, 4 * cluster_bits_words(ncell, n), true)) != HL_OK) return s;
     if ((s = ensure(c, c->summary, 4 * cluster_summary_words(ncell, n), true)) != HL_OK) return s;
-    if ((s = ensure(c, c->raw, sizeof(uint32_t) * ncell)) != HL_OK) return s;
+    if ((s = ensure(c, c->raw, cluster_raw_bytes((uint32_t)ncell))) != HL_OK) return s;
+    if ((s = ensure(c, c->markq, mark_queue_bytes(n))) != HL_OK) return s;
     HL_HIP(c, hipSetDevice(c->device));
+    // the overflow word(s) are reset by the first cull kernel and OR-ed by the
+    // scan: the caller's word when given, and always the context's copy
     uint32_t* flags = overflow_d ? overflow_d : ctx_words(c);
-    HL_HIP(c, launch_fill_u32(c->stream, flags, 0, 1));
-    if (overflow_d) HL_HIP(c, launch_fill_u32(c->stream, ctx_words(c), 0, 1));
-    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, n, camera_d, (HotLight*)c->hot.p,
-                          (CullLight*)c->cull.p, ctx_words(c) + 1));
+    uint32_t* flags2 = overflow_d ? ctx_words(c) : nullptr;
+    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, n, camera_d, nullptr, (CullLight*)c->cull.p,
+                          ctx_words(c) + 1));
     ClusterArgs a = make_cluster_args(p, n);
-    ClusterScratch w{c->geom.p, (uint32_t*)c->bits.p, (uint32_t*)c->summary.p, (uint32_t*)c->raw.p};
+    ClusterScratch w{c->geom.p, (uint32_t*)c->bits.p, (uint32_t*)c->summary.p, (uint32_t*)c->raw.p,
+                     mark_queue_at(c->markq.p, n)};
     HL_HIP(c, launch_cluster_cull(c->stream, a, camera_d, (const CullLight*)c->cull.p, w, offsets_d,
-                                  counts_d, pool_d, pool_cap, flags));
-    if (overflow_d) HL_HIP(c, hipMemcpyAsync(ctx_words(c), overflow_d, 4, hipMemcpyDeviceToDevice, c->stream));
+                                  counts_d, pool_d, pool_cap, flags, flags2));
     return HL_OK;
 }
 

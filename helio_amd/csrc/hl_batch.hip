@@ -62,10 +62,12 @@ struct hl_plan {
     BatchShadeView* d_shade = nullptr;
     uint32_t* d_light_base = nullptr;
     uint32_t* d_cell_base = nullptr;
+    uint32_t* d_tile_base = nullptr;
     uint32_t* d_items = nullptr;
     uint32_t* d_queue = nullptr;
     uint32_t* d_flags = nullptr;
-    uint32_t total_lights = 0, total_cells = 0, num_items = 0, grid_blocks = 0;
+    void* d_markq = nullptr;  // mark_queue_bytes(total_lights)
+    uint32_t total_lights = 0, total_cells = 0, total_tiles = 0, num_items = 0, grid_blocks = 0;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -97,11 +99,13 @@ void free_bufs(ViewBufs& b) {
 void free_plan_device(hl_plan* p) {
     for (auto& b : p->bufs) free_bufs(b);
     for (void* q : {(void*)p->d_cull, (void*)p->d_shade, (void*)p->d_light_base, (void*)p->d_cell_base,
-                    (void*)p->d_items, (void*)p->d_queue, (void*)p->d_flags})
+                    (void*)p->d_tile_base,
+                    (void*)p->d_items, (void*)p->d_queue, (void*)p->d_flags, p->d_markq})
         if (q) (void)hipFree(q);
+    p->d_markq = nullptr;
     p->d_cull = nullptr;
     p->d_shade = nullptr;
-    p->d_light_base = p->d_cell_base = p->d_items = p->d_queue = p->d_flags = nullptr;
+    p->d_light_base = p->d_cell_base = p->d_tile_base = p->d_items = p->d_queue = p->d_flags = nullptr;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     p->exec = nullptr;
@@ -128,8 +132,9 @@ hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
         HL_HIP(c, launch_fill_u32(s, (uint32_t*)d.d_light_intensity_d, 0, p->pool_size));
     }
     HL_HIP(c, launch_fill_u32(s, p->d_flags, 0, nv));
-    HL_HIP(c, launch_batch_cull(s, p->pool, p->d_cull, p->d_light_base, p->d_cell_base, nv,
-                                p->total_lights, p->total_cells));
+    HL_HIP(c, launch_batch_cull(s, p->pool, p->d_cull, p->d_light_base, p->d_cell_base,
+                                p->d_tile_base, nv, p->total_lights, p->total_cells, p->total_tiles,
+                                mark_queue_at(p->d_markq, p->total_lights)));
     HL_HIP(c, launch_shade_batch(s, p->d_shade, p->d_items, p->num_items, p->d_queue, p->grid_blocks));
     for (uint32_t v = 0; v < nv; ++v) {
         const hl_view_desc& d = p->views[v];
@@ -149,7 +154,7 @@ hl_status build_plan(hl_plan* p) {
     p->bufs.assign(nv, ViewBufs{});
     p->hcull.assign(nv, BatchCullView{});
     p->hshade.assign(nv, BatchShadeView{});
-    std::vector<uint32_t> light_base(nv + 1, 0), cell_base(nv + 1, 0);
+    std::vector<uint32_t> light_base(nv + 1, 0), cell_base(nv + 1, 0), tile_base(nv + 1, 0);
     hl_status s;
     if ((s = plan_alloc(p, (void**)&p->d_flags, 4 * (size_t)nv)) != HL_OK) return s;
     for (uint32_t v = 0; v < nv; ++v) {
@@ -168,7 +173,7 @@ hl_status build_plan(hl_plan* p) {
         if ((s = plan_alloc(p, &b.geom, cluster_geom_bytes(ncol, ncell))) != HL_OK) return s;
         if ((s = plan_alloc(p, &b.bits, 4 * cluster_bits_words(ncell, n), true)) != HL_OK) return s;
         if ((s = plan_alloc(p, &b.summary, 4 * cluster_summary_words(ncell, n), true)) != HL_OK) return s;
-        if ((s = plan_alloc(p, &b.raw, 4 * (size_t)ncell)) != HL_OK) return s;
+        if ((s = plan_alloc(p, &b.raw, cluster_raw_bytes(ncell))) != HL_OK) return s;
         if ((s = plan_alloc(p, &b.offsets, 4 * (size_t)ncell)) != HL_OK) return s;
         if ((s = plan_alloc(p, &b.counts, 4 * (size_t)ncell)) != HL_OK) return s;
         // provisional pool; sized exactly after a first cull below
@@ -196,20 +201,26 @@ hl_status build_plan(hl_plan* p) {
         cv.ncell = ncell;
         light_base[v + 1] = light_base[v] + n;
         cell_base[v + 1] = cell_base[v] + ncell;
+        tile_base[v + 1] = tile_base[v] + cluster_scan_tiles(ncell);
     }
     p->total_lights = light_base[nv];
     p->total_cells = cell_base[nv];
+    p->total_tiles = tile_base[nv];
     if ((s = plan_alloc(p, (void**)&p->d_cull, sizeof(BatchCullView) * nv)) != HL_OK) return s;
     if ((s = plan_alloc(p, (void**)&p->d_light_base, 4 * (size_t)(nv + 1))) != HL_OK) return s;
     if ((s = plan_alloc(p, (void**)&p->d_cell_base, 4 * (size_t)(nv + 1))) != HL_OK) return s;
+    if ((s = plan_alloc(p, &p->d_markq, mark_queue_bytes(p->total_lights))) != HL_OK) return s;
+    if ((s = plan_alloc(p, (void**)&p->d_tile_base, 4 * (size_t)(nv + 1))) != HL_OK) return s;
+    HL_HIP(c, hipMemcpy(p->d_tile_base, tile_base.data(), 4 * (nv + 1), hipMemcpyHostToDevice));
     HL_HIP(c, hipMemcpy(p->d_light_base, light_base.data(), 4 * (nv + 1), hipMemcpyHostToDevice));
     HL_HIP(c, hipMemcpy(p->d_cell_base, cell_base.data(), 4 * (nv + 1), hipMemcpyHostToDevice));
     // size each pool from a first cull: exact totals + headroom
     for (int pass = 0; pass < 2; ++pass) {
         HL_HIP(c, hipMemcpy(p->d_cull, p->hcull.data(), sizeof(BatchCullView) * nv, hipMemcpyHostToDevice));
         HL_HIP(c, launch_fill_u32(c->stream, p->d_flags, 0, nv));
-        HL_HIP(c, launch_batch_cull(c->stream, p->pool, p->d_cull, p->d_light_base, p->d_cell_base, nv,
-                                    p->total_lights, p->total_cells));
+        HL_HIP(c, launch_batch_cull(c->stream, p->pool, p->d_cull, p->d_light_base, p->d_cell_base,
+                                    p->d_tile_base, nv, p->total_lights, p->total_cells,
+                                    p->total_tiles, mark_queue_at(p->d_markq, p->total_lights)));
         HL_HIP(c, hipStreamSynchronize(c->stream));
         bool grew = false;
         for (uint32_t v = 0; v < nv; ++v) {

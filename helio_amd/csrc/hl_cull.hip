@@ -9,11 +9,14 @@
 //                     (projected bounding box x slice range); a wavefront tests
 //                     those candidates 64 at a time with the exact membership
 //                     rule and sets the member bits in a two-level bitmask
-//                     (bits + a summary bit per non-empty word)
-//  k_cluster_count    per cluster: popcount over its non-empty words
-//  k_cluster_scan     exclusive scan of the capped counts (single block)
-//  k_cluster_fill     per cluster: ascending indices from the bitmask, which it
-//                     clears again for the next call
+//                     (bits + a summary bit per non-empty word); past its first
+//                     chunks a light queues the rest
+//  k_cluster_mark_tail the queued chunks, spread over all CUs
+//  k_cluster_count    per cluster (16 lanes): popcount over its non-empty words
+//  k_cluster_scan     exclusive scan of the capped counts, a block per tile of
+//                     1024 clusters (tile sums come from the count kernel)
+//  k_cluster_fill     per cluster (16 lanes): ascending indices from the
+//                     bitmask, which it clears again for the next call
 // Membership is decided by the same exact FP32 tests whatever the order of
 // evaluation, so the lists equal the per-cluster brute force bit for bit.
 #include "hl_internal.hpp"
@@ -43,7 +46,8 @@ __device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
             h.p1 = make_float4(ci.x * ci.w, ci.y * ci.w, ci.z * ci.w, inner);
         }
         h.p2 = make_float4(dro.x, dro.y, dro.z, type == HL_LIGHT_SPOT ? dro.w : kNoCone);
-        h.p3 = ci;
+        const float margin = 1e-5f * (fabsf(dro.x) + fabsf(dro.y) + fabsf(dro.z));
+        h.p3 = make_float4(1.0f / (range * range), ci.w, margin, 0.0f);
         hot[i] = h;
     }
     if (cull) {
@@ -205,68 +209,61 @@ __device__ __forceinline__ void geom_cell(const ClusterArgs& a,
     geom[col] = g;
 }
 
-// first k in [0, n) with v[k] >= x (v ascending); n if none
-__device__ __forceinline__ int lower_bound(const float* v, int n, float x) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (v[mid] < x) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Membership of light (c) in cell (col, k): the definition of the clustered
 // lists -- lateral planes of the column with the column padding, the cell's
-// AABB with the slice padding, and the cone for spot lights.
-__device__ __forceinline__ bool cell_member(const ClusterArgs& a, const CullLight& c,
-                                            const ColumnGeom& g, const CellBox& bx, uint32_t k) {
+// AABB with the slice padding (pad = slice_r_pad[k]), and the cone for spot
+// lights.  Every test is evaluated (no short circuit) so that the column and
+// box records are loaded together rather than one after another.
+__device__ __forceinline__ bool cell_member(const CullLight& c, const ColumnGeom& g,
+                                            const CellBox& bx, float pad) {
 #pragma clang fp contract(off)
-    const float pad = a.slice_r_pad[k];
     const float rc = c.vpos.w + pad;
-    if (!(hl_x_plane_pass(g.pl[0], c.vpos, rc) && hl_x_plane_pass(g.pl[1], c.vpos, rc) &&
-          hl_x_plane_pass(g.pl[2], c.vpos, rc) && hl_x_plane_pass(g.pl[3], c.vpos, rc)))
-        return false;
     const float3 lo = make_float3(bx.lo[0], bx.lo[1], bx.lo[2]);
     const float3 hi = make_float3(bx.hi[0], bx.hi[1], bx.hi[2]);
-    if (!hl_x_sphere_aabb(c.vpos, rc, lo, hi)) return false;
-    if (c.cone.w != kNoCone && !hl_x_cone_aabb(c.vpos, c.cone, pad, lo, hi)) return false;
-    return true;
+    bool in = g.on != 0u;
+    in &= hl_x_plane_pass(g.pl[0], c.vpos, rc);
+    in &= hl_x_plane_pass(g.pl[1], c.vpos, rc);
+    in &= hl_x_plane_pass(g.pl[2], c.vpos, rc);
+    in &= hl_x_plane_pass(g.pl[3], c.vpos, rc);
+    in &= hl_x_sphere_aabb(c.vpos, rc, lo, hi);
+    if (c.cone.w != kNoCone) in &= hl_x_cone_aabb(c.vpos, c.cone, pad, lo, hi);
+    return in;
 }
 
-// One wavefront per light.  The light bounds the cells it can reach -- the
-// slices its view distance +- (radius + padding) spans and the columns whose
-// lateral planes it can pass at its own depth, each widened by one -- and the
-// wave tests those candidate cells 64 at a time with the exact membership
-// rule, setting the member bits in the two-level bitmask.
-__device__ __forceinline__ void mark_light(const ClusterArgs& a,
-                                           const GpuCameraUniforms* __restrict__ cam,
-                                           const CullLight* __restrict__ cl,
-                                           const ColumnGeom* __restrict__ geom,
-                                           const CellBox* __restrict__ boxes,
-                                           uint32_t* __restrict__ bits,
-                                           uint32_t* __restrict__ summary, uint32_t W, uint32_t SW,
-                                           uint32_t li) {
-    const uint32_t lane = threadIdx.x & 63u;
-    if (li >= a.num_lights) return;
+// Candidate cells of a light: the slices its view distance +- (radius +
+// padding) spans and the columns whose lateral planes it can pass at its own
+// depth, each widened by one.  False when there are none.
+struct MarkBox {
+    uint32_t x0, y0, s0, nx, ny, nz;
+};
+
+// first k in [0, n) with v[k] >= x, for ascending v held one per lane (lane
+// k holds v[k], n <= 64): the number of lanes below n with v[k] < x
+__device__ __forceinline__ int lane_lower_bound(float vk, int n, float x) {
+    return (int)__popcll(hl_ballot((int)hl_lane() < n && vk < x));
+}
+
+// All 64 lanes must be active (the slice bounds are found by ballots); sp is
+// slice_r_pad[lane] (lanes below grid_z).
+__device__ __forceinline__ bool light_cells(const ClusterArgs& a,
+                                            const GpuCameraUniforms* __restrict__ cam,
+                                            const CullLight& L, float sp, MarkBox& m) {
     const int gx = (int)a.grid_x, gy = (int)a.grid_y, gz = (int)a.grid_z;
-    CullLight L = cl[li];
-    L.vpos = make_float4(hl_uniform(L.vpos.x), hl_uniform(L.vpos.y), hl_uniform(L.vpos.z),
-                         hl_uniform(L.vpos.w));
-    L.cone = make_float4(hl_uniform(L.cone.x), hl_uniform(L.cone.y), hl_uniform(L.cone.z),
-                         hl_uniform(L.cone.w));
+    const uint32_t lane = hl_lane();
+    const float sn = lane < a.grid_z ? a.slice_near[lane] : 0.0f;
+    const float sf = lane < a.grid_z ? a.slice_far[lane] : 0.0f;
     int x0 = 0, x1 = gx - 1, y0 = 0, y1 = gy - 1, s0 = 0, s1 = gz - 1;
     if (!__builtin_isinf(L.vpos.w)) {
         const float zc = -L.vpos.z;
         // slices under the largest padding, then again under the largest
         // padding of those slices (paddings grow with distance)
         float rr = L.vpos.w + a.r_pad_column;
-        s1 = clampi(lower_bound(a.slice_near, gz, zc + rr) + 1, 0, gz - 1);
-        rr = L.vpos.w + a.slice_r_pad[s1];
-        s0 = lower_bound(a.slice_far, gz, zc - rr) - 1;
-        s1 = lower_bound(a.slice_near, gz, zc + rr) + 1;
+        s1 = clampi(lane_lower_bound(sn, gz, zc + rr) + 1, 0, gz - 1);
+        rr = L.vpos.w + hl_readlane(sp, (uint32_t)s1);
+        s0 = lane_lower_bound(sf, gz, zc - rr) - 1;
+        s1 = lane_lower_bound(sn, gz, zc + rr) + 1;
         if (zc > 1e-4f) {
             // A lateral plane through the origin at angle alpha from the view
             // axis passes a centre up to rr / cos(alpha) beyond the column's
@@ -293,170 +290,349 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
     y1 = clampi(y1, -1, gy - 1);
     s0 = clampi(s0, 0, gz - 1);
     s1 = clampi(s1, -1, gz - 1);
-    if (x1 < x0 || y1 < y0 || s1 < s0) return;
-    const uint32_t nx = (uint32_t)(x1 - x0 + 1), ny = (uint32_t)(y1 - y0 + 1),
-                   nz = (uint32_t)(s1 - s0 + 1);
-    const uint32_t work = nx * ny * nz;
-    const uint32_t ncol = a.grid_x * a.grid_y;
-    const uint32_t bit = 1u << (li & 31u);
-    const uint32_t word = li >> 5;
-    for (uint32_t t0 = 0; t0 < work; t0 += 64u) {
-        const uint32_t t = t0 + lane;
-        if (t >= work) break;
-        const uint32_t kz = t % nz;
-        const uint32_t rest = t / nz;
-        const uint32_t col = ((uint32_t)x0 + rest % nx) + a.grid_x * ((uint32_t)y0 + rest / nx);
-        const uint32_t k = (uint32_t)s0 + kz;
-        const ColumnGeom g = geom[col];
-        if (!g.on) continue;
-        const size_t cell = (size_t)col + (size_t)ncol * k;
-        if (!cell_member(a, L, g, boxes[cell], k)) continue;
-        const uint32_t old = atomicOr(&bits[cell * W + word], bit);
-        if (old == 0u) atomicOr(&summary[cell * SW + (word >> 5)], 1u << (word & 31u));
-    }
+    if (x1 < x0 || y1 < y0 || s1 < s0) return false;
+    m.x0 = (uint32_t)x0;
+    m.y0 = (uint32_t)y0;
+    m.s0 = (uint32_t)s0;
+    m.nx = (uint32_t)(x1 - x0 + 1);
+    m.ny = (uint32_t)(y1 - y0 + 1);
+    m.nz = (uint32_t)(s1 - s0 + 1);
+    return true;
 }
 
-// one wavefront per cluster: number of members (before the cap)
-__device__ __forceinline__ void count_cell(uint32_t ncell, uint32_t W, uint32_t SW,
-                                           const uint32_t* __restrict__ bits,
-                                           const uint32_t* __restrict__ summary,
-                                           uint32_t* __restrict__ raw, uint32_t cell) {
-    if (cell >= ncell) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t total = 0;
-    for (uint32_t s0 = 0; s0 < SW; s0 += 64u) {
-        const uint32_t j = s0 + lane;
-        uint32_t sw = j < SW ? summary[(size_t)cell * SW + j] : 0u;
-        while (sw) {
-            const uint32_t b = (uint32_t)__builtin_ctz(sw);
-            sw &= sw - 1u;
-            total += (uint32_t)__popc(bits[(size_t)cell * W + 32u * j + b]);
+__device__ __forceinline__ uint4 pack_box(const MarkBox& m) {
+    return make_uint4(m.x0, m.y0, (m.s0 << 16) | m.nz, (m.nx << 16) | m.ny);
+}
+__device__ __forceinline__ MarkBox unpack_box(uint4 v) {
+    MarkBox m;
+    m.x0 = v.x;
+    m.y0 = v.y;
+    m.s0 = v.z >> 16;
+    m.nz = v.z & 0xffffu;
+    m.nx = v.w >> 16;
+    m.ny = v.w & 0xffffu;
+    return m;
+}
+
+// Candidate cells chunk*64 .. chunk*64+63 of light li (one per lane, slice
+// fastest): the exact membership rule, then the member bit and its summary
+// bit (no-return atomics).  All 64 lanes must be active; sp is
+// slice_r_pad[lane] (lanes below grid_z).
+__device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight& L,
+                                           const MarkBox& m, uint32_t li, float sp,
+                                           const ColumnGeom* __restrict__ geom,
+                                           const CellBox* __restrict__ boxes,
+                                           uint32_t* __restrict__ bits,
+                                           uint32_t* __restrict__ summary, uint32_t W,
+                                           uint32_t SW, uint32_t chunk) {
+    const uint32_t work = m.nx * m.ny * m.nz;
+    const uint32_t t0 = chunk * 64u + (threadIdx.x & 63u);
+    const uint32_t t = t0 < work ? t0 : work - 1u;
+    uint32_t rest, iy;
+    if (work <= (1u << 21)) {
+        rest = hl_floor_div(t, m.nz);
+        iy = hl_floor_div(rest, m.nx);
+    } else {
+        rest = t / m.nz;
+        iy = rest / m.nx;
+    }
+    const uint32_t kz = t - rest * m.nz, ix = rest - iy * m.nx;
+    const uint32_t col = (m.x0 + ix) + a.grid_x * (m.y0 + iy);
+    const uint32_t k = m.s0 + kz;
+    const float pad = __shfl(sp, (int)k, 64);  // every lane active here
+    if (t0 >= work) return;
+    const size_t cell = (size_t)col + (size_t)(a.grid_x * a.grid_y) * k;
+    if (!cell_member(L, geom[col], boxes[cell], pad)) return;
+    const uint32_t word = li >> 5;
+    atomicOr(&bits[cell * W + word], 1u << (li & 31u));
+    atomicOr(&summary[cell * SW + (word >> 5)], 1u << (word & 31u));
+}
+
+__device__ __forceinline__ CullLight load_light_uniform(const CullLight* __restrict__ cl, uint32_t li) {
+    CullLight L = cl[li];
+    L.vpos = make_float4(hl_uniform(L.vpos.x), hl_uniform(L.vpos.y), hl_uniform(L.vpos.z),
+                         hl_uniform(L.vpos.w));
+    L.cone = make_float4(hl_uniform(L.cone.x), hl_uniform(L.cone.y), hl_uniform(L.cone.z),
+                         hl_uniform(L.cone.w));
+    return L;
+}
+
+// One wavefront per light: its candidate box, its first kMarkInline chunks,
+// and the rest queued as (qlight, chunk) entries with the box stored at
+// bounds[qlight] (qlight: the light's index in the queue's numbering).
+// Chunks that do not fit in the light's sub-queue stay with this wave.
+__device__ __forceinline__ void mark_light(const ClusterArgs& a,
+                                           const GpuCameraUniforms* __restrict__ cam,
+                                           const CullLight* __restrict__ cl,
+                                           const ColumnGeom* __restrict__ geom,
+                                           const CellBox* __restrict__ boxes,
+                                           uint32_t* __restrict__ bits,
+                                           uint32_t* __restrict__ summary, uint32_t W, uint32_t SW,
+                                           uint32_t li, const MarkQueue& q, uint32_t qlight) {
+    if (li >= a.num_lights) return;
+    const CullLight L = load_light_uniform(cl, li);
+    const float sp = hl_lane() < a.grid_z ? a.slice_r_pad[hl_lane()] : 0.0f;
+    MarkBox m;
+    if (!light_cells(a, cam, L, sp, m)) return;
+    const uint32_t nchunk = (m.nx * m.ny * m.nz + 63u) / 64u;
+    uint32_t qlo = nchunk, qhi = nchunk;  // chunks [qlo, qhi) are queued
+    if (nchunk > kMarkInline) {
+        const uint32_t extra = nchunk - kMarkInline;
+        const uint32_t sub = qlight % kMarkQueues;
+        uint32_t base = 0;
+        if ((threadIdx.x & 63u) == 0) base = atomicAdd(q.count + sub * kMarkCountStride, extra);
+        base = hl_readlane(base, 0);
+        const uint32_t fit = base >= q.seg ? 0u : min(extra, q.seg - base);
+        if (fit) {
+            if ((threadIdx.x & 63u) == 0) q.bounds[qlight] = pack_box(m);
+            uint2* seg = q.entries + (size_t)sub * q.seg + base;
+            for (uint32_t j = threadIdx.x & 63u; j < fit; j += 64u)
+                seg[j] = make_uint2(qlight, kMarkInline + j);
+            qlo = kMarkInline;
+            qhi = kMarkInline + fit;
         }
     }
-    const uint32_t t = (uint32_t)dpp_sum((float)total);  // exact below 2^24 lights
-    if (lane == 0) raw[cell] = t;
+    for (uint32_t c = 0; c < nchunk; ++c) {
+        if (c >= qlo && c < qhi) continue;
+        mark_chunk(a, L, m, li, sp, geom, boxes, bits, summary, W, SW, c);
+    }
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = hl_lane();
+// Count and fill work on 16 lanes per cluster (four clusters per wavefront).
+// The non-empty bitmask words of a cluster are first listed in LDS in
+// ascending order (lane j of a group owns summary word j (+ 16 r), i.e. bitmask
+// words 32 j .. 32 j + 31) and then read together, eight per lane, so that a
+// cluster costs one round trip to memory for its words whatever their number
+// (up to kWordWin per window).
+constexpr uint32_t kCellLanes = 16;
+constexpr uint32_t kWordWin = 128;
+constexpr uint32_t kWordsPerLane = kWordWin / kCellLanes;
+struct CellSmem {
+    uint32_t words[256 / kCellLanes][kWordWin];
+};
+
+__device__ __forceinline__ uint32_t group_incl_scan(uint32_t v) {
+    const uint32_t sl = threadIdx.x & (kCellLanes - 1u);
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o, 64);
-        if (lane >= (uint32_t)o) v += u;
+    for (uint32_t o = 1; o < kCellLanes; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, kCellLanes);
+        if (sl >= o) v += u;
     }
     return v;
 }
 
-// Single block of 1024: capped counts -> offsets (exclusive scan in cluster
-// order), pool truncation, overflow flags.  Tiles of 4096 counts, read
-// coalesced, four per thread.
+// Calls f(list, n) for each window of up to kWordWin non-empty word indices of
+// the cluster, in ascending order, uniformly over the cluster's 16 lanes (SW
+// is uniform over them).  clear: zero the summary words afterwards.
+template <class F>
+__device__ __forceinline__ void for_nonempty_words(uint32_t* __restrict__ summary_cell, uint32_t SW,
+                                                   bool on, bool clear, uint32_t* list, F&& f) {
+    const uint32_t sl = threadIdx.x & (kCellLanes - 1u);
+    for (uint32_t s0 = 0; s0 < SW; s0 += kCellLanes) {
+        const uint32_t j = s0 + sl;
+        const uint32_t sw = (on && j < SW) ? summary_cell[j] : 0u;
+        const uint32_t c = (uint32_t)__popc(sw);
+        const uint32_t incl = group_incl_scan(c);
+        const uint32_t T = __shfl(incl, kCellLanes - 1u, kCellLanes);
+        const uint32_t P = incl - c;
+        for (uint32_t wb = 0; wb < T; wb += kWordWin) {
+            uint32_t k = P;
+            for (uint32_t m = sw; m; m &= m - 1u, ++k)
+                if (k >= wb && k < wb + kWordWin) list[k - wb] = 32u * j + (uint32_t)__builtin_ctz(m);
+            hl_wave_sync();
+            f(list, T - wb < kWordWin ? T - wb : kWordWin);
+            hl_wave_sync();
+        }
+        if (clear && sw) summary_cell[j] = 0u;
+    }
+}
+
+// number of members of a cluster (before the cap), returned to every lane of
+// the cluster's group; lane 0 of the group also stores it in raw[cell]
+__device__ __forceinline__ uint32_t count_cell(uint32_t ncell, uint32_t W, uint32_t SW,
+                                               const uint32_t* __restrict__ bits,
+                                               const uint32_t* __restrict__ summary,
+                                               uint32_t* __restrict__ raw, uint32_t cell,
+                                               CellSmem& sm) {
+    const uint32_t sl = threadIdx.x & (kCellLanes - 1u);
+    const bool on = cell < ncell;
+    const uint32_t* cb = bits + (size_t)(on ? cell : 0u) * W;
+    uint32_t total = 0;
+    for_nonempty_words(const_cast<uint32_t*>(summary) + (size_t)(on ? cell : 0u) * SW, SW, on, false,
+                       sm.words[threadIdx.x / kCellLanes], [&](const uint32_t* list, uint32_t n) {
+                           uint32_t wv[kWordsPerLane];
+#pragma unroll
+                           for (uint32_t q = 0; q < kWordsPerLane; ++q) {
+                               const uint32_t e = kWordsPerLane * sl + q;
+                               wv[q] = cb[list[e < n ? e : 0u]];
+                           }
+#pragma unroll
+                           for (uint32_t q = 0; q < kWordsPerLane; ++q)
+                               if (kWordsPerLane * sl + q < n) total += (uint32_t)__popc(wv[q]);
+                       });
+#pragma unroll
+    for (uint32_t o = kCellLanes / 2; o >= 1; o >>= 1) total += __shfl_xor(total, o, kCellLanes);
+    if (on && sl == 0) raw[cell] = total;
+    return total;
+}
+
+// Per block of 16 clusters: adds each cluster's capped count to the sum of
+// its scan tile (tile_sum null: no cluster), one atomic per run of equal
+// tiles (a block's clusters lie in one tile except at view boundaries of a
+// batch).
+struct TileSumSmem {
+    unsigned long long v[256 / kCellLanes];
+    unsigned long long* p[256 / kCellLanes];
+};
+__device__ __forceinline__ void add_tile_sums(TileSumSmem& sm, unsigned long long* tile_sum,
+                                              uint32_t capped) {
+    if ((threadIdx.x & (kCellLanes - 1u)) == 0) {
+        sm.v[threadIdx.x / kCellLanes] = capped;
+        sm.p[threadIdx.x / kCellLanes] = tile_sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long* cur = nullptr;
+        unsigned long long acc = 0;
+        for (uint32_t i = 0; i < 256u / kCellLanes; ++i) {
+            if (sm.p[i] != cur) {
+                if (cur && acc) atomicAdd(cur, acc);
+                cur = sm.p[i];
+                acc = 0;
+            }
+            acc += sm.v[i];
+        }
+        if (cur && acc) atomicAdd(cur, acc);
+    }
+}
+
+// Block per scan tile (256 threads x 4 consecutive clusters): offsets are
+// the exclusive scan of the capped counts in cluster order, from the sums of
+// the preceding tiles plus the scan within the tile; pool truncation and
+// overflow flags (OR-ed into flags_out and, when given, flags_out2).
+constexpr uint32_t kScanVPT = kScanTile / 256u;
 struct ScanSmem {
-    unsigned long long wave[16];
-    unsigned long long carry;
+    unsigned long long wave[4];
+    unsigned long long red[4];
     uint32_t flags;
 };
 
-__device__ __forceinline__ void scan_cells(const uint32_t* __restrict__ raw, uint32_t ncell,
-                                           uint32_t cap, uint32_t pool_cap,
-                                           uint32_t* __restrict__ offsets,
-                                           uint32_t* __restrict__ counts,
-                                           uint32_t* __restrict__ flags_out, ScanSmem& sm) {
-    unsigned long long* s_wave = sm.wave;
-    unsigned long long& s_carry = sm.carry;
-    uint32_t& s_flags = sm.flags;
+__device__ __forceinline__ void scan_tile(const uint32_t* __restrict__ raw, uint32_t ncell,
+                                          uint32_t cap, uint32_t pool_cap,
+                                          uint32_t* __restrict__ offsets,
+                                          uint32_t* __restrict__ counts,
+                                          uint32_t* __restrict__ flags_out,
+                                          uint32_t* __restrict__ flags_out2, uint32_t tile,
+                                          ScanSmem& sm) {
+    const unsigned long long* tile_sums =
+        (const unsigned long long*)((const uint8_t*)raw + cluster_tile_sums_offset(ncell));
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    if (t == 0) {
-        s_carry = 0;
-        s_flags = 0;
+    if (t == 0) sm.flags = 0;
+    // sums of the preceding tiles
+    unsigned long long before = 0;
+    for (uint32_t j = t; j < tile; j += 256u) before += tile_sums[j];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) before += __shfl_xor(before, o, 64);
+    if (lane == 0) sm.red[wave] = before;
+    const uint32_t c0 = tile * kScanTile + kScanVPT * t;
+    uint32_t v[kScanVPT];
+    if (c0 + kScanVPT <= ncell) {  // raw is 256-B aligned, c0 a multiple of 4
+#pragma unroll
+        for (uint32_t q = 0; q < kScanVPT; q += 4) {
+            const uint4 u = *reinterpret_cast<const uint4*>(raw + c0 + q);
+            v[q] = u.x;
+            v[q + 1] = u.y;
+            v[q + 2] = u.z;
+            v[q + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t q = 0; q < kScanVPT; ++q) v[q] = c0 + q < ncell ? raw[c0 + q] : 0u;
     }
-    __syncthreads();
     uint32_t fl = 0;
-    for (uint32_t start = 0; start < ncell; start += 4096u) {
-        uint32_t v[4];
-        uint32_t local = 0;
+    unsigned long long local = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t c = start + 4u * t + (uint32_t)q;
-            uint32_t n = c < ncell ? raw[c] : 0u;
-            if (cap && n > cap) {
-                n = cap;
-                fl |= HL_OVERFLOW_CAP;
-            }
-            v[q] = n;
-            local += n;
+    for (uint32_t q = 0; q < kScanVPT; ++q) {
+        if (cap && v[q] > cap) {
+            v[q] = cap;
+            fl |= HL_OVERFLOW_CAP;
         }
-        // local <= 4 * 2^24: the wave scan stays in 32 bits; tile sums go to 64
-        const uint32_t incl = wave_incl_scan(local);
-        if (lane == 63) s_wave[wave] = incl;
-        __syncthreads();
-        unsigned long long before = s_carry;
-        for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
-        unsigned long long off = before + (incl - local);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t c = start + 4u * t + (uint32_t)q;
-            if (c < ncell) {
-                const uint32_t o = off > pool_cap ? pool_cap : (uint32_t)off;
-                const uint32_t room = pool_cap - o;
-                uint32_t n = v[q];
-                if (n > room) {
-                    n = room;
-                    fl |= HL_OVERFLOW_POOL;
-                }
-                offsets[c] = o;
-                counts[c] = n;
-            }
-            off += v[q];
-        }
-        __syncthreads();
-        if (t == 0) {
-            unsigned long long tot = 0;
-            for (uint32_t w = 0; w < 16u; ++w) tot += s_wave[w];
-            s_carry += tot;
-        }
-        __syncthreads();
+        local += v[q];
     }
-    if (fl) atomicOr(&s_flags, fl);
+    unsigned long long incl = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long u = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o) incl += u;
+    }
+    if (lane == 63) sm.wave[wave] = incl;
     __syncthreads();
-    if (t == 0) flags_out[0] |= s_flags;
+    unsigned long long off = sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3] + (incl - local);
+    for (uint32_t w = 0; w < wave; ++w) off += sm.wave[w];
+#pragma unroll
+    for (uint32_t q = 0; q < kScanVPT; ++q) {
+        const uint32_t c = c0 + q;
+        if (c < ncell) {
+            const uint32_t o = off > pool_cap ? pool_cap : (uint32_t)off;
+            const uint32_t room = pool_cap - o;
+            uint32_t n = v[q];
+            if (n > room) {
+                n = room;
+                fl |= HL_OVERFLOW_POOL;
+            }
+            offsets[c] = o;
+            counts[c] = n;
+        }
+        off += v[q];
+    }
+    if (fl) atomicOr(&sm.flags, fl);
+    __syncthreads();
+    if (t == 0 && sm.flags) {
+        atomicOr(flags_out, sm.flags);
+        if (flags_out2) atomicOr(flags_out2, sm.flags);
+    }
 }
 
-// one wavefront per cluster: write the members in ascending order up to the
-// (capped, pool-truncated) count, and clear the cluster's bitmask words.
+// members of a cluster in ascending order up to its (capped, pool-truncated)
+// count; clears the cluster's bitmask words for the next call
 __device__ __forceinline__ void fill_cell(uint32_t ncell, uint32_t W, uint32_t SW,
                                           uint32_t* __restrict__ bits,
                                           uint32_t* __restrict__ summary,
                                           const uint32_t* __restrict__ offsets,
                                           const uint32_t* __restrict__ counts,
-                                          uint32_t* __restrict__ pool, uint32_t cell) {
-    if (cell >= ncell) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t cnt = counts[cell];
-    uint32_t* dst = pool + offsets[cell];
-    uint32_t* cb = bits + (size_t)cell * W;
+                                          uint32_t* __restrict__ pool, uint32_t cell, CellSmem& sm) {
+    const uint32_t sl = threadIdx.x & (kCellLanes - 1u);
+    const bool on = cell < ncell;
+    const uint32_t cnt = on ? counts[cell] : 0u;
+    uint32_t* dst = on ? pool + offsets[cell] : pool;
+    uint32_t* cb = bits + (size_t)(on ? cell : 0u) * W;
     uint32_t written = 0;
-    for (uint32_t s0 = 0; s0 < SW; s0 += 64u) {
-        const uint32_t j = s0 + lane;
-        const uint32_t sw0 = j < SW ? summary[(size_t)cell * SW + j] : 0u;
-        uint32_t lt = 0;
-        for (uint32_t sw = sw0; sw; sw &= sw - 1u)
-            lt += (uint32_t)__popc(cb[32u * j + (uint32_t)__builtin_ctz(sw)]);
-        const uint32_t incl = wave_incl_scan(lt);
-        uint32_t pos = written + incl - lt;
-        for (uint32_t sw = sw0; sw; sw &= sw - 1u) {
-            const uint32_t w = 32u * j + (uint32_t)__builtin_ctz(sw);
-            uint32_t word = cb[w];
-            cb[w] = 0u;
-            while (word) {
-                const uint32_t b = (uint32_t)__builtin_ctz(word);
-                word &= word - 1u;
-                if (pos < cnt) dst[pos] = 32u * w + b;
-                ++pos;
-            }
-        }
-        if (sw0) summary[(size_t)cell * SW + j] = 0u;
-        written += hl_readlane(incl, 63);
-    }
+    for_nonempty_words(summary + (size_t)(on ? cell : 0u) * SW, SW, on, true,
+                       sm.words[threadIdx.x / kCellLanes], [&](const uint32_t* list, uint32_t n) {
+                           uint32_t wi[kWordsPerLane], wv[kWordsPerLane];
+#pragma unroll
+                           for (uint32_t q = 0; q < kWordsPerLane; ++q) {
+                               const uint32_t e = kWordsPerLane * sl + q;
+                               wi[q] = list[e < n ? e : 0u];
+                               wv[q] = cb[wi[q]];
+                           }
+                           uint32_t lt = 0;
+#pragma unroll
+                           for (uint32_t q = 0; q < kWordsPerLane; ++q) {
+                               if (kWordsPerLane * sl + q >= n) wv[q] = 0u;
+                               lt += (uint32_t)__popc(wv[q]);
+                           }
+                           const uint32_t incl = group_incl_scan(lt);
+                           uint32_t pos = written + incl - lt;
+#pragma unroll
+                           for (uint32_t q = 0; q < kWordsPerLane; ++q) {
+                               if (kWordsPerLane * sl + q >= n) continue;
+                               cb[wi[q]] = 0u;
+                               for (uint32_t m = wv[q]; m; m &= m - 1u, ++pos)
+                                   if (pos < cnt) dst[pos] = 32u * wi[q] + (uint32_t)__builtin_ctz(m);
+                           }
+                           written += __shfl(incl, kCellLanes - 1u, kCellLanes);
+                       });
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
@@ -466,11 +642,25 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
 }
 
 // ------------------------------------------------------ one view per launch
+// The geometry kernel also resets the mark queue and the overflow words (it
+// runs before every kernel that reads them).
 __global__ __launch_bounds__(256) void k_cluster_geom(ClusterArgs a,
                                                       const GpuCameraUniforms* __restrict__ cam,
                                                       ColumnGeom* __restrict__ geom,
-                                                      CellBox* __restrict__ boxes) {
-    geom_cell(a, cam, geom, boxes, blockIdx.x * blockDim.x + threadIdx.x);
+                                                      CellBox* __restrict__ boxes,
+                                                      uint32_t* __restrict__ qcount,
+                                                      uint32_t* __restrict__ flags,
+                                                      uint32_t* __restrict__ flags2,
+                                                      unsigned long long* __restrict__ tile_sums,
+                                                      uint32_t ntiles) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < kMarkQueues) qcount[g * kMarkCountStride] = 0u;
+    if (g == 0) {
+        *flags = 0u;
+        if (flags2) *flags2 = 0u;
+    }
+    if (g < ntiles) tile_sums[g] = 0ull;
+    geom_cell(a, cam, geom, boxes, g);
 }
 
 __global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
@@ -480,25 +670,57 @@ __global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
                                                       const CellBox* __restrict__ boxes,
                                                       uint32_t* __restrict__ bits,
                                                       uint32_t* __restrict__ summary, uint32_t W,
-                                                      uint32_t SW) {
-    mark_light(a, cam, cl, geom, boxes, bits, summary, W, SW, blockIdx.x * 4u + (threadIdx.x >> 6));
+                                                      uint32_t SW, MarkQueue q) {
+    const uint32_t li = blockIdx.x * 4u + (threadIdx.x >> 6);
+    mark_light(a, cam, cl, geom, boxes, bits, summary, W, SW, li, q, li);
+}
+
+// Queued chunks over a fixed grid of wavefronts: nw / kMarkQueues waves per
+// sub-queue stride over its valid entries, min(count, seg), then exit.
+__global__ __launch_bounds__(256) void k_cluster_mark_tail(ClusterArgs a,
+                                                           const CullLight* __restrict__ cl,
+                                                           const ColumnGeom* __restrict__ geom,
+                                                           const CellBox* __restrict__ boxes,
+                                                           uint32_t* __restrict__ bits,
+                                                           uint32_t* __restrict__ summary,
+                                                           uint32_t W, uint32_t SW, MarkQueue q) {
+    const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6), nw = gridDim.x * 4u;
+    const uint32_t sub = w % kMarkQueues, step = nw / kMarkQueues;  // nw: a multiple of kMarkQueues
+    const uint32_t n = min(hl_uniform(q.count[sub * kMarkCountStride]), q.seg);
+    const uint2* seg = q.entries + (size_t)sub * q.seg;
+    const float sp = hl_lane() < a.grid_z ? a.slice_r_pad[hl_lane()] : 0.0f;
+    for (uint32_t e = w / kMarkQueues; e < n; e += step) {
+        const uint2 en = seg[e];
+        const uint32_t li = hl_uniform(en.x), chunk = hl_uniform(en.y);
+        const CullLight L = load_light_uniform(cl, li);
+        const uint4 pb = q.bounds[li];
+        const MarkBox m = unpack_box(make_uint4(hl_uniform(pb.x), hl_uniform(pb.y), hl_uniform(pb.z),
+                                                hl_uniform(pb.w)));
+        mark_chunk(a, L, m, li, sp, geom, boxes, bits, summary, W, SW, chunk);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_cluster_count(uint32_t ncell, uint32_t W, uint32_t SW,
                                                        const uint32_t* __restrict__ bits,
                                                        const uint32_t* __restrict__ summary,
-                                                       uint32_t* __restrict__ raw) {
-    count_cell(ncell, W, SW, bits, summary, raw, blockIdx.x * 4u + (threadIdx.x >> 6));
+                                                       uint32_t* __restrict__ raw, uint32_t cap) {
+    __shared__ TileSumSmem sm;
+    __shared__ CellSmem cs;
+    const uint32_t cell = (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes;
+    const uint32_t n = count_cell(ncell, W, SW, bits, summary, raw, cell, cs);
+    unsigned long long* ts = (unsigned long long*)((uint8_t*)raw + cluster_tile_sums_offset(ncell));
+    add_tile_sums(sm, cell < ncell ? ts + cell / kScanTile : nullptr, (cap && n > cap) ? cap : n);
 }
 
-__global__ __launch_bounds__(1024) void k_cluster_scan(const uint32_t* __restrict__ raw,
-                                                       uint32_t ncell, uint32_t cap,
-                                                       uint32_t pool_cap,
-                                                       uint32_t* __restrict__ offsets,
-                                                       uint32_t* __restrict__ counts,
-                                                       uint32_t* __restrict__ flags_out) {
+__global__ __launch_bounds__(256) void k_cluster_scan(const uint32_t* __restrict__ raw,
+                                                      uint32_t ncell, uint32_t cap,
+                                                      uint32_t pool_cap,
+                                                      uint32_t* __restrict__ offsets,
+                                                      uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ flags_out,
+                                                      uint32_t* __restrict__ flags_out2) {
     __shared__ ScanSmem sm;
-    scan_cells(raw, ncell, cap, pool_cap, offsets, counts, flags_out, sm);
+    scan_tile(raw, ncell, cap, pool_cap, offsets, counts, flags_out, flags_out2, blockIdx.x, sm);
 }
 
 __global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t W, uint32_t SW,
@@ -507,7 +729,9 @@ __global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t W
                                                       const uint32_t* __restrict__ offsets,
                                                       const uint32_t* __restrict__ counts,
                                                       uint32_t* __restrict__ pool) {
-    fill_cell(ncell, W, SW, bits, summary, offsets, counts, pool, blockIdx.x * 4u + (threadIdx.x >> 6));
+    __shared__ CellSmem cs;
+    fill_cell(ncell, W, SW, bits, summary, offsets, counts, pool,
+              (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes, cs);
 }
 
 // ------------------------------------------------- all views in one launch
@@ -540,71 +764,123 @@ __global__ __launch_bounds__(256) void k_batch_prep(const GpuLight* __restrict__
 
 __global__ __launch_bounds__(256) void k_batch_geom(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ cell_base,
-                                                    uint32_t nviews) {
+                                                    uint32_t nviews, uint32_t* __restrict__ qcount) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < kMarkQueues) qcount[g * kMarkCountStride] = 0u;
     if (g >= cell_base[nviews]) return;
     const uint32_t vi = find_view(cell_base, nviews, g);
     const BatchCullView& v = views[vi];
-    geom_cell(v.ca, v.cam, (ColumnGeom*)v.geom, view_boxes(v), g - cell_base[vi]);
+    const uint32_t c = g - cell_base[vi];
+    if (c < cluster_scan_tiles_dev(v.ncell))
+        ((unsigned long long*)((uint8_t*)v.raw + cluster_tile_sums_offset(v.ncell)))[c] = 0ull;
+    geom_cell(v.ca, v.cam, (ColumnGeom*)v.geom, view_boxes(v), c);
 }
 
+// one wavefront per light of any view; queue entries carry the global index
 __global__ __launch_bounds__(256) void k_batch_mark(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ light_base,
-                                                    uint32_t nviews) {
+                                                    uint32_t nviews, MarkQueue q) {
     const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (g >= light_base[nviews]) return;
     const uint32_t vi = find_view(light_base, nviews, g);
     const BatchCullView& v = views[vi];
     mark_light(v.ca, v.cam, v.cull, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary,
-               v.W, v.SW, g - light_base[vi]);
+               v.W, v.SW, g - light_base[vi], q, g);
+}
+
+__global__ __launch_bounds__(256) void k_batch_mark_tail(const BatchCullView* __restrict__ views,
+                                                         const uint32_t* __restrict__ light_base,
+                                                         uint32_t nviews, MarkQueue q) {
+    const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6), nw = gridDim.x * 4u;
+    const uint32_t sub = w % kMarkQueues, step = nw / kMarkQueues;  // nw: a multiple of kMarkQueues
+    const uint32_t n = min(hl_uniform(q.count[sub * kMarkCountStride]), q.seg);
+    const uint2* seg = q.entries + (size_t)sub * q.seg;
+    for (uint32_t e = w / kMarkQueues; e < n; e += step) {
+        const uint2 en = seg[e];
+        const uint32_t g = hl_uniform(en.x), chunk = hl_uniform(en.y);
+        const uint32_t vi = hl_uniform(find_view(light_base, nviews, g));
+        const BatchCullView& v = views[vi];
+        const uint32_t li = g - light_base[vi];
+        const CullLight L = load_light_uniform(v.cull, li);
+        const float sp = hl_lane() < v.ca.grid_z ? v.ca.slice_r_pad[hl_lane()] : 0.0f;
+        const uint4 pb = q.bounds[g];
+        const MarkBox m = unpack_box(make_uint4(hl_uniform(pb.x), hl_uniform(pb.y), hl_uniform(pb.z),
+                                                hl_uniform(pb.w)));
+        mark_chunk(v.ca, L, m, li, sp, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary,
+                   v.W, v.SW, chunk);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_batch_count(const BatchCullView* __restrict__ views,
                                                      const uint32_t* __restrict__ cell_base,
                                                      uint32_t nviews) {
-    const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (g >= cell_base[nviews]) return;
-    const uint32_t vi = find_view(cell_base, nviews, g);
+    __shared__ TileSumSmem sm;
+    __shared__ CellSmem cs;
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes;
+    const bool on = g < cell_base[nviews];
+    const uint32_t vi = find_view(cell_base, nviews, on ? g : cell_base[nviews] - 1u);
     const BatchCullView& v = views[vi];
-    count_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.raw, g - cell_base[vi]);
+    const uint32_t c = g - cell_base[vi];
+    const uint32_t n = count_cell(on ? v.ncell : 0u, v.W, v.SW, v.bits, v.summary, v.raw, c, cs);
+    unsigned long long* ts = (unsigned long long*)((uint8_t*)v.raw + cluster_tile_sums_offset(v.ncell));
+    const uint32_t cap = v.ca.cap;
+    add_tile_sums(sm, on ? ts + c / kScanTile : nullptr, (cap && n > cap) ? cap : n);
 }
 
-// one block per view
-__global__ __launch_bounds__(1024) void k_batch_scan(const BatchCullView* __restrict__ views) {
+// one block per scan tile of any view
+__global__ __launch_bounds__(256) void k_batch_scan(const BatchCullView* __restrict__ views,
+                                                    const uint32_t* __restrict__ tile_base,
+                                                    uint32_t nviews) {
     __shared__ ScanSmem sm;
-    const BatchCullView& v = views[blockIdx.x];
-    scan_cells(v.raw, v.ncell, v.ca.cap, v.pool_cap, v.offsets, v.counts, v.flags, sm);
+    const uint32_t vi = find_view(tile_base, nviews, blockIdx.x);
+    const BatchCullView& v = views[vi];
+    scan_tile(v.raw, v.ncell, v.ca.cap, v.pool_cap, v.offsets, v.counts, v.flags, nullptr,
+              blockIdx.x - tile_base[vi], sm);
 }
 
 __global__ __launch_bounds__(256) void k_batch_fill(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ cell_base,
                                                     uint32_t nviews) {
-    const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (g >= cell_base[nviews]) return;
-    const uint32_t vi = find_view(cell_base, nviews, g);
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes;
+    const bool on = g < cell_base[nviews];
+    const uint32_t vi = find_view(cell_base, nviews, on ? g : cell_base[nviews] - 1u);
     const BatchCullView& v = views[vi];
-    fill_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.offsets, v.counts, v.pool, g - cell_base[vi]);
+    __shared__ CellSmem cs;
+    fill_cell(on ? v.ncell : 0u, v.W, v.SW, v.bits, v.summary, v.offsets, v.counts, v.pool,
+              g - cell_base[vi], cs);
+}
+
+// grid of the queued-chunk launches: enough wavefronts to spread the tail
+// of a few heavy lights over every CU, few enough to exit fast when idle
+static uint32_t tail_blocks(uint32_t lights) {
+    constexpr uint32_t q = kMarkQueues / 4u;  // blocks per round over the sub-queues
+    uint32_t b = (lights / 16u + q - 1u) / q * q;
+    return b < q ? q : (b > 1024u ? 1024u : b);
 }
 
 hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCullView* views,
-                             const uint32_t* light_base, const uint32_t* cell_base, uint32_t nviews,
-                             uint32_t total_lights, uint32_t total_cells) {
+                             const uint32_t* light_base, const uint32_t* cell_base,
+                             const uint32_t* tile_base, uint32_t nviews, uint32_t total_lights,
+                             uint32_t total_cells, uint32_t total_tiles, const MarkQueue& mq) {
     hipError_t e;
     if (total_lights) {
         k_batch_prep<<<(total_lights + 255) / 256, 256, 0, s>>>(pool, views, light_base, nviews);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    k_batch_geom<<<(total_cells + 255) / 256, 256, 0, s>>>(views, cell_base, nviews);
+    k_batch_geom<<<(total_cells + 255) / 256, 256, 0, s>>>(views, cell_base, nviews, mq.count);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (total_lights) {
-        k_batch_mark<<<(total_lights + 3) / 4, 256, 0, s>>>(views, light_base, nviews);
+        k_batch_mark<<<(total_lights + 3) / 4, 256, 0, s>>>(views, light_base, nviews, mq);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        k_batch_mark_tail<<<tail_blocks(total_lights), 256, 0, s>>>(views, light_base, nviews, mq);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    k_batch_count<<<(total_cells + 3) / 4, 256, 0, s>>>(views, cell_base, nviews);
+    const uint32_t cblocks = (total_cells + 256u / kCellLanes - 1u) / (256u / kCellLanes);
+    k_batch_count<<<cblocks, 256, 0, s>>>(views, cell_base, nviews);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_scan<<<nviews, 1024, 0, s>>>(views);
+    k_batch_scan<<<total_tiles, 256, 0, s>>>(views, tile_base, nviews);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_fill<<<(total_cells + 3) / 4, 256, 0, s>>>(views, cell_base, nviews);
+    k_batch_fill<<<cblocks, 256, 0, s>>>(views, cell_base, nviews);
     return hipGetLastError();
 }
 
@@ -641,24 +917,30 @@ size_t cluster_geom_bytes(uint32_t ncol, uint32_t ncell) {
 hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCameraUniforms* cam,
                                const CullLight* cl, const ClusterScratch& w, uint32_t* offsets,
                                uint32_t* counts, uint32_t* pool, uint32_t pool_cap,
-                               uint32_t* flags) {
+                               uint32_t* flags, uint32_t* flags2) {
     const uint32_t ncol = a.grid_x * a.grid_y;
     const uint32_t ncell = ncol * a.grid_z;
     const uint32_t W = (a.num_lights + 31u) / 32u, SW = (W + 31u) / 32u;
     ColumnGeom* geom = (ColumnGeom*)w.geom;
     CellBox* boxes = (CellBox*)((uint8_t*)w.geom + cluster_geom_offset_boxes(ncol));
-    k_cluster_geom<<<(ncell + 255) / 256, 256, 0, s>>>(a, cam, geom, boxes);
+    const uint32_t ntiles = cluster_scan_tiles(ncell);
+    unsigned long long* tile_sums = (unsigned long long*)((uint8_t*)w.raw + cluster_tile_sums_offset(ncell));
+    k_cluster_geom<<<(ncell + 255) / 256, 256, 0, s>>>(a, cam, geom, boxes, w.mq.count, flags, flags2,
+                                                       tile_sums, ntiles);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.num_lights) {
         k_cluster_mark<<<(a.num_lights + 3) / 4, 256, 0, s>>>(a, cam, cl, geom, boxes, w.bits,
-                                                                  w.summary, W, SW);
+                                                                  w.summary, W, SW, w.mq);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        k_cluster_mark_tail<<<tail_blocks(a.num_lights), 256, 0, s>>>(a, cl, geom, boxes, w.bits,
+                                                                       w.summary, W, SW, w.mq);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    const uint32_t blocks = (ncell + 3) / 4;
-    k_cluster_count<<<blocks, 256, 0, s>>>(ncell, W, SW, w.bits, w.summary, w.raw);
+    const uint32_t blocks = (ncell + 256u / kCellLanes - 1u) / (256u / kCellLanes);
+    k_cluster_count<<<blocks, 256, 0, s>>>(ncell, W, SW, w.bits, w.summary, w.raw, a.cap);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_cluster_scan<<<1, 1024, 0, s>>>(w.raw, ncell, a.cap, pool_cap, offsets, counts, flags);
+    k_cluster_scan<<<ntiles, 256, 0, s>>>(w.raw, ncell, a.cap, pool_cap, offsets, counts, flags, flags2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_cluster_fill<<<blocks, 256, 0, s>>>(ncell, W, SW, w.bits, w.summary, offsets, counts, pool);
     return hipGetLastError();

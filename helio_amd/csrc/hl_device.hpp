@@ -28,7 +28,9 @@ namespace hl {
 //   p0 = (position.xyz, range)          range = +inf for directional lights
 //   p1 = (color.rgb * intensity, cos inner)
 //   p2 = (direction.xyz, cos outer)     cos outer = -2 for non-spot lights
-//   p3 = (color.rgb, intensity)
+//   p3 = (1 / range^2, intensity, cone margin, 0)
+// The cone margin, 1e-5 * |direction|_1, bounds (tenfold) the difference
+// between a fast cos(angle) and the exact one of deferred_lighting.wgsl:739.
 struct alignas(16) HotLight {
     float4 p0, p1, p2, p3;
 };
@@ -210,6 +212,15 @@ __device__ __forceinline__ float hl_f_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ float hl_f_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float hl_f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// floor(x / c) for x < 2^21, c >= 1, in four VALU operations instead of an
+// integer division: (x + 0.5) / c is at least 0.5 / c away from an integer,
+// and the estimate (reciprocal within 1 ulp, one fma rounding) is within
+// 1.8e-7 relative, i.e. within (x + 0.5) * 1.8e-7 / c < 0.5 / c.
+__device__ __forceinline__ uint32_t hl_floor_div(uint32_t x, uint32_t c) {
+    const float ic = hl_f_rcp((float)c);
+    return (uint32_t)__builtin_fmaf((float)x, ic, 0.5f * ic);
+}
+
 // ----------------------------------------------------------- texel decode
 __device__ __forceinline__ float hl_half(uint16_t h) {
     _Float16 v;
@@ -217,14 +228,14 @@ __device__ __forceinline__ float hl_half(uint16_t h) {
     return (float)v;
 }
 
-// v / 255 exactly: one multiply by the rounded reciprocal and one residual
-// correction (Markstein); equal to the IEEE quotient for every v in 0..255.
+// v / 255 exactly: v * (1/255) with the reciprocal split into its float part
+// and the float of the remainder, fma(v, hi, v * lo); equal to the IEEE
+// quotient for every v in 0..255 (checked in exact rational arithmetic).
 __device__ __forceinline__ float hl_unorm8(uint32_t v) {
-    const float r = 1.0f / 255.0f;
-    float fv = (float)v;
-    float q = fv * r;
-    float e = __builtin_fmaf(-q, 255.0f, fv);
-    return __builtin_fmaf(e, r, q);
+    const float hi = 0.003921568859368563f;     // float(1/255)
+    const float lo = -2.319175823606301e-10f;   // float(1/255 - hi)
+    const float fv = (float)v;
+    return __builtin_fmaf(fv, hi, fv * lo);
 }
 
 typedef unsigned int hl_u32x2 __attribute__((ext_vector_type(2)));
@@ -312,29 +323,56 @@ __device__ __forceinline__ float hl_dpp(float old, float src) {
                                                        CTRL, ROW_MASK, 0xF, false));
 }
 
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int hl_dpp_i(int old, int src) {
+    return __builtin_amdgcn_update_dpp(old, src, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Order-preserving map of non-NaN floats onto signed integers (and back: the
+// map is an involution): positive floats keep their bits, negative floats
+// flip their magnitude bits.  -0 maps below +0.
+__device__ __forceinline__ int hl_fkey(float x) {
+    const int b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float hl_funkey(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+
 // Full-wave reductions: quad swaps, half-row and row mirrors, then the
 // row_bcast:15 / row_bcast:31 steps; lane 63 holds the result, which is read
-// back as a wave-uniform value.  All 64 lanes must be active.
-__device__ __forceinline__ float dpp_min(float v) {
-    const float I = __builtin_inff();
-    v = fminf(v, hl_dpp<0xB1>(I, v));
-    v = fminf(v, hl_dpp<0x4E>(I, v));
-    v = fminf(v, hl_dpp<0x141>(I, v));
-    v = fminf(v, hl_dpp<0x140>(I, v));
-    v = fminf(v, hl_dpp<0x142, 0xA>(I, v));
-    v = fminf(v, hl_dpp<0x143, 0xC>(I, v));
-    return hl_readlane(v, 63);
+// back as a wave-uniform value.  All 64 lanes must be active.  Integer min/max
+// with their identity as the DPP "old" operand fold each step into one
+// v_min_i32_dpp / v_max_i32_dpp (float min/max would add a move, a NaN
+// quieting max and hazard nops per step).
+__device__ __forceinline__ int dpp_min_key(int v) {
+    constexpr int I = 0x7fffffff;
+    v = min(v, hl_dpp_i<0xB1>(I, v));
+    v = min(v, hl_dpp_i<0x4E>(I, v));
+    v = min(v, hl_dpp_i<0x141>(I, v));
+    v = min(v, hl_dpp_i<0x140>(I, v));
+    v = min(v, hl_dpp_i<0x142, 0xA>(I, v));
+    v = min(v, hl_dpp_i<0x143, 0xC>(I, v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
-__device__ __forceinline__ float dpp_max(float v) {
-    const float I = -__builtin_inff();
-    v = fmaxf(v, hl_dpp<0xB1>(I, v));
-    v = fmaxf(v, hl_dpp<0x4E>(I, v));
-    v = fmaxf(v, hl_dpp<0x141>(I, v));
-    v = fmaxf(v, hl_dpp<0x140>(I, v));
-    v = fmaxf(v, hl_dpp<0x142, 0xA>(I, v));
-    v = fmaxf(v, hl_dpp<0x143, 0xC>(I, v));
-    return hl_readlane(v, 63);
+__device__ __forceinline__ int dpp_max_key(int v) {
+    constexpr int I = (int)0x80000000;
+    v = max(v, hl_dpp_i<0xB1>(I, v));
+    v = max(v, hl_dpp_i<0x4E>(I, v));
+    v = max(v, hl_dpp_i<0x141>(I, v));
+    v = max(v, hl_dpp_i<0x140>(I, v));
+    v = max(v, hl_dpp_i<0x142, 0xA>(I, v));
+    v = max(v, hl_dpp_i<0x143, 0xC>(I, v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
+// min / max over the lanes with `take` set (+inf / -inf when none); inputs
+// must not be NaN.
+__device__ __forceinline__ float dpp_min_if(bool take, float v) {
+    return hl_funkey(dpp_min_key(take ? hl_fkey(v) : 0x7fffffff));
+}
+__device__ __forceinline__ float dpp_max_if(bool take, float v) {
+    return hl_funkey(dpp_max_key(take ? hl_fkey(v) : (int)0x80000000));
+}
+__device__ __forceinline__ float dpp_min(float v) { return dpp_min_if(true, v); }
+__device__ __forceinline__ float dpp_max(float v) { return dpp_max_if(true, v); }
 __device__ __forceinline__ float dpp_sum(float v) {
     v = v + hl_dpp<0xB1>(0.0f, v);
     v = v + hl_dpp<0x4E>(0.0f, v);
@@ -344,6 +382,11 @@ __device__ __forceinline__ float dpp_sum(float v) {
     v = v + hl_dpp<0x143, 0xC>(0.0f, v);
     return hl_readlane(v, 63);
 }
+
+// Placed in a rarely taken block, keeps the optimizer from executing the block
+// unconditionally and selecting its result (an empty volatile asm statement
+// has side effects, so the block cannot be speculated).
+__device__ __forceinline__ void hl_rare() { __asm__ volatile(""); }
 
 // Wave-local ordering point between LDS writes and reads of other lanes.
 __device__ __forceinline__ void hl_wave_sync() {

@@ -33,7 +33,7 @@ struct hl_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf hot, cull, geom, bits, summary, raw, words, accum;
+    DevBuf hot, cull, geom, bits, summary, raw, markq, words, accum;
     // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
 };
 
@@ -68,11 +68,45 @@ hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const 
 // Scratch of the clustered cull.  bits (ncell * ceil(L/32) words) and summary
 // (ncell * ceil(L/1024) words) must be zero before the first call; every call
 // leaves them zero again.
+// Work queue of the cluster mark.  A light's candidate cells are enumerated
+// in chunks of 64; the wave of a light marks its first kMarkInline chunks and
+// queues the rest, which a second launch spreads over all CUs (a light close
+// to the camera can have thousands of candidate cells).  The queue is split
+// into kMarkQueues sub-queues (light i uses sub-queue i % kMarkQueues), each
+// with its own counter on its own 128-B line, so that the claims of
+// thousands of lights do not serialize on one address.
+constexpr uint32_t kMarkInline = 2;
+constexpr uint32_t kMarkQueues = 64;
+constexpr uint32_t kMarkCountStride = 32;  // u32 words between counters
+struct MarkQueue {
+    uint32_t* count;  // kMarkQueues counters, reset by the geometry kernel (may exceed seg)
+    uint4* bounds;    // per light: x0, y0, s0 << 16 | nz, nx << 16 | ny
+    uint2* entries;   // kMarkQueues segments of seg entries (light, chunk)
+    uint32_t seg;
+};
+inline uint32_t mark_queue_seg(uint32_t num_lights) {
+    return (4u * num_lights + 4096u + kMarkQueues - 1u) / kMarkQueues;
+}
+inline size_t mark_queue_bytes(uint32_t num_lights) {
+    return 4 * (size_t)kMarkQueues * kMarkCountStride + 16 * (size_t)num_lights +
+           8 * (size_t)kMarkQueues * mark_queue_seg(num_lights);
+}
+inline MarkQueue mark_queue_at(void* base, uint32_t num_lights) {
+    MarkQueue q;
+    const size_t counts = 4 * (size_t)kMarkQueues * kMarkCountStride;
+    q.count = (uint32_t*)base;
+    q.bounds = (uint4*)((uint8_t*)base + counts);
+    q.entries = (uint2*)((uint8_t*)base + counts + 16 * (size_t)num_lights);
+    q.seg = mark_queue_seg(num_lights);
+    return q;
+}
+
 struct ClusterScratch {
     void* geom;          // cluster_geom_bytes(columns, cells)
     uint32_t* bits;
     uint32_t* summary;
-    uint32_t* raw;       // ncell
+    uint32_t* raw;       // cluster_raw_bytes(ncell)
+    MarkQueue mq;        // mark_queue_bytes(lights)
 };
 size_t cluster_geom_bytes(uint32_t ncol, uint32_t ncell);
 size_t cluster_geom_offset_boxes(uint32_t ncol);
@@ -81,6 +115,19 @@ inline size_t cluster_bits_words(uint32_t ncell, uint32_t num_lights) {
 }
 inline size_t cluster_summary_words(uint32_t ncell, uint32_t num_lights) {
     return (size_t)ncell * (((num_lights + 31u) / 32u + 31u) / 32u);
+}
+// Raw per-cluster counts, then (256-B aligned) one 64-bit sum of capped
+// counts per scan tile of kScanTile clusters.
+constexpr uint32_t kScanTile = 1024;
+__host__ __device__ inline uint32_t cluster_scan_tiles_dev(uint32_t ncell) {
+    return (ncell + kScanTile - 1u) / kScanTile;
+}
+inline uint32_t cluster_scan_tiles(uint32_t ncell) { return cluster_scan_tiles_dev(ncell); }
+__host__ __device__ inline size_t cluster_tile_sums_offset(uint32_t ncell) {
+    return (4 * (size_t)ncell + 255) & ~(size_t)255;
+}
+inline size_t cluster_raw_bytes(uint32_t ncell) {
+    return cluster_tile_sums_offset(ncell) + 8 * (size_t)cluster_scan_tiles(ncell);
 }
 // One view of a batched cull (device memory): its cluster arguments, camera,
 // light subset, and its own per-view buffers.
@@ -93,7 +140,7 @@ struct BatchCullView {
     void* geom;              // cluster_geom_bytes(columns, cells)
     uint32_t* bits;
     uint32_t* summary;
-    uint32_t* raw;
+    uint32_t* raw;           // cluster_raw_bytes(ncell)
     uint32_t* offsets;
     uint32_t* counts;
     uint32_t* pool;
@@ -101,13 +148,14 @@ struct BatchCullView {
     uint32_t pool_cap, W, SW, ncell;
 };
 hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCullView* views,
-                             const uint32_t* light_base, const uint32_t* cell_base, uint32_t nviews,
-                             uint32_t total_lights, uint32_t total_cells);
+                             const uint32_t* light_base, const uint32_t* cell_base,
+                             const uint32_t* tile_base, uint32_t nviews, uint32_t total_lights,
+                             uint32_t total_cells, uint32_t total_tiles, const MarkQueue& mq);
 
 hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCameraUniforms* cam,
                                const CullLight* cl, const ClusterScratch& w, uint32_t* offsets,
                                uint32_t* counts, uint32_t* pool, uint32_t pool_cap,
-                               uint32_t* flags);
+                               uint32_t* flags, uint32_t* flags2);
 
 // launchers (hl_shade.hip)
 enum ShadeMode : int { kModeForward = 0, kModeCount = 1, kModeBackward = 2, kModeFusedL2 = 3 };

@@ -31,6 +31,8 @@ struct Pixel {
     float alb[3];
     float rough, metal, ao;
     float ndv, kk, gv, ev, one_m;
+    float kd[3];     // (1 - metallic) albedo / pi
+    float a2, a2gv;  // roughness^4 and roughness^4 G1(NdV)
     bool f0_free[3];
 };
 
@@ -45,15 +47,28 @@ struct Term {
     float F[3], B[3];
 };
 
+// Range test of deferred_lighting.wgsl:1128-1131 (dist > range rejects) from
+// the fast squared distance; inside the +-1e-5 relative band around range^2,
+// which covers the rounding of both sides many times over, the exact
+// restatement decides.  The band is rare: the exact branch is taken only when
+// some lane of the wave needs it.
+__device__ __forceinline__ bool range_test(float tx, float ty, float tz, float d2, float r) {
+    const float r2 = r * r;
+    bool in = d2 < r2 * (1.0f - 1e-5f);
+    const bool band = !in && !(d2 > r2 * (1.0f + 1e-5f));
+    if (hl_ballot(band)) {
+        hl_rare();
+        if (band) in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > r);
+    }
+    return in;
+}
+
 // Fast arithmetic, exact decisions.  Returns false when the term is exactly
-// zero in the restated shader; in_range reports the range test of
-// deferred_lighting.wgsl:1128-1131.
+// zero in the restated shader.
 template <bool kFull>
-__device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Term& t,
-                                          bool& in_range) {
+__device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Term& t) {
     const bool dir = __builtin_isinf(L.p0.w);
     if (dir) {
-        in_range = true;
         const float dx = -L.p2.x, dy = -L.p2.y, dz = -L.p2.z;
         const float inv = hl_f_rsq(dx * dx + dy * dy + dz * dz);
         t.lx = dx * inv;
@@ -68,14 +83,7 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     } else {
         const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
         const float d2 = tx * tx + ty * ty + tz * tz;
-        const float r = L.p0.w;
-        const float r2 = r * r;
-        bool in;
-        if (d2 < r2 * (1.0f - 1e-5f)) in = true;
-        else if (d2 > r2 * (1.0f + 1e-5f)) in = false;
-        else in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > r);
-        in_range = in;
-        if (!in) return false;
+        if (!range_test(tx, ty, tz, d2, L.p0.w)) return false;
         const float inv_d = hl_f_rsq(d2);
         const float d = d2 * inv_d;
         t.d = d;
@@ -84,27 +92,34 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
         t.ly = ty * inv_d;
         t.lz = tz * inv_d;
         t.att0 = hl_f_rcp(d2 + 0.0001f);
-        const float nd = d * hl_f_rcp(r);
-        const float nd2 = nd * nd;
+        const float nd2 = d2 * L.p3.x;  // (dist / range)^2
         t.fall = fmaxf(0.0f, 1.0f - nd2 * nd2);
         t.s = 1.0f;
         t.dsdcos = 0.0f;
         if (L.p2.w != kNoCone) {
-            // The cone factor divides by cos(inner) - cos(outer) (0.005 for a
-            // 10-degree cone), which would amplify a last-bit error of a fast L
-            // ~200x; it is evaluated exactly as deferred_lighting.wgsl:739-742.
-            float cos_a;
-            {
+            // smoothstep(cos outer, cos inner, cos_a) (deferred_lighting.wgsl:
+            // 739-742) divides by cos(inner) - cos(outer) (0.005 for a 10-degree
+            // penumbra), which would amplify a last-bit error of a fast L ~200x.
+            // Outside the penumbra by more than the margin the factor is exactly
+            // 0 (the term vanishes) or 1; inside, it is evaluated exactly.
+            const float cf = -(t.lx * L.p2.x + t.ly * L.p2.y + t.lz * L.p2.z);
+            const bool ordered = L.p2.w < L.p1.w;
+            if (ordered && cf < L.p2.w - L.p3.z) return false;
+            if (!(ordered && cf > L.p1.w + L.p3.z)) {
+                hl_rare();
+                float cos_a;
+                {
 #pragma clang fp contract(off)
-                const float dist = sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz));
-                const float ex = tx / dist, ey = ty / dist, ez = tz / dist;
-                cos_a = hl_x_dot3(-ex, -ey, -ez, L.p2.x, L.p2.y, L.p2.z);
-            }
-            t.s = hl_x_smoothstep(L.p2.w, L.p1.w, cos_a);
-            if (kFull) {
-                const float inv_w = hl_f_rcp(L.p1.w - L.p2.w);
-                const float tt = (cos_a - L.p2.w) * inv_w;
-                t.dsdcos = (tt > 0.0f && tt < 1.0f) ? 6.0f * tt * (1.0f - tt) * inv_w : 0.0f;
+                    const float dist = sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz));
+                    const float ex = tx / dist, ey = ty / dist, ez = tz / dist;
+                    cos_a = hl_x_dot3(-ex, -ey, -ez, L.p2.x, L.p2.y, L.p2.z);
+                }
+                t.s = hl_x_smoothstep(L.p2.w, L.p1.w, cos_a);
+                if (kFull) {
+                    const float inv_w = hl_f_rcp(L.p1.w - L.p2.w);
+                    const float tt = (cos_a - L.p2.w) * inv_w;
+                    t.dsdcos = (tt > 0.0f && tt < 1.0f) ? 6.0f * tt * (1.0f - tt) * inv_w : 0.0f;
+                }
             }
         }
         const float atten = t.att0 * t.fall * t.s;
@@ -117,6 +132,7 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
         const float err = 8.0f * ci * t.att0 * 4e-6f * t.s + m * 4e-6f + 1e-12f;
         if (m < kThresh - err) return false;
         if (m < kThresh + err) {  // ambiguous: decide with the exact restatement
+            hl_rare();
             bool dummy;
             if (hl_x_radiance_max(L, p.wx, p.wy, p.wz, &dummy) < kThresh) return false;
         }
@@ -133,21 +149,27 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     t.u = fminf(fmaxf(1.0f - t.hdv, 0.0f), 1.0f);
     const float u2 = t.u * t.u;
     t.P = u2 * u2 * t.u;
-    const float a = p.rough * p.rough;
-    t.a2 = a * a;
+    t.a2 = p.a2;
     t.ndh = fmaxf(p.nx * t.hx + p.ny * t.hy + p.nz * t.hz, 0.0f);
-    t.den = t.ndh * t.ndh * (t.a2 - 1.0f) + 1.0f;
+    t.den = t.ndh * t.ndh * (p.a2 - 1.0f) + 1.0f;
     t.Q = kPi * t.den * t.den + 0.0001f;
-    t.D = t.a2 * hl_f_rcp(t.Q);
     t.el = t.ndl * (1.0f - p.kk) + p.kk + 0.0001f;
-    t.gl = t.ndl * hl_f_rcp(t.el);
-    t.G = p.gv * t.gl;
     t.den2 = 4.0f * p.ndv * t.ndl + 0.0001f;
-    const float spec = t.D * t.G * hl_f_rcp(t.den2);
+    float spec;
+    if (kFull) {
+        t.D = p.a2 * hl_f_rcp(t.Q);
+        t.gl = t.ndl * hl_f_rcp(t.el);
+        t.G = p.gv * t.gl;
+        spec = t.D * t.G * hl_f_rcp(t.den2);
+    } else {
+        // D G / den2 with one reciprocal: a2 gv NdL / (Q el den2); for
+        // roughness in [0, 1] each factor lies in [1e-4, 4.1]
+        spec = p.a2gv * t.ndl * hl_f_rcp(t.Q * t.el * t.den2);
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         t.F[c] = p.f0[c] + (1.0f - p.f0[c]) * t.P;
-        t.B[c] = (1.0f - t.F[c]) * p.one_m * p.alb[c] * kInvPi + spec * t.F[c];
+        t.B[c] = p.kd[c] + t.F[c] * (spec - p.kd[c]);  // (1 - F) kd + spec F
     }
     return true;
 }
@@ -172,8 +194,10 @@ __device__ __forceinline__ void adjoint_term(const HotLight& L, const Pixel& p, 
     const float atten = t.att0 * t.fall * t.s;
     float adj_NdL = 0.0f, adj_atten = 0.0f, adj_D = 0.0f, adj_G = 0.0f, adj_den2 = 0.0f, adj_P = 0.0f;
     float dI = 0.0f;
-    const float cI = L.p3.w;
-    const float col[3] = {L.p3.x, L.p3.y, L.p3.z};
+    // color = (color * intensity) / intensity: an active term has radiance
+    // >= 0.002, hence intensity != 0
+    const float inv_I = hl_f_rcp(L.p3.y);
+    const float cI[3] = {L.p1.x, L.p1.y, L.p1.z};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float W = t.rad[c] * t.ndl;
@@ -184,10 +208,10 @@ __device__ __forceinline__ void adjoint_term(const HotLight& L, const Pixel& p, 
         dM += gw * (-(1.0f - t.F[c]) * p.alb[c] * kInvPi +
                     (p.f0_free[c] ? (p.alb[c] - 0.04f) * (1.0f - t.P) * dBdF : 0.0f));
         dR += gw * dSdr * t.F[c];
-        dI += gl[c] * t.B[c] * col[c] * atten * t.ndl;
+        dI += gl[c] * t.B[c] * cI[c] * inv_I * atten * t.ndl;
         const float adj_B = gw;
         adj_NdL += gl[c] * t.B[c] * t.rad[c];
-        adj_atten += gl[c] * t.B[c] * t.ndl * col[c] * cI;
+        adj_atten += gl[c] * t.B[c] * t.ndl * cI[c];
         adj_D += adj_B * t.G * t.F[c] * inv_den2;
         adj_G += adj_B * t.D * t.F[c] * inv_den2;
         adj_den2 -= adj_B * DG_den2 * t.F[c] * inv_den2;
@@ -252,10 +276,9 @@ __device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, u
         const uint32_t ucell = hl_readlane(cell, leader);
         const bool mine = valid && cell == ucell;
         todo &= ~hl_ballot(mine);
-        const float inf = __builtin_inff();
-        const float bx0 = dpp_min(mine ? p.wx : inf), bx1 = dpp_max(mine ? p.wx : -inf);
-        const float by0 = dpp_min(mine ? p.wy : inf), by1 = dpp_max(mine ? p.wy : -inf);
-        const float bz0 = dpp_min(mine ? p.wz : inf), bz1 = dpp_max(mine ? p.wz : -inf);
+        const float bx0 = dpp_min_if(mine, p.wx), bx1 = dpp_max_if(mine, p.wx);
+        const float by0 = dpp_min_if(mine, p.wy), by1 = dpp_max_if(mine, p.wy);
+        const float bz0 = dpp_min_if(mine, p.wz), bz1 = dpp_max_if(mine, p.wz);
         uint32_t cnt = hl_uniform(a.counts[ucell]);
         uint32_t base = GRID == HL_GRID_TILES ? ucell * HL_MAX_LIGHTS_PER_TILE
                                               : hl_uniform(a.offsets[ucell]);
@@ -340,14 +363,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     }
     const GpuCameraUniforms* cam = a.cam;
     {
-        // N, V and NdV are per-pixel and exact: NdV cancels at grazing view
-        // (dot of two unit vectors ~1e-6), where a last-bit difference in the
-        // normalization becomes a percent-level difference in G1(NdV).
+        // world position exactly as deferred_lighting.wgsl:1003-1008: the
+        // range, cone and threshold decisions depend on it
 #pragma clang fp contract(off)
-        const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
-        p.nx = n.x;
-        p.ny = n.y;
-        p.nz = n.z;
         const float u = ((float)x + 0.5f) / (float)a.width;
         const float v = ((float)y + 0.5f) / (float)a.height;
         const float ndx = u * 2.0f - 1.0f, ndy = 1.0f - v * 2.0f;
@@ -356,12 +374,35 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         p.wx = hl_x_mat_row(m, 0, ndx, ndy, depth, 1.0f) / hw;
         p.wy = hl_x_mat_row(m, 1, ndx, ndy, depth, 1.0f) / hw;
         p.wz = hl_x_mat_row(m, 2, ndx, ndy, depth, 1.0f) / hw;
-        const float3 vv = hl_x_normalize(cam->position_near[0] - p.wx, cam->position_near[1] - p.wy,
-                                         cam->position_near[2] - p.wz);
-        p.vx = vv.x;
-        p.vy = vv.y;
-        p.vz = vv.z;
-        p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
+    }
+    {
+        const float ex = cam->position_near[0] - p.wx, ey = cam->position_near[1] - p.wy,
+                    ez = cam->position_near[2] - p.wz;
+        const float in_ = hl_f_rsq(Nr.x * Nr.x + Nr.y * Nr.y + Nr.z * Nr.z);
+        const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
+        p.nx = Nr.x * in_;
+        p.ny = Nr.y * in_;
+        p.nz = Nr.z * in_;
+        p.vx = ex * iv;
+        p.vy = ey * iv;
+        p.vz = ez * iv;
+        p.ndv = fmaxf(p.nx * p.vx + p.ny * p.vy + p.nz * p.vz, 0.0f);
+        // At grazing view NdV is a difference of ~1 terms: a last-bit error of
+        // N or V (< 1e-6 absolute in NdV) is a relative error of 1e-6 / NdV in
+        // G1(NdV).  Below 0.02 N, V and NdV are recomputed exactly.
+        if (valid && p.ndv < 0.02f) {
+#pragma clang fp contract(off)
+            hl_rare();
+            const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
+            const float3 vv = hl_x_normalize(ex, ey, ez);
+            p.nx = n.x;
+            p.ny = n.y;
+            p.nz = n.z;
+            p.vx = vv.x;
+            p.vy = vv.y;
+            p.vz = vv.z;
+            p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
+        }
     }
     p.alb[0] = A.x;
     p.alb[1] = A.y;
@@ -388,6 +429,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         p.kk = r1 * r1 * 0.125f;
         p.ev = p.ndv * (1.0f - p.kk) + p.kk + 0.0001f;
         p.gv = p.ndv * hl_f_rcp(p.ev);
+        const float ra = p.rough * p.rough;
+        p.a2 = ra * ra;
+        p.a2gv = p.a2 * p.gv;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.kd[c] = p.one_m * p.alb[c] * kInvPi;
     }
 
     // ---- debug views (deferred_lighting.wgsl:1032-1038)
@@ -408,7 +454,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     if (GRID == HL_GRID_TILES) {
         cell = (y >> 4) * a.tile_stride_x + (x >> 4);
     } else {
-        const uint32_t cx = x / a.cell_w, cy = y / a.cell_h;
+        const uint32_t cx = hl_floor_div(x, a.cell_w), cy = hl_floor_div(y, a.cell_h);
         uint32_t cz = 0;
         if (valid && a.grid_z > 1u) {
             // guess from the view distance, then settle on the exact thresholds
@@ -437,12 +483,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
                                   return;
                               }
                               const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
-                              const float d2 = tx * tx + ty * ty + tz * tz;
-                              const float r2 = L.p0.w * L.p0.w;
-                              bool in;
-                              if (d2 < r2 * (1.0f - 1e-5f)) in = true;
-                              else if (d2 > r2 * (1.0f + 1e-5f)) in = false;
-                              else in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > L.p0.w);
+                              const bool in = range_test(tx, ty, tz, tx * tx + ty * ty + tz * tz, L.p0.w);
                               pairs += in ? 1u : 0u;
                           });
         // exact: at most 64 pixels x lights per cell per wave, far below 2^24
@@ -451,26 +492,29 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         return;
     }
 
-    if (MODE != kModeBackward) {
-        walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
-            if (!mine) return;
-            Term t;
-            bool in_range;
-            if (eval_term<false>(L, p, t, in_range)) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) lo[c] += t.B[c] * t.rad[c] * t.ndl;
-            }
-        });
-    }
-    // ambient + combine (deferred_lighting.wgsl:1196-1231)
+    // ambient + emissive (deferred_lighting.wgsl:1196-1231), added to the sum
+    // of the lights at the end
     const float ht = p.ny * 0.5f + 0.5f;
-    float mixv[3], o[3];
+    float mixv[3], amb[3], o[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float sky = a.sky[c];
         mixv[c] = sky * 0.15f * (1.0f - ht) + sky * ht;
-        o[c] = lo[c] + mixv[c] * p.alb[c] * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
+        amb[c] = mixv[c] * p.alb[c] * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
     }
+    if (MODE != kModeBackward) {
+        walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
+            if (!mine) return;
+            Term t;
+            if (eval_term<false>(L, p, t)) {
+                const float w = t.ndl;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) lo[c] += t.B[c] * (t.rad[c] * w);
+            }
+        });
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = lo[c] + amb[c];
     if (MODE == kModeForward) {
         if (valid) hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
         return;
@@ -503,8 +547,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         bool act = false;
         if (mine) {
             Term t;
-            bool in_range;
-            act = eval_term<true>(L, p, t, in_range);
+            act = eval_term<true>(L, p, t);
             if (act) adjoint_term(L, p, t, gl, dA, dR, dM, g);
         }
         if (hl_ballot(act) == 0) return;
@@ -531,6 +574,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 // ------------------------------------------------------------------ kernels
 // One view: one block per block tile, blocks remapped so that consecutive
 // tiles share an XCD (and its L2: neighbouring tiles share cells and lights).
+// (Resident-block variants that loop over tiles measured slower here: the
+// kernel is VALU-issue bound, not dispatch bound.)
 template <int FA, int FN, int GRID, int MODE>
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     __shared__ ShadeSmem sm;
