@@ -247,6 +247,19 @@ hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const 
     return HL_OK;
 }
 
+// Persistent shading launch on the context stream with the claim counters of
+// the current parity; the other half is zeroed by the launch for the next one.
+hl_status shade_launch(hl_context* c, int set, int mode, const ShadeArgs& a) {
+    hl_status s = ensure(c, c->shadeq, 2 * 4 * (size_t)kShadeQueueWords, true);
+    if (s != HL_OK) return s;
+    uint32_t* q = (uint32_t*)c->shadeq.p;
+    uint32_t* use = q + (size_t)c->shade_parity * kShadeQueueWords;
+    uint32_t* next = q + (size_t)(c->shade_parity ^ 1u) * kShadeQueueWords;
+    HL_HIP(c, launch_shade(c->stream, set, mode, a, use, next));
+    c->shade_parity ^= 1u;
+    return HL_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ exports
@@ -292,8 +305,8 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
     if (!c) return HL_EINVAL;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->markq, &c->words,
-                      &c->accum})
+    for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->markq, &c->shadeq,
+                      &c->words, &c->accum})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return HL_OK;
@@ -301,6 +314,10 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
 
 HL_API hl_status hl_set_stream(hl_context* c, void* stream) {
     if (!c) return HL_EINVAL;
+    // work of the old stream finishes first: the context's scratch (and the
+    // shading claim counters a launch leaves for the next one) is reused in
+    // stream order
+    HL_HIP(c, hipStreamSynchronize(c->stream));
     c->stream = (hipStream_t)stream;
     return HL_OK;
 }
@@ -539,7 +556,7 @@ HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* cam
     a.out = out_d;
     a.out_pitch = out_pitch_bytes;
     a.flags = flags;
-    HL_HIP(c, launch_shade(c->stream, set, kModeForward, a));
+    if ((s = shade_launch(c, set, kModeForward, a)) != HL_OK) return s;
     return HL_OK;
 }
 
@@ -560,7 +577,7 @@ HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUnifor
     unsigned long long* ctr = (unsigned long long*)(ctx_words(c) + 2);
     HL_HIP(c, launch_fill_u32(c->stream, (uint32_t*)ctr, 0, 2));
     a.pair_counter = ctr;
-    HL_HIP(c, launch_shade(c->stream, set, kModeCount, a));
+    if ((s = shade_launch(c, set, kModeCount, a)) != HL_OK) return s;
     unsigned long long v = 0;
     HL_HIP(c, hipMemcpyAsync(&v, ctr, 8, hipMemcpyDeviceToHost, c->stream));
     HL_HIP(c, hipStreamSynchronize(c->stream));
@@ -597,9 +614,9 @@ hl_status run_adjoint(hl_context* c, const GpuCameraUniforms* camera_d, const De
     }
     if (!d_pixel_d) return fail(c, HL_EINVAL, "d_pixel is null");
     if (n && (!d_pos_d || !d_int_d)) return fail(c, HL_EINVAL, "light gradient outputs are null");
-    if ((s = ensure(c, c->accum, sizeof(double) * 4 * (size_t)(n ? n : 1))) != HL_OK) return s;
+    if ((s = ensure(c, c->accum, sizeof(double) * grad_accum_doubles(n))) != HL_OK) return s;
     double* acc = (double*)c->accum.p;
-    HL_HIP(c, launch_zero_f64(c->stream, acc, 4 * (size_t)n));
+    HL_HIP(c, launch_zero_f64(c->stream, acc, grad_accum_doubles(n)));
     if (fused) HL_HIP(c, launch_zero_f64(c->stream, loss_d, 1));
     a.dout = in_d;
     a.dout_pitch = in_pitch;
@@ -608,7 +625,7 @@ hl_status run_adjoint(hl_context* c, const GpuCameraUniforms* camera_d, const De
     a.loss = loss_d;
     a.d_pixel = d_pixel_d;
     a.d_light = acc;
-    HL_HIP(c, launch_shade(c->stream, set, fused ? kModeFusedL2 : kModeBackward, a));
+    if ((s = shade_launch(c, set, fused ? kModeFusedL2 : kModeBackward, a)) != HL_OK) return s;
     HL_HIP(c, launch_light_grads_to_f32(c->stream, acc, n, nullptr, d_pos_d, d_int_d));
     return HL_OK;
 }

@@ -67,7 +67,10 @@ struct hl_plan {
     uint32_t* d_queue = nullptr;
     uint32_t* d_flags = nullptr;
     void* d_markq = nullptr;  // mark_queue_bytes(total_lights)
-    uint32_t total_lights = 0, total_cells = 0, total_tiles = 0, num_items = 0, grid_blocks = 0;
+    uint32_t total_lights = 0, total_cells = 0, total_tiles = 0, num_items = 0;
+    // shading items grouped by (G-buffer format set, mode): one persistent
+    // launch per non-empty group; group = set * 2 + (adjoint ? 1 : 0)
+    uint32_t grp_first[6] = {}, grp_count[6] = {}, grp_grid[6] = {};
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -126,7 +129,7 @@ hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
     for (uint32_t v = 0; v < nv; ++v) {
         const hl_view_desc& d = p->views[v];
         if (!is_adjoint(d)) continue;
-        HL_HIP(c, launch_zero_f64(s, (double*)p->bufs[v].acc, 4 * (size_t)p->nlights[v]));
+        HL_HIP(c, launch_zero_f64(s, (double*)p->bufs[v].acc, grad_accum_doubles(p->nlights[v])));
         HL_HIP(c, launch_zero_f64(s, d.loss_d, 1));
         HL_HIP(c, launch_fill_u32(s, (uint32_t*)d.d_light_pos_d, 0, 3 * (size_t)p->pool_size));
         HL_HIP(c, launch_fill_u32(s, (uint32_t*)d.d_light_intensity_d, 0, p->pool_size));
@@ -135,7 +138,11 @@ hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
     HL_HIP(c, launch_batch_cull(s, p->pool, p->d_cull, p->d_light_base, p->d_cell_base,
                                 p->d_tile_base, nv, p->total_lights, p->total_cells, p->total_tiles,
                                 mark_queue_at(p->d_markq, p->total_lights)));
-    HL_HIP(c, launch_shade_batch(s, p->d_shade, p->d_items, p->num_items, p->d_queue, p->grid_blocks));
+    HL_HIP(c, launch_fill_u32(s, p->d_queue, 0, 6 * (size_t)kShadeQueueWords));
+    for (int g = 0; g < 6; ++g)
+        HL_HIP(c, launch_shade_batch(s, g / 2, (g & 1) ? kModeFusedL2 : kModeForward, p->d_shade,
+                                     p->d_items + p->grp_first[g], p->grp_count[g],
+                                     p->d_queue + (size_t)g * kShadeQueueWords, p->grp_grid[g]));
     for (uint32_t v = 0; v < nv; ++v) {
         const hl_view_desc& d = p->views[v];
         if (!is_adjoint(d)) continue;
@@ -180,7 +187,8 @@ hl_status build_plan(hl_plan* p) {
         b.pool_cap = (uint32_t)std::min<uint64_t>((uint64_t)ncell * std::min<uint32_t>(n, 32u) + 4096u,
                                                   0x7fffffffu);
         if ((s = plan_alloc(p, &b.pool, 4 * (size_t)b.pool_cap)) != HL_OK) return s;
-        if (is_adjoint(d) && (s = plan_alloc(p, &b.acc, 8 * 4 * (size_t)(n ? n : 1))) != HL_OK) return s;
+        if (is_adjoint(d) && (s = plan_alloc(p, &b.acc, sizeof(double) * grad_accum_doubles(n))) != HL_OK)
+            return s;
         BatchCullView& cv = p->hcull[v];
         cv.ca = make_cluster_args(&cp, n);
         cv.cam = d.camera_d;
@@ -221,7 +229,12 @@ hl_status build_plan(hl_plan* p) {
         HL_HIP(c, launch_batch_cull(c->stream, p->pool, p->d_cull, p->d_light_base, p->d_cell_base,
                                     p->d_tile_base, nv, p->total_lights, p->total_cells,
                                     p->total_tiles, mark_queue_at(p->d_markq, p->total_lights)));
-        HL_HIP(c, hipStreamSynchronize(c->stream));
+        {
+            const hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess)
+                return fail(c, HL_EHIP, "plan cull pass %d (%u views, %u lights, %u clusters): %s", pass, nv,
+                            p->total_lights, p->total_cells, hipGetErrorString(e));
+        }
         bool grew = false;
         for (uint32_t v = 0; v < nv; ++v) {
             std::vector<uint32_t> raw(p->hcull[v].ncell);
@@ -246,6 +259,7 @@ hl_status build_plan(hl_plan* p) {
     HL_HIP(c, hipMemcpy(p->d_cull, p->hcull.data(), sizeof(BatchCullView) * nv, hipMemcpyHostToDevice));
     // shading records
     std::vector<std::pair<double, uint32_t>> order;
+    std::vector<int> vset(nv, 0);
     for (uint32_t v = 0; v < nv; ++v) {
         const hl_view_desc& d = p->views[v];
         hl_light_grid grid{};
@@ -260,8 +274,7 @@ hl_status build_plan(hl_plan* p) {
         hl_status st = build_shade_args(c, d.camera_d, &d.globals, &d.gbuffer, p->nlights[v], &grid,
                                         &sv.sa, &set, is_adjoint(d));
         if (st != HL_OK) return st;
-        if (set != kSetRef)
-            return fail(c, HL_EUNSUPPORTED, "view %u: batched views use the reference G-buffer formats", v);
+        vset[v] = set;
         set_tile_shape(sv.sa, p->cfg[v].tw_log2);
         sv.sa.lights = (const HotLight*)p->bufs[v].hot;
         sv.sa.out = d.out_d;
@@ -284,19 +297,24 @@ hl_status build_plan(hl_plan* p) {
                          return a.first > b.first;
                      });
     std::vector<uint32_t> items;
-    for (const auto& o : order) {
-        const uint32_t v = o.second;
-        for (uint32_t blk = 0; blk < p->hshade[v].sa.num_blocks; ++blk) items.push_back((v << 24) | blk);
+    int cus = 0;
+    HL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    for (int g = 0; g < 6; ++g) {
+        p->grp_first[g] = (uint32_t)items.size();
+        for (const auto& o : order) {
+            const uint32_t v = o.second;
+            if (vset[v] * 2 + (is_adjoint(p->views[v]) ? 1 : 0) != g) continue;
+            for (uint32_t blk = 0; blk < p->hshade[v].sa.num_blocks; ++blk) items.push_back((v << 24) | blk);
+        }
+        p->grp_count[g] = (uint32_t)items.size() - p->grp_first[g];
+        // 8 blocks of 4 waves per CU, fewer when there are fewer items
+        p->grp_grid[g] = std::min((uint32_t)std::max(1, cus) * 8u, std::max(1u, p->grp_count[g]));
     }
     p->num_items = (uint32_t)items.size();
     if ((s = plan_alloc(p, (void**)&p->d_items, 4 * (items.empty() ? 1 : items.size()))) != HL_OK) return s;
     if (!items.empty())
         HL_HIP(c, hipMemcpy(p->d_items, items.data(), 4 * items.size(), hipMemcpyHostToDevice));
-    if ((s = plan_alloc(p, (void**)&p->d_queue, 64)) != HL_OK) return s;
-    int cus = 0;
-    HL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    p->grid_blocks = (uint32_t)std::max(1, cus) * 8u;  // 8 blocks of 4 waves per CU
-    if (p->grid_blocks > p->num_items) p->grid_blocks = std::max(1u, p->num_items);
+    if ((s = plan_alloc(p, (void**)&p->d_queue, 4 * 6 * (size_t)kShadeQueueWords)) != HL_OK) return s;
     return HL_OK;
 }
 
@@ -332,22 +350,29 @@ hl_status time_config(hl_context* c, const GpuLight* pool, uint32_t pool_size,
     hl_status s = build_plan(&tp);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<float> t;
+    hipError_t e = hipSuccess;
     if (s == HL_OK) {
-        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) s = HL_EHIP;
+        if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+            s = fail(c, HL_EHIP, "autotune: hipEventCreate: %s", hipGetErrorString(e));
         for (int r = 0; r < 6 && s == HL_OK; ++r) {
-            if (hipEventRecord(e0, c->stream) != hipSuccess) s = HL_EHIP;
-            if (s == HL_OK) s = enqueue_frame(&tp, c->stream);
-            if (s == HL_OK && hipEventRecord(e1, c->stream) != hipSuccess) s = HL_EHIP;
+            if ((e = hipEventRecord(e0, c->stream)) != hipSuccess) break;
+            if ((s = enqueue_frame(&tp, c->stream)) != HL_OK) break;
+            if ((e = hipEventRecord(e1, c->stream)) != hipSuccess) break;
+            // a fault of this frame's kernels surfaces at the stream synchronisation
+            if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) break;
             float x = 0.0f;
-            if (s == HL_OK && (hipEventSynchronize(e1) != hipSuccess ||
-                               hipEventElapsedTime(&x, e0, e1) != hipSuccess))
-                s = HL_EHIP;
+            if ((e = hipEventElapsedTime(&x, e0, e1)) != hipSuccess) break;
             if (r > 0) t.push_back(x);
         }
+        if (s == HL_OK && e != hipSuccess)
+            s = fail(c, HL_EHIP, "autotune frame (view %ux%u, tile %u, cell %u, slices %u): %s",
+                     d.gbuffer.width, d.gbuffer.height, 1u << cf.tw_log2, cf.cell, cf.gz,
+                     hipGetErrorString(e));
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    (void)hipStreamSynchronize(c->stream);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess && s == HL_OK)
+        s = fail(c, HL_EHIP, "autotune: hipStreamSynchronize: %s", hipGetErrorString(e));
     free_plan_device(&tp);
     if (s != HL_OK) return s;
     std::sort(t.begin(), t.end());

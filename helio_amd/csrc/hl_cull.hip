@@ -316,18 +316,18 @@ __device__ __forceinline__ MarkBox unpack_box(uint4 v) {
 
 // Candidate cells chunk*64 .. chunk*64+63 of light li (one per lane, slice
 // fastest): the exact membership rule, then the member bit and its summary
-// bit (no-return atomics).  All 64 lanes must be active; sp is
-// slice_r_pad[lane] (lanes below grid_z).
+// bit (no-return atomics).  The column, box and slice-padding loads of a lane
+// are independent and issued together.
 __device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight& L,
-                                           const MarkBox& m, uint32_t li, float sp,
+                                           const MarkBox& m, uint32_t li,
                                            const ColumnGeom* __restrict__ geom,
                                            const CellBox* __restrict__ boxes,
                                            uint32_t* __restrict__ bits,
                                            uint32_t* __restrict__ summary, uint32_t W,
                                            uint32_t SW, uint32_t chunk) {
     const uint32_t work = m.nx * m.ny * m.nz;
-    const uint32_t t0 = chunk * 64u + (threadIdx.x & 63u);
-    const uint32_t t = t0 < work ? t0 : work - 1u;
+    const uint32_t t = chunk * 64u + (threadIdx.x & 63u);
+    if (t >= work) return;
     uint32_t rest, iy;
     if (work <= (1u << 21)) {
         rest = hl_floor_div(t, m.nz);
@@ -339,10 +339,8 @@ __device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight
     const uint32_t kz = t - rest * m.nz, ix = rest - iy * m.nx;
     const uint32_t col = (m.x0 + ix) + a.grid_x * (m.y0 + iy);
     const uint32_t k = m.s0 + kz;
-    const float pad = __shfl(sp, (int)k, 64);  // every lane active here
-    if (t0 >= work) return;
     const size_t cell = (size_t)col + (size_t)(a.grid_x * a.grid_y) * k;
-    if (!cell_member(L, geom[col], boxes[cell], pad)) return;
+    if (!cell_member(L, geom[col], boxes[cell], a.slice_r_pad[k])) return;
     const uint32_t word = li >> 5;
     atomicOr(&bits[cell * W + word], 1u << (li & 31u));
     atomicOr(&summary[cell * SW + (word >> 5)], 1u << (word & 31u));
@@ -358,8 +356,9 @@ __device__ __forceinline__ CullLight load_light_uniform(const CullLight* __restr
 }
 
 // One wavefront per light: its candidate box, its first kMarkInline chunks,
-// and the rest queued as (qlight, chunk) entries with the box stored at
-// bounds[qlight] (qlight: the light's index in the queue's numbering).
+// and the rest queued as (qlight, qview << 24 | chunk) entries with the box
+// stored at bounds[qlight] (qlight: the light's index in the queue's
+// numbering, qview: its view in a batch; chunks stay below 2^20).
 // Chunks that do not fit in the light's sub-queue stay with this wave.
 __device__ __forceinline__ void mark_light(const ClusterArgs& a,
                                            const GpuCameraUniforms* __restrict__ cam,
@@ -368,7 +367,8 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
                                            const CellBox* __restrict__ boxes,
                                            uint32_t* __restrict__ bits,
                                            uint32_t* __restrict__ summary, uint32_t W, uint32_t SW,
-                                           uint32_t li, const MarkQueue& q, uint32_t qlight) {
+                                           uint32_t li, const MarkQueue& q, uint32_t qlight,
+                                           uint32_t qview) {
     if (li >= a.num_lights) return;
     const CullLight L = load_light_uniform(cl, li);
     const float sp = hl_lane() < a.grid_z ? a.slice_r_pad[hl_lane()] : 0.0f;
@@ -387,14 +387,14 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
             if ((threadIdx.x & 63u) == 0) q.bounds[qlight] = pack_box(m);
             uint2* seg = q.entries + (size_t)sub * q.seg + base;
             for (uint32_t j = threadIdx.x & 63u; j < fit; j += 64u)
-                seg[j] = make_uint2(qlight, kMarkInline + j);
+                seg[j] = make_uint2(qlight, (qview << 24) | (kMarkInline + j));
             qlo = kMarkInline;
             qhi = kMarkInline + fit;
         }
     }
     for (uint32_t c = 0; c < nchunk; ++c) {
         if (c >= qlo && c < qhi) continue;
-        mark_chunk(a, L, m, li, sp, geom, boxes, bits, summary, W, SW, c);
+        mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, c);
     }
 }
 
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
                                                       uint32_t* __restrict__ summary, uint32_t W,
                                                       uint32_t SW, MarkQueue q) {
     const uint32_t li = blockIdx.x * 4u + (threadIdx.x >> 6);
-    mark_light(a, cam, cl, geom, boxes, bits, summary, W, SW, li, q, li);
+    mark_light(a, cam, cl, geom, boxes, bits, summary, W, SW, li, q, li, 0u);
 }
 
 // Queued chunks over a fixed grid of wavefronts: nw / kMarkQueues waves per
@@ -688,15 +688,14 @@ __global__ __launch_bounds__(256) void k_cluster_mark_tail(ClusterArgs a,
     const uint32_t sub = w % kMarkQueues, step = nw / kMarkQueues;  // nw: a multiple of kMarkQueues
     const uint32_t n = min(hl_uniform(q.count[sub * kMarkCountStride]), q.seg);
     const uint2* seg = q.entries + (size_t)sub * q.seg;
-    const float sp = hl_lane() < a.grid_z ? a.slice_r_pad[hl_lane()] : 0.0f;
     for (uint32_t e = w / kMarkQueues; e < n; e += step) {
         const uint2 en = seg[e];
-        const uint32_t li = hl_uniform(en.x), chunk = hl_uniform(en.y);
+        const uint32_t li = hl_uniform(en.x), chunk = hl_uniform(en.y) & 0xFFFFFFu;
         const CullLight L = load_light_uniform(cl, li);
         const uint4 pb = q.bounds[li];
         const MarkBox m = unpack_box(make_uint4(hl_uniform(pb.x), hl_uniform(pb.y), hl_uniform(pb.z),
                                                 hl_uniform(pb.w)));
-        mark_chunk(a, L, m, li, sp, geom, boxes, bits, summary, W, SW, chunk);
+        mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, chunk);
     }
 }
 
@@ -785,7 +784,7 @@ __global__ __launch_bounds__(256) void k_batch_mark(const BatchCullView* __restr
     const uint32_t vi = find_view(light_base, nviews, g);
     const BatchCullView& v = views[vi];
     mark_light(v.ca, v.cam, v.cull, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary,
-               v.W, v.SW, g - light_base[vi], q, g);
+               v.W, v.SW, g - light_base[vi], q, g, vi);
 }
 
 __global__ __launch_bounds__(256) void k_batch_mark_tail(const BatchCullView* __restrict__ views,
@@ -797,17 +796,16 @@ __global__ __launch_bounds__(256) void k_batch_mark_tail(const BatchCullView* __
     const uint2* seg = q.entries + (size_t)sub * q.seg;
     for (uint32_t e = w / kMarkQueues; e < n; e += step) {
         const uint2 en = seg[e];
-        const uint32_t g = hl_uniform(en.x), chunk = hl_uniform(en.y);
-        const uint32_t vi = hl_uniform(find_view(light_base, nviews, g));
+        const uint32_t g = hl_uniform(en.x), packed = hl_uniform(en.y);
+        const uint32_t vi = packed >> 24, chunk = packed & 0xFFFFFFu;
         const BatchCullView& v = views[vi];
         const uint32_t li = g - light_base[vi];
         const CullLight L = load_light_uniform(v.cull, li);
-        const float sp = hl_lane() < v.ca.grid_z ? v.ca.slice_r_pad[hl_lane()] : 0.0f;
         const uint4 pb = q.bounds[g];
         const MarkBox m = unpack_box(make_uint4(hl_uniform(pb.x), hl_uniform(pb.y), hl_uniform(pb.z),
                                                 hl_uniform(pb.w)));
-        mark_chunk(v.ca, L, m, li, sp, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary,
-                   v.W, v.SW, chunk);
+        mark_chunk(v.ca, L, m, li, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary, v.W,
+                   v.SW, chunk);
     }
 }
 
@@ -850,13 +848,10 @@ __global__ __launch_bounds__(256) void k_batch_fill(const BatchCullView* __restr
               g - cell_base[vi], cs);
 }
 
-// grid of the queued-chunk launches: enough wavefronts to spread the tail
-// of a few heavy lights over every CU, few enough to exit fast when idle
-static uint32_t tail_blocks(uint32_t lights) {
-    constexpr uint32_t q = kMarkQueues / 4u;  // blocks per round over the sub-queues
-    uint32_t b = (lights / 16u + q - 1u) / q * q;
-    return b < q ? q : (b > 1024u ? 1024u : b);
-}
+// grid of the queued-chunk launches: 64 wavefronts per sub-queue, enough to
+// spread the chunks of a few heavy lights over every CU whatever the light
+// count (idle waves read one counter and exit); a multiple of kMarkQueues / 4
+static uint32_t tail_blocks(uint32_t) { return 16u * kMarkQueues; }
 
 hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCullView* views,
                              const uint32_t* light_base, const uint32_t* cell_base,

@@ -82,7 +82,7 @@ struct ShadeArgs {
     const float* dout;       // dL/dout RGBA32F (or target for the fused L2 variant)
     uint32_t dout_pitch;
     float* d_pixel;          // 5 planes
-    double* d_light;         // 4 per light: dpos.xyz, dI (FP64 accumulation)
+    double* d_light;         // kGradSlots x 4 per light: dpos.xyz, dI (FP64 accumulation)
     double* loss;            // fused L2 loss accumulator
     uint32_t block_tiles_x;  // block tiles per row
     uint32_t num_blocks;     // block tiles of the view

@@ -33,7 +33,8 @@ struct hl_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf hot, cull, geom, bits, summary, raw, markq, words, accum;
+    DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum;
+    uint32_t shade_parity = 0;  // which half of shadeq the next shading launch claims from
     // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
 };
 
@@ -159,7 +160,15 @@ hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCam
 
 // launchers (hl_shade.hip)
 enum ShadeMode : int { kModeForward = 0, kModeCount = 1, kModeBackward = 2, kModeFusedL2 = 3 };
-hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& a);
+// Claim counters of one persistent shading launch: kShadeQueues counters,
+// kShadeQueueStride words apart, zeroed beforehand (launch_fill_u32).
+constexpr uint32_t kShadeQueues = 64;
+constexpr uint32_t kShadeQueueStride = 32;
+constexpr uint32_t kShadeQueueWords = kShadeQueues * kShadeQueueStride;
+// One view, persistent: queue = zeroed claim counters (kShadeQueueWords),
+// queue_next = the counter set to zero for the next launch on the stream.
+hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& a, uint32_t* queue,
+                        uint32_t* queue_next);
 // block tiles of a view for a wavefront tile width (1 << tw_log2)
 inline void set_tile_shape(ShadeArgs& a, uint32_t tw_log2) {
     const uint32_t bw = 4u << tw_log2, bh = 64u >> tw_log2;
@@ -167,8 +176,15 @@ inline void set_tile_shape(ShadeArgs& a, uint32_t tw_log2) {
     a.block_tiles_x = (a.width + bw - 1) / bw;
     a.num_blocks = a.block_tiles_x * ((a.height + bh - 1) / bh);
 }
-hipError_t launch_shade_batch(hipStream_t s, const BatchShadeView* views, const uint32_t* items,
-                              uint32_t num_items, uint32_t* queue, uint32_t grid_blocks);
+hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
+                              const uint32_t* items, uint32_t num_items, uint32_t* queue,
+                              uint32_t grid_blocks);
+// Per-light FP64 gradient accumulators: kGradSlots slot-major copies of 4
+// doubles (dpos.xyz, dI) per light, summed by launch_light_grads_to_f32.
+constexpr uint32_t kGradSlots = 8;
+inline size_t grad_accum_doubles(uint32_t num_lights) {
+    return (size_t)kGradSlots * 4 * (num_lights ? num_lights : 1);
+}
 hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n);
 hipError_t launch_light_grads_to_f32(hipStream_t s, const double* acc, uint32_t n,
                                      const uint32_t* subset, float* d_pos, float* d_int);

@@ -542,6 +542,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         gl[2] = g.z;
     }
     float dA[3] = {0.0f, 0.0f, 0.0f}, dR = 0.0f, dM = 0.0f;
+    // per-light gradients go to one of kGradSlots copies (by wave) so that the
+    // FP64 atomics of many waves on one light do not serialise on one address
+    const uint32_t slot = (b * 4u + wave) % kGradSlots;
     walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
         float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         bool act = false;
@@ -552,12 +555,10 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         }
         if (hl_ballot(act) == 0) return;
         const float sx = dpp_sum(g.x), sy = dpp_sum(g.y), sz = dpp_sum(g.z), si = dpp_sum(g.w);
-        if (lane == 0) {
-            double* dl = a.d_light + 4 * (size_t)li;
-            atomicAdd(dl + 0, (double)sx);
-            atomicAdd(dl + 1, (double)sy);
-            atomicAdd(dl + 2, (double)sz);
-            atomicAdd(dl + 3, (double)si);
+        // lanes 0..3 add the four sums to this wave's slot of the light
+        if (lane < 4u) {
+            const float v = lane == 0u ? sx : (lane == 1u ? sy : (lane == 2u ? sz : si));
+            atomicAdd(a.d_light + 4 * ((size_t)slot * a.num_lights + li) + lane, (double)v);
         }
     });
     if (valid) {
@@ -572,81 +573,173 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 }
 
 // ------------------------------------------------------------------ kernels
-// One view: one block per block tile, blocks remapped so that consecutive
-// tiles share an XCD (and its L2: neighbouring tiles share cells and lights).
-// (Resident-block variants that loop over tiles measured slower here: the
-// kernel is VALU-issue bound, not dispatch bound.)
-template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    __shared__ ShadeSmem sm;
-    shade_tile<FA, FN, GRID, MODE>(a, hl_xcd_remap(blockIdx.x, gridDim.x), sm);
+// Work distribution of the persistent launches: num_items items dealt
+// round-robin to kShadeQueues sub-queues (sub-queue q holds items q,
+// q + kShadeQueues, ...), each with its own claim counter on its own 128-B
+// line, so that the claims of thousands of blocks do not serialise on one
+// address.  A block claims from its own sub-queue until it is drained, then
+// reads all counters at once (one per thread of the first wave) and moves to
+// the next sub-queue with items left; it calls f(item) for every item it
+// claims and returns when every sub-queue is drained.
+struct ClaimSmem {
+    uint32_t item, q;
+};
+static_assert(kShadeQueues == 64, "one lane of the first wave per sub-queue");
+
+__device__ __forceinline__ uint32_t queue_len(uint32_t num_items, uint32_t q) {
+    return q < num_items ? (num_items - q + kShadeQueues - 1u) / kShadeQueues : 0u;
 }
 
-// Many views, one persistent launch: each block repeatedly claims the next
-// (view, block tile) item from an atomic queue until the queue is exhausted
-// (every block exits once its claim is past the end).  Items are ordered
-// heaviest first by the host.  Views run forward or forward + L2 adjoint.
+template <class F>
+__device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __restrict__ queue,
+                                            ClaimSmem& cs, F&& f) {
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t q = blockIdx.x % kShadeQueues;
+    for (;;) {
+        const uint32_t nq = queue_len(num_items, q);
+        if (threadIdx.x == 0) {
+            uint32_t* ctr = queue + q * kShadeQueueStride;
+            // a drained sub-queue is seen without claiming from it
+            const uint32_t seen = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            cs.item = seen >= nq ? kNone : atomicAdd(ctr, 1u);
+        }
+        __syncthreads();
+        const uint32_t j = cs.item;
+        __syncthreads();
+        if (j < nq) {
+            f(q + j * kShadeQueues);
+            continue;
+        }
+        // drained: the next sub-queue (cyclically after q) with items left
+        if (threadIdx.x < 64u) {
+            const uint32_t r = threadIdx.x;
+            const uint32_t len = queue_len(num_items, r);
+            const uint32_t seen =
+                len ? __hip_atomic_load(queue + r * kShadeQueueStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    : 0u;
+            const uint64_t left = hl_ballot(seen < len);
+            if (threadIdx.x == 0) {
+                const uint32_t s0 = (q + 1u) % 64u;  // search from s0 on, wrapping
+                const uint64_t rot = s0 == 0u ? left : ((left >> s0) | (left << (64u - s0)));
+                cs.q = rot ? (s0 + (uint32_t)__builtin_ctzll(rot)) % 64u : kNone;
+            }
+        }
+        __syncthreads();
+        q = cs.q;
+        __syncthreads();
+        if (q == kNone) return;
+    }
+}
+
+// One view, persistent: the block tiles of the view are the items.  The
+// counters of the previous launch on the stream (finished by stream order)
+// are zeroed here for the next one, so consecutive launches alternate
+// between two counter sets without a separate reset.
+template <int FA, int FN, int GRID, int MODE>
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
+                                               uint32_t* __restrict__ queue_next) {
+    __shared__ ShadeSmem sm;
+    __shared__ ClaimSmem cs;
+    if (blockIdx.x == 0 && threadIdx.x < kShadeQueues) queue_next[threadIdx.x * kShadeQueueStride] = 0u;
+    claim_items(a.num_blocks, queue, cs, [&](uint32_t t) { shade_tile<FA, FN, GRID, MODE>(a, t, sm); });
+}
+
+// Many views, one persistent launch: the items are (view, block tile) pairs
+// ordered heaviest first by the host.  One launch covers the views of one
+// G-buffer format set and one mode (forward, or forward + L2 adjoint): the
+// adjoint needs about twice the registers, which would halve the forward's
+// occupancy in a shared kernel.
+template <int FA, int FN, int MODE>
 __global__ __launch_bounds__(256) void k_shade_batch(const BatchShadeView* __restrict__ views,
                                                      const uint32_t* __restrict__ items,
                                                      uint32_t num_items,
                                                      uint32_t* __restrict__ queue) {
     __shared__ ShadeSmem sm;
-    __shared__ uint32_t s_item;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
-        __syncthreads();
-        const uint32_t it = s_item;
-        __syncthreads();
-        if (it >= num_items) break;
+    __shared__ ClaimSmem cs;
+    claim_items(num_items, queue, cs, [&](uint32_t it) {
         const uint32_t packed = items[it];
-        const BatchShadeView& v = views[packed >> 24];
-        const uint32_t blk = packed & 0xFFFFFFu;
-        if (v.mode == kModeFusedL2)
-            shade_tile<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT, HL_GRID_CLUSTERS, kModeFusedL2>(v.sa, blk, sm);
-        else
-            shade_tile<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT, HL_GRID_CLUSTERS, kModeForward>(v.sa, blk, sm);
+        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(views[packed >> 24].sa, packed & 0xFFFFFFu, sm);
+    });
+}
+
+template <int FA, int FN>
+static void launch_batch_fmt(hipStream_t s, int mode, const BatchShadeView* views,
+                             const uint32_t* items, uint32_t num_items, uint32_t* queue,
+                             uint32_t grid_blocks) {
+    if (mode == kModeFusedL2)
+        k_shade_batch<FA, FN, kModeFusedL2><<<grid_blocks, 256, 0, s>>>(views, items, num_items, queue);
+    else
+        k_shade_batch<FA, FN, kModeForward><<<grid_blocks, 256, 0, s>>>(views, items, num_items, queue);
+}
+
+hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
+                              const uint32_t* items, uint32_t num_items, uint32_t* queue,
+                              uint32_t grid_blocks) {
+    if (num_items == 0) return hipSuccess;
+    switch (gbuf_set) {
+        case kSetRef:
+            launch_batch_fmt<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT>(s, mode, views, items, num_items, queue,
+                                                                    grid_blocks);
+            break;
+        case kSet16:
+            launch_batch_fmt<HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA16_FLOAT>(s, mode, views, items, num_items, queue,
+                                                                     grid_blocks);
+            break;
+        default:
+            launch_batch_fmt<HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA32_FLOAT>(s, mode, views, items, num_items, queue,
+                                                                     grid_blocks);
+            break;
     }
-}
-
-__global__ void k_reset_queue(uint32_t* __restrict__ q, uint32_t n) {
-    if (threadIdx.x < n) q[threadIdx.x] = 0u;
-}
-
-hipError_t launch_shade_batch(hipStream_t s, const BatchShadeView* views, const uint32_t* items,
-                              uint32_t num_items, uint32_t* queue, uint32_t grid_blocks) {
-    k_reset_queue<<<1, 64, 0, s>>>(queue, 1);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || num_items == 0) return e;
-    k_shade_batch<<<grid_blocks, 256, 0, s>>>(views, items, num_items, queue);
     return hipGetLastError();
 }
 
+// Resident blocks of a persistent launch: what the kernel's occupancy allows
+// on every CU (queried once per kernel), at most one per item.
+template <int FA, int FN, int GRID, int MODE>
+static uint32_t persistent_blocks(uint32_t items) {
+    static int per_cu = -1, cus = 0;
+    if (per_cu < 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade<FA, FN, GRID, MODE>, 256, 0) !=
+                hipSuccess || n < 1)
+            n = 4, cus = cus > 0 ? cus : 256;
+        per_cu = n;
+    }
+    const uint32_t b = (uint32_t)(per_cu * cus);
+    return items < b ? items : b;
+}
+
 template <int FA, int FN, int MODE>
-static hipError_t launch_grid(hipStream_t s, const ShadeArgs& a) {
-    dim3 grid(a.num_blocks), block(256);
+static hipError_t launch_grid(hipStream_t s, const ShadeArgs& a, uint32_t* queue, uint32_t* queue_next) {
     if (a.grid_mode == HL_GRID_TILES)
-        k_shade<FA, FN, HL_GRID_TILES, MODE><<<grid, block, 0, s>>>(a);
+        k_shade<FA, FN, HL_GRID_TILES, MODE>
+            <<<persistent_blocks<FA, FN, HL_GRID_TILES, MODE>(a.num_blocks), 256, 0, s>>>(a, queue, queue_next);
     else
-        k_shade<FA, FN, HL_GRID_CLUSTERS, MODE><<<grid, block, 0, s>>>(a);
+        k_shade<FA, FN, HL_GRID_CLUSTERS, MODE>
+            <<<persistent_blocks<FA, FN, HL_GRID_CLUSTERS, MODE>(a.num_blocks), 256, 0, s>>>(a, queue,
+                                                                                           queue_next);
     return hipGetLastError();
 }
 
 template <int MODE>
-static hipError_t launch_fmt(hipStream_t s, int set, const ShadeArgs& a) {
+static hipError_t launch_fmt(hipStream_t s, int set, const ShadeArgs& a, uint32_t* q, uint32_t* qn) {
     switch (set) {
-        case kSetRef: return launch_grid<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT, MODE>(s, a);
-        case kSet16: return launch_grid<HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA16_FLOAT, MODE>(s, a);
-        default: return launch_grid<HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA32_FLOAT, MODE>(s, a);
+        case kSetRef: return launch_grid<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT, MODE>(s, a, q, qn);
+        case kSet16: return launch_grid<HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA16_FLOAT, MODE>(s, a, q, qn);
+        default: return launch_grid<HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA32_FLOAT, MODE>(s, a, q, qn);
     }
 }
 
-hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& a) {
+hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& a, uint32_t* queue,
+                        uint32_t* queue_next) {
     if (a.num_blocks == 0) return hipSuccess;
     switch (mode) {
-        case kModeForward: return launch_fmt<kModeForward>(s, gbuf_set, a);
-        case kModeCount: return launch_fmt<kModeCount>(s, gbuf_set, a);
-        case kModeBackward: return launch_fmt<kModeBackward>(s, gbuf_set, a);
-        case kModeFusedL2: return launch_fmt<kModeFusedL2>(s, gbuf_set, a);
+        case kModeForward: return launch_fmt<kModeForward>(s, gbuf_set, a, queue, queue_next);
+        case kModeCount: return launch_fmt<kModeCount>(s, gbuf_set, a, queue, queue_next);
+        case kModeBackward: return launch_fmt<kModeBackward>(s, gbuf_set, a, queue, queue_next);
+        case kModeFusedL2: return launch_fmt<kModeFusedL2>(s, gbuf_set, a, queue, queue_next);
         default: return hipErrorInvalidValue;
     }
 }
@@ -658,18 +751,24 @@ __global__ void k_zero_f64(double* __restrict__ p, size_t n) {
     for (; i < n; i += stride) p[i] = 0.0;
 }
 
-// FP64 accumulators (4 per light of the launch) -> FP32 outputs indexed by the
-// light's pool index (subset[i], or i).
+// FP64 accumulators (kGradSlots copies of 4 per light of the launch, slot
+// major) -> their sums as FP32 outputs indexed by the light's pool index
+// (subset[i], or i).
 __global__ void k_light_grads_to_f32(const double* __restrict__ acc, uint32_t n,
                                      const uint32_t* __restrict__ subset, float* __restrict__ d_pos,
                                      float* __restrict__ d_int) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (uint32_t s = 0; s < kGradSlots; ++s)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) g[c] += acc[4 * ((size_t)s * n + i) + c];
     const uint32_t o = subset ? subset[i] : i;
-    d_pos[3 * (size_t)o + 0] = (float)acc[4 * (size_t)i + 0];
-    d_pos[3 * (size_t)o + 1] = (float)acc[4 * (size_t)i + 1];
-    d_pos[3 * (size_t)o + 2] = (float)acc[4 * (size_t)i + 2];
-    d_int[o] = (float)acc[4 * (size_t)i + 3];
+    d_pos[3 * (size_t)o + 0] = (float)g[0];
+    d_pos[3 * (size_t)o + 1] = (float)g[1];
+    d_pos[3 * (size_t)o + 2] = (float)g[2];
+    d_int[o] = (float)g[3];
 }
 
 hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n) {

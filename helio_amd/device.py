@@ -265,6 +265,22 @@ class DeferredLightPass:
                           C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
                           self.out.ptr, self.width * 16, flags)
 
+    def forward_backward_l2(self, scene: DeviceScene, grid: abi.hl_light_grid, target: DeviceArray,
+                            grads: "AdjointBuffers") -> None:
+        """Forward into self.out, then the L2 loss against target and its
+        gradients (hl_deferred_forward_backward_l2)."""
+        self.ctx.lib.call("hl_deferred_forward_backward_l2", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                          C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid), target.ptr,
+                          self.width * 16, self.out.ptr, self.width * 16, grads.loss.ptr, grads.d_pixel.ptr,
+                          grads.d_pos.ptr, grads.d_int.ptr)
+
+    def backward(self, scene: DeviceScene, grid: abi.hl_light_grid, d_out: DeviceArray,
+                 grads: "AdjointBuffers") -> None:
+        """Gradients for a given dL/dout (RGBA32F, alpha ignored), hl_deferred_backward."""
+        self.ctx.lib.call("hl_deferred_backward", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                          C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid), d_out.ptr,
+                          self.width * 16, grads.d_pixel.ptr, grads.d_pos.ptr, grads.d_int.ptr)
+
     def count_pairs(self, scene: DeviceScene, grid: abi.hl_light_grid) -> int:
         n = C.c_uint64()
         self.ctx.lib.call("hl_count_pixel_light_pairs", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
@@ -273,3 +289,16 @@ class DeferredLightPass:
 
     def output(self) -> np.ndarray:
         return self.out.download(np.float32, (self.height, self.width, 4))
+
+
+class AdjointBuffers:
+    """Outputs of the adjoint: loss (FP64), 5 per-pixel gradient planes
+    (albedo r, g, b, roughness, metallic) and per-light position / intensity
+    gradients."""
+
+    def __init__(self, ctx: Context, width: int, height: int, num_lights: int):
+        n = max(num_lights, 1)
+        self.loss = ctx.alloc(8)
+        self.d_pixel = ctx.alloc(abi.HL_PIXEL_GRAD_PLANES * width * height * 4)
+        self.d_pos = ctx.alloc(3 * n * 4)
+        self.d_int = ctx.alloc(n * 4)

@@ -239,6 +239,35 @@ def make_scene(s: SceneSpec | str, **over) -> HostScene:
                      formats=(fa, ta, fn, tn), texel=(ta, tn))
 
 
+def perturb_for_adjoint(gt: HostScene, seed: int = 7) -> HostScene:
+    """Initial guess of the differentiable-shading workload from ground truth
+    (fp32 G-buffer): albedo + N(0,0.1) clamped to [0,1], roughness + N(0,0.1)
+    clamped to [0.05,1], light positions + N(0,0.5), intensities x U[0.7,1.3].
+    F0 is re-resolved from the perturbed albedo and metallic as the G-buffer
+    producer does (gbuffer.wgsl:324-328) and packed into normal.a / orm.a /
+    emissive.a."""
+    if gt.spec.gbuf_format != GBUF_F32:
+        raise ValueError("the adjoint workload uses an fp32 G-buffer")
+    W, H = gt.width, gt.height
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = make_scene(gt.spec)
+    alb = g.albedo.view(np.float32).reshape(H, W, 4)
+    orm = g.orm.view(np.float32).reshape(H, W, 4)
+    alb[..., :3] = np.clip(alb[..., :3] + rng.normal(0, 0.1, (H, W, 3)), 0.0, 1.0).astype(np.float32)
+    orm[..., 1] = np.clip(orm[..., 1] + rng.normal(0, 0.1, (H, W)), 0.05, 1.0).astype(np.float32)
+    m = orm[..., 2:3]
+    f0 = np.clip(np.float32(0.04) * (np.float32(1) - m) + alb[..., :3] * m, 0.0, np.float32(0.999)).astype(np.float32)
+    nrm = g.normal.view(np.float32).reshape(H, W, 4)
+    emi = g.emissive.view(np.float32).reshape(H, W, 4)
+    nrm[..., 3], orm[..., 3], emi[..., 3] = f0[..., 0], f0[..., 1], f0[..., 2]
+    lights = g.lights.copy()
+    n = len(lights)
+    lights["position_range"][:, :3] += rng.normal(0, 0.5, (n, 3)).astype(np.float32)
+    lights["color_intensity"][:, 3] *= rng.uniform(0.7, 1.3, n).astype(np.float32)
+    g.lights = lights
+    return g
+
+
 def light_subset(seed: int, view: int, pool: int, n: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.uint32)
     if synth_lib().hl_synth_subset(seed, view, pool, n, out.ctypes.data if n else None) != 0:

@@ -56,20 +56,5 @@ def make_problem(width=512, height=512, num_lights=256, seed=7):
     out, _ = forward64(gt, px, gt.lights)
     target = np.zeros((height, width, 4), np.float32)
     target[..., :3] = out
-    rng = np.random.Generator(np.random.PCG64(seed))
-    guess = synth.make_scene("c4_512_adj", width=width, height=height, num_lights=num_lights)
-    alb = guess.albedo.view(np.float32).reshape(height, width, 4)
-    orm = guess.orm.view(np.float32).reshape(height, width, 4)
-    alb[..., :3] = np.clip(alb[..., :3] + rng.normal(0, 0.1, (height, width, 3)), 0.0, 1.0).astype(np.float32)
-    orm[..., 1] = np.clip(orm[..., 1] + rng.normal(0, 0.1, (height, width)), 0.05, 1.0).astype(np.float32)
-    # the G-buffer producer resolves F0 from albedo and metallic (gbuffer.wgsl:324-328)
-    m = orm[..., 2:3]
-    f0 = np.clip(np.float32(0.04) * (np.float32(1) - m) + alb[..., :3] * m, 0.0, np.float32(0.999)).astype(np.float32)
-    nrm = guess.normal.view(np.float32).reshape(height, width, 4)
-    emi = guess.emissive.view(np.float32).reshape(height, width, 4)
-    nrm[..., 3], orm[..., 3], emi[..., 3] = f0[..., 0], f0[..., 1], f0[..., 2]
-    lights = guess.lights.copy()
-    lights["position_range"][:, :3] += rng.normal(0, 0.5, (num_lights, 3)).astype(np.float32)
-    lights["color_intensity"][:, 3] *= rng.uniform(0.7, 1.3, num_lights).astype(np.float32)
-    guess.lights = lights
+    guess = synth.perturb_for_adjoint(gt, seed)
     return guess, target, gt.lights
