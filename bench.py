@@ -252,6 +252,9 @@ def main() -> int:
                          "memory fault once, cause not yet found)")
     ap.add_argument("--tune-trials", type=int, default=4)
     ap.add_argument("--workload", default="c3_4k_16k")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (a kernel of known traffic for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X "
@@ -260,6 +263,17 @@ def main() -> int:
         return 2
 
     from helio_amd.device import Context
+
+    if args.calibrate_copy:
+        with Context(0) as ctx:
+            n = args.calibrate_copy << 20
+            a, b = ctx.alloc(n), ctx.alloc(n)
+            a.fill_u32(0x3F800000)
+            for _ in range(5):
+                ctx.lib.call("hl_memcpy_dtod", ctx.h, b.ptr, a.ptr, n)
+            ctx.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n, "copies": 5}), flush=True)
+        return 0
 
     with Context(0) as ctx:
         r4k = bench_4k(ctx, args)
@@ -274,13 +288,14 @@ def main() -> int:
     flops = FLOP_PER_PIXEL * shaded + FLOP_PER_PAIR * q
     gbs = bytes_launch / (shade_ms * 1e-3) / 1e9
     tflops = flops / (shade_ms * 1e-3) / 1e12
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # calibrated against a copy of known size (the summary states both runs)
     traffic = None
-    tfile = ROOT / "profiles" / "traffic_shade_c3_4k_16k.json"
+    tfile = ROOT / "profiles" / "traffic_shade_c3_4k_16k.md"
     if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+        for line in tfile.read_text().splitlines():
+            if line.startswith("bytes_per_launch = "):
+                traffic = int(line.split("=")[1])
     params = r4k["params"]
     res = {
         "metric": "ms/frame & pixel-light evals/s at 4K/16k lights; fwd+adjoint ms at 512²",

@@ -281,8 +281,10 @@ HL_API hl_status hl_deferred_forward_backward_l2(
 /* ------------------------------------------------------ batched multi-view */
 /* One view of a batch: a camera, a G-buffer, a subset of a shared light pool
  * and an output.  The reference renders each view with its own full graph
- * submit (crates/helio/src/renderer/render.rs:804-873); a plan runs them all
- * in one persistent launch captured in a HIP graph. */
+ * submit (crates/helio/src/renderer/render.rs:804-873); a plan culls all views
+ * with one launch per cull stage and shades them with one persistent launch
+ * per G-buffer format set and mode (forward / forward + L2 adjoint), the whole
+ * frame capturable in a HIP graph. */
 typedef struct hl_view_desc {
     const GpuCameraUniforms* camera_d;
     GpuCameraUniforms camera;          /* host copy of the same camera (for slice set-up) */
