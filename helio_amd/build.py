@@ -3,6 +3,7 @@
     libhelio_hip.so    HIP kernels + C ABI (hipcc, gfx950)
     libhelio_synth.so  synthetic scene generator (gcc, OpenMP)
     oracle/liboracle.so  CPU checker used by the tests (gcc, -ffp-contract=off)
+    tests/cpp/test_passes  C++ tests of include/helio_hip/passes.hpp (g++)
 
 Run as ``python -m helio_amd.build``.  Every call recompiles.
 """
@@ -94,10 +95,22 @@ def build_oracle() -> Path:
     return out
 
 
+def build_cpp_tests() -> Path:
+    """The C++ pass-mirror tests, linked against the three libraries in-tree
+    (found at run time through an $ORIGIN-relative rpath)."""
+    src = ROOT / "tests" / "cpp" / "test_passes.cpp"
+    out = src.with_suffix("")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}", str(src),
+          "-o", str(out), f"-L{PKG}", f"-L{ORACLE}", "-lhelio_hip", "-lhelio_synth", "-loracle",
+          "-Wl,-rpath,$ORIGIN/../../helio_amd:$ORIGIN/../../oracle"])
+    return out
+
+
 def build_all(verbose: bool = True) -> None:
     with ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_hip), ex.submit(build_synth), ex.submit(build_oracle)]
         outs = [f.result() for f in futs]
+    outs.append(build_cpp_tests())
     if verbose:
         for o in outs:
             print(f"built {o.relative_to(ROOT)}")

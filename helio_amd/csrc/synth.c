@@ -22,30 +22,9 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "../../include/helio_hip/types.h"
+#include "synth.h"
 
 #define SYN_API __attribute__((visibility("default")))
-
-typedef struct hl_synth_config {
-    uint32_t width, height;
-    uint32_t gbuf_format;          /* 0: RGBA8/RGBA16F, 1: all fp16, 2: all fp32 */
-    uint32_t num_lights;
-    float spot_fraction;           /* share of spot lights (exact count, random slots) */
-    float spot_outer_min_deg, spot_outer_max_deg;
-    float spot_inner_ratio;        /* inner half-angle = ratio * outer */
-    float light_box_center[3], light_box_size[3];
-    float range_min, range_max;
-    float intensity_min, intensity_max;
-    float color_min, color_max;
-    float eye[3], target[3];
-    float fov_y_deg, near_plane, far_plane;
-    uint32_t num_boxes;
-    float box_field_half;          /* boxes placed in [-h, h]^2 on the ground */
-    float box_size_min, box_size_max, box_height_min, box_height_max;
-    float depth_noise;             /* sigma of the view-depth noise */
-    uint32_t _pad;
-    uint64_t seed;
-} hl_synth_config;
 
 /* ------------------------------------------------------------------ RNG */
 static inline uint64_t sm64(uint64_t x) {

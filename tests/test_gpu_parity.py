@@ -166,3 +166,36 @@ def test_zero_lights(ctx):
     ref, pairs = O.shade(sc, O.tile_grid_host(sc, lists, counts), flags=abi.HL_SHADE_CLEAR_SKY)
     assert pairs == 0
     check_radiance(out, ref)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4, 5, 20, 21])
+def test_debug_views_parity(ctx, mode):
+    """Debug views of fs_main (deferred_lighting.wgsl:1032-1038): albedo for
+    1/2/4/20/21 (bit-exact decode), the normal mapped to [0,1] for 5 (1e-4)."""
+    sc = scene("c1_720p_64")
+    print(f"[debug view] {mode}", flush=True)
+    ds, cull, lists, counts = hip_tiles(ctx, sc)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.debug_mode = mode
+    shade.prepare(ds)
+    shade.out.fill_u32(0)
+    shade.execute(ds, cull.publish())
+    out = shade.output()
+    grid = O.tile_grid_host(sc, np.ascontiguousarray(lists.reshape(-1)), counts)
+    ref, _ = O.shade(sc, grid, globals_=shade.globals)
+    lit = sc.depth < 1.0
+    if mode == 5:
+        check_radiance(out, ref, lit)
+    else:
+        assert np.array_equal(out[lit], ref[lit])
+
+
+def test_unsupported_debug_view_is_rejected(ctx):
+    sc = scene("c1_720p_64")
+    ds, cull, _, _ = hip_tiles(ctx, sc)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.debug_mode = 3  # needs the shadow atlas
+    shade.prepare(ds)
+    with pytest.raises(abi.HelioError) as e:
+        shade.execute(ds, cull.publish())
+    assert e.value.status == abi.HL_EUNSUPPORTED
