@@ -101,7 +101,8 @@ class hl_view_desc(C.Structure):
                 ("gbuffer", hl_gbuffer), ("light_indices_d", C.c_void_p), ("num_lights", C.c_uint32),
                 ("out_d", C.c_void_p), ("out_pitch_bytes", C.c_uint32), ("target_d", C.c_void_p),
                 ("target_pitch_bytes", C.c_uint32), ("loss_d", C.c_void_p), ("d_pixel_d", C.c_void_p),
-                ("d_light_pos_d", C.c_void_p), ("d_light_intensity_d", C.c_void_p)]
+                ("d_light_pos_d", C.c_void_p), ("d_light_intensity_d", C.c_void_p),
+                ("clusters", C.POINTER(hl_cluster_params))]
 
 
 class hl_autotune_opts(C.Structure):

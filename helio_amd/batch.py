@@ -27,6 +27,7 @@ class BatchView:
     d_pixel: DeviceArray | None = None
     d_pos: DeviceArray | None = None
     d_int: DeviceArray | None = None
+    clusters: abi.hl_cluster_params | None = None  # fixed grid; None = the plan chooses
 
     def desc(self) -> abi.hl_view_desc:
         d = abi.hl_view_desc()
@@ -39,6 +40,8 @@ class BatchView:
         d.num_lights = self.num_lights
         d.out_d = self.out.ptr
         d.out_pitch_bytes = sc.host.width * 16
+        if self.clusters is not None:
+            d.clusters = C.pointer(self.clusters)
         if self.adjoint:
             d.target_d = self.target.ptr
             d.target_pitch_bytes = sc.host.width * 16

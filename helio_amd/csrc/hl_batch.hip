@@ -168,12 +168,16 @@ hl_status build_plan(hl_plan* p) {
         const hl_view_desc& d = p->views[v];
         const ViewCfg& cf = p->cfg[v];
         const uint32_t W = d.gbuffer.width, H = d.gbuffer.height;
-        const uint32_t gx = (W + cf.cell - 1) / cf.cell, gy = (H + cf.cell - 1) / cf.cell;
         hl_cluster_params& cp = p->cparams[v];
-        hl_status st = hl_cluster_params_init(&d.camera, W, H, gx, gy, cf.gz, 0, &cp);
-        if (st != HL_OK) return fail(c, st, "view %u: camera is not a standard-Z perspective", v);
+        if (d.clusters) {
+            cp = *d.clusters;  // checked by hl_batch_plan_create
+        } else {
+            const uint32_t gx = (W + cf.cell - 1) / cf.cell, gy = (H + cf.cell - 1) / cf.cell;
+            hl_status st = hl_cluster_params_init(&d.camera, W, H, gx, gy, cf.gz, 0, &cp);
+            if (st != HL_OK) return fail(c, st, "view %u: camera is not a standard-Z perspective", v);
+        }
         const uint32_t n = p->nlights[v];
-        const uint32_t ncol = gx * gy, ncell = ncol * cf.gz;
+        const uint32_t ncol = cp.grid_x * cp.grid_y, ncell = ncol * cp.grid_z;
         ViewBufs& b = p->bufs[v];
         if ((s = plan_alloc(p, &b.hot, sizeof(HotLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
         if ((s = plan_alloc(p, &b.cull, sizeof(CullLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
@@ -208,7 +212,7 @@ hl_status build_plan(hl_plan* p) {
         cv.SW = (cv.W + 31u) / 32u;
         cv.ncell = ncell;
         light_base[v + 1] = light_base[v] + n;
-        cell_base[v + 1] = cell_base[v] + ncell;
+        cell_base[v + 1] = cell_base[v] + (ncell + kBatchCellAlign - 1u) / kBatchCellAlign * kBatchCellAlign;
         tile_base[v + 1] = tile_base[v] + cluster_scan_tiles(ncell);
     }
     p->total_lights = light_base[nv];
@@ -318,18 +322,29 @@ hl_status build_plan(hl_plan* p) {
     return HL_OK;
 }
 
+// Views of one class share a tuned configuration.  A view with a fixed
+// cluster grid keys on that grid too (only its wave tile is tuned).
 std::string class_key(const hl_view_desc& d, uint32_t n) {
-    char buf[128];
+    char buf[160];
     int bucket = n ? (int)floor(log2((double)n)) : -1;
-    snprintf(buf, sizeof buf, "gfx950 %u %u %d %d", d.gbuffer.width, d.gbuffer.height,
-             is_adjoint(d) ? 1 : 0, bucket);
+    if (d.clusters)
+        snprintf(buf, sizeof buf, "gfx950 %u %u %d %d fixed-%ux%ux%u-%u", d.gbuffer.width,
+                 d.gbuffer.height, is_adjoint(d) ? 1 : 0, bucket, d.clusters->grid_x,
+                 d.clusters->grid_y, d.clusters->grid_z, d.clusters->cap);
+    else
+        snprintf(buf, sizeof buf, "gfx950 %u %u %d %d", d.gbuffer.width, d.gbuffer.height,
+                 is_adjoint(d) ? 1 : 0, bucket);
     return buf;
 }
 
-std::vector<ViewCfg> candidates() {
+std::vector<ViewCfg> candidates(bool fixed_grid) {
     std::vector<ViewCfg> out;
     // default first, then the rest of the grid
     out.push_back({3, 64, 24, 0.0f});
+    if (fixed_grid) {
+        out.push_back({4, 64, 24, 0.0f});
+        return out;
+    }
     for (uint32_t tw : {3u, 4u})
         for (uint32_t cell : {32u, 64u, 128u})
             for (uint32_t gz : {16u, 24u, 32u, 48u})
@@ -380,22 +395,23 @@ hl_status time_config(hl_context* c, const GpuLight* pool, uint32_t pool_size,
     return HL_OK;
 }
 
+// One line per view class: "<class key> | <tile log2> <cell px> <slices> <ms>".
 void load_cache(const char* path, std::map<std::string, ViewCfg>& cache) {
     if (!path) return;
     FILE* f = fopen(path, "r");
     if (!f) return;
-    char line[256];
+    char line[512];
     while (fgets(line, sizeof line, f)) {
-        char arch[32];
-        unsigned w, h, tw, cell, gz;
-        int adj, bucket;
+        const char* bar = strchr(line, '|');
+        if (!bar || bar == line) continue;
+        unsigned tw, cell, gz;
         float ms;
-        if (sscanf(line, "%31s %u %u %d %d %u %u %u %f", arch, &w, &h, &adj, &bucket, &tw, &cell, &gz,
-                   &ms) == 9) {
-            char key[128];
-            snprintf(key, sizeof key, "%s %u %u %d %d", arch, w, h, adj, bucket);
-            cache[key] = ViewCfg{tw, cell, gz, ms};
-        }
+        if (sscanf(bar + 1, "%u %u %u %f", &tw, &cell, &gz, &ms) != 4 || tw < 3 || tw > 5 || !cell || !gz ||
+            gz > HL_MAX_SLICES)
+            continue;
+        std::string key(line, (size_t)(bar - line));
+        while (!key.empty() && key.back() == ' ') key.pop_back();
+        cache[key] = ViewCfg{tw, cell, gz, ms};
     }
     fclose(f);
 }
@@ -405,7 +421,7 @@ void save_cache(const char* path, const std::map<std::string, ViewCfg>& cache) {
     FILE* f = fopen(path, "w");
     if (!f) return;
     for (const auto& kv : cache)
-        fprintf(f, "%s %u %u %u %.6f\n", kv.first.c_str(), kv.second.tw_log2, kv.second.cell,
+        fprintf(f, "%s | %u %u %u %.6f\n", kv.first.c_str(), kv.second.tw_log2, kv.second.cell,
                 kv.second.gz, kv.second.ms);
     fclose(f);
 }
@@ -458,6 +474,16 @@ HL_API hl_status hl_batch_plan_create(hl_context* c, const GpuLight* pool_d, uin
             delete p;
             return fail(c, HL_EINVAL, "view %u: output pitch invalid", v);
         }
+        if (d.clusters) {
+            hl_status st = check_cluster_params(c, d.clusters);
+            if (st == HL_OK && (d.clusters->width != d.gbuffer.width || d.clusters->height != d.gbuffer.height))
+                st = fail(c, HL_EINVAL, "view %u: cluster grid is for %ux%u, view is %ux%u", v,
+                          d.clusters->width, d.clusters->height, d.gbuffer.width, d.gbuffer.height);
+            if (st != HL_OK) {
+                delete p;
+                return st;
+            }
+        }
         if (is_adjoint(d) && (!d.loss_d || !d.d_pixel_d || !d.d_light_pos_d || !d.d_light_intensity_d ||
                               d.target_pitch_bytes < 16u * d.gbuffer.width || d.target_pitch_bytes % 16)) {
             delete p;
@@ -466,17 +492,18 @@ HL_API hl_status hl_batch_plan_create(hl_context* c, const GpuLight* pool_d, uin
         p->nlights.push_back(n);
     }
     // configuration per view
-    const std::vector<ViewCfg> cands = candidates();
-    p->cfg.assign(num_views, cands[0]);
+    p->cfg.assign(num_views, candidates(false)[0]);
     if (opts && opts->enable) {
         std::map<std::string, ViewCfg> cache;
         load_cache(opts->cache_path, cache);
         bool dirty = false;
-        const size_t trials = opts->max_trials ? std::min<size_t>(opts->max_trials, cands.size()) : cands.size();
         for (uint32_t v = 0; v < num_views; ++v) {
             const std::string key = class_key(views[v], p->nlights[v]);
             auto it = cache.find(key);
             if (it == cache.end()) {
+                const std::vector<ViewCfg> cands = candidates(views[v].clusters != nullptr);
+                const size_t trials =
+                    opts->max_trials ? std::min<size_t>(opts->max_trials, cands.size()) : cands.size();
                 ViewCfg best = cands[0];
                 best.ms = 1e30f;
                 for (size_t k = 0; k < trials; ++k) {

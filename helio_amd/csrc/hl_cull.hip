@@ -735,6 +735,9 @@ __global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t W
 
 // ------------------------------------------------- all views in one launch
 // base[] holds nviews + 1 prefix offsets; returns the view owning index g.
+// Cell ranges start on multiples of kBatchCellAlign, so a block of the
+// geometry, count and fill kernels lies in one view: its lookup (and every
+// view field it reads) depends on blockIdx only and stays scalar.
 __device__ __forceinline__ uint32_t find_view(const uint32_t* __restrict__ base, uint32_t nviews,
                                               uint32_t g) {
     uint32_t lo = 0, hi = nviews;  // base[lo] <= g < base[hi]
@@ -766,8 +769,7 @@ __global__ __launch_bounds__(256) void k_batch_geom(const BatchCullView* __restr
                                                     uint32_t nviews, uint32_t* __restrict__ qcount) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < kMarkQueues) qcount[g * kMarkCountStride] = 0u;
-    if (g >= cell_base[nviews]) return;
-    const uint32_t vi = find_view(cell_base, nviews, g);
+    const uint32_t vi = find_view(cell_base, nviews, blockIdx.x * blockDim.x);
     const BatchCullView& v = views[vi];
     const uint32_t c = g - cell_base[vi];
     if (c < cluster_scan_tiles_dev(v.ncell))
@@ -779,7 +781,7 @@ __global__ __launch_bounds__(256) void k_batch_geom(const BatchCullView* __restr
 __global__ __launch_bounds__(256) void k_batch_mark(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ light_base,
                                                     uint32_t nviews, MarkQueue q) {
-    const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t g = hl_uniform(blockIdx.x * 4u + (threadIdx.x >> 6));
     if (g >= light_base[nviews]) return;
     const uint32_t vi = find_view(light_base, nviews, g);
     const BatchCullView& v = views[vi];
@@ -814,15 +816,14 @@ __global__ __launch_bounds__(256) void k_batch_count(const BatchCullView* __rest
                                                      uint32_t nviews) {
     __shared__ TileSumSmem sm;
     __shared__ CellSmem cs;
-    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes;
-    const bool on = g < cell_base[nviews];
-    const uint32_t vi = find_view(cell_base, nviews, on ? g : cell_base[nviews] - 1u);
+    constexpr uint32_t kCellsPerBlock = 256u / kCellLanes;
+    const uint32_t vi = find_view(cell_base, nviews, blockIdx.x * kCellsPerBlock);
     const BatchCullView& v = views[vi];
-    const uint32_t c = g - cell_base[vi];
-    const uint32_t n = count_cell(on ? v.ncell : 0u, v.W, v.SW, v.bits, v.summary, v.raw, c, cs);
+    const uint32_t c = blockIdx.x * kCellsPerBlock + threadIdx.x / kCellLanes - cell_base[vi];
+    const uint32_t n = count_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.raw, c, cs);
     unsigned long long* ts = (unsigned long long*)((uint8_t*)v.raw + cluster_tile_sums_offset(v.ncell));
     const uint32_t cap = v.ca.cap;
-    add_tile_sums(sm, on ? ts + c / kScanTile : nullptr, (cap && n > cap) ? cap : n);
+    add_tile_sums(sm, c < v.ncell ? ts + c / kScanTile : nullptr, (cap && n > cap) ? cap : n);
 }
 
 // one block per scan tile of any view
@@ -839,13 +840,12 @@ __global__ __launch_bounds__(256) void k_batch_scan(const BatchCullView* __restr
 __global__ __launch_bounds__(256) void k_batch_fill(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ cell_base,
                                                     uint32_t nviews) {
-    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes;
-    const bool on = g < cell_base[nviews];
-    const uint32_t vi = find_view(cell_base, nviews, on ? g : cell_base[nviews] - 1u);
+    constexpr uint32_t kCellsPerBlock = 256u / kCellLanes;
+    const uint32_t vi = find_view(cell_base, nviews, blockIdx.x * kCellsPerBlock);
     const BatchCullView& v = views[vi];
     __shared__ CellSmem cs;
-    fill_cell(on ? v.ncell : 0u, v.W, v.SW, v.bits, v.summary, v.offsets, v.counts, v.pool,
-              g - cell_base[vi], cs);
+    fill_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.offsets, v.counts, v.pool,
+              blockIdx.x * kCellsPerBlock + threadIdx.x / kCellLanes - cell_base[vi], cs);
 }
 
 // grid of the queued-chunk launches: 64 wavefronts per sub-queue, enough to
@@ -857,6 +857,7 @@ hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCul
                              const uint32_t* light_base, const uint32_t* cell_base,
                              const uint32_t* tile_base, uint32_t nviews, uint32_t total_lights,
                              uint32_t total_cells, uint32_t total_tiles, const MarkQueue& mq) {
+    if (total_cells % kBatchCellAlign) return hipErrorInvalidValue;
     hipError_t e;
     if (total_lights) {
         k_batch_prep<<<(total_lights + 255) / 256, 256, 0, s>>>(pool, views, light_base, nviews);
@@ -870,7 +871,7 @@ hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCul
         k_batch_mark_tail<<<tail_blocks(total_lights), 256, 0, s>>>(views, light_base, nviews, mq);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    const uint32_t cblocks = (total_cells + 256u / kCellLanes - 1u) / (256u / kCellLanes);
+    const uint32_t cblocks = total_cells / (256u / kCellLanes);  // total_cells: kBatchCellAlign multiple
     k_batch_count<<<cblocks, 256, 0, s>>>(views, cell_base, nviews);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_batch_scan<<<total_tiles, 256, 0, s>>>(views, tile_base, nviews);

@@ -132,6 +132,9 @@ inline size_t cluster_raw_bytes(uint32_t ncell) {
 }
 // One view of a batched cull (device memory): its cluster arguments, camera,
 // light subset, and its own per-view buffers.
+// Each view's cell range in a batched cull starts on a multiple of this (the
+// cells of one block of the geometry kernel), so no block spans two views.
+constexpr uint32_t kBatchCellAlign = 256;
 struct BatchCullView {
     ClusterArgs ca;
     const GpuCameraUniforms* cam;

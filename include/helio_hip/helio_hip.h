@@ -301,6 +301,10 @@ typedef struct hl_view_desc {
     float* d_pixel_d;                  /* 5 planes of width*height              */
     float* d_light_pos_d;              /* pool-sized: 3 floats per pool light   */
     float* d_light_intensity_d;        /* pool-sized                            */
+    /* Optional fixed cluster grid (from hl_cluster_params_init for this view's
+     * camera and size; cap honoured).  NULL = the plan chooses the grid (the
+     * autotuner, when enabled, searches it); set = only the wave tile is tuned. */
+    const hl_cluster_params* clusters;
 } hl_view_desc;
 
 typedef struct hl_autotune_opts {

@@ -9,9 +9,12 @@ Bars:
     relative + 1e-7 of the largest magnitude (FP64 atomics reorder sums);
   * each replay of the captured HIP graph equals the eager frame, with eager
     work interleaved between replays;
-  * the autotuner returns a valid configuration and its cache round-trips.
+  * the autotuner returns a valid configuration and its cache round-trips;
+  * views given a fixed cluster grid (with a per-cluster cap) get exactly that
+    grid, and equal the single-view path on it bit for bit.
 """
 import ctypes as C
+import dataclasses
 import os
 
 import numpy as np
@@ -36,11 +39,11 @@ def mv(ctx):
     return pool, pool_size, views, hosts, pool_lights
 
 
-def single_view(ctx, hlib, v, cfg):
+def single_view(ctx, hlib, v, cfg, cap=0):
     """The single-view path on the plan's cluster grid."""
     sc = v.scene.host
     sub_lights = sc.lights
-    p = synth.cluster_params(hlib, sc, grid=(cfg.grid_x, cfg.grid_y, cfg.grid_z), cap=0)
+    p = synth.cluster_params(hlib, sc, grid=(cfg.grid_x, cfg.grid_y, cfg.grid_z), cap=cap)
     lights = ctx.upload(sub_lights) if len(sub_lights) else ctx.alloc(128)
     cull = LightCullPass(ctx, sc.width, sc.height, clusters=p, pool_cap=1 << 22)
     cam = v.scene.camera
@@ -167,3 +170,44 @@ def test_plan_rejects_bad_subset(ctx, mv):
     with pytest.raises(abi.HelioError) as e:
         BatchPlan(ctx, pool, pool_size, [bv])
     assert e.value.status == abi.HL_EINVAL
+
+
+def test_fixed_cluster_grid(ctx, hlib, mv):
+    pool, pool_size, views, hosts, _ = mv
+    print("[batch] fixed cluster grids", flush=True)
+    idx = list(range(0, len(views), 7)) + [len(views) - 1]
+    sel, fixed = [], []
+    for k, i in enumerate(idx):
+        sc = views[i].scene.host
+        grid = ((sc.width + 31) // 32, (sc.height + 31) // 32, 16 if k % 2 else 40)
+        cap = 24 if k % 3 == 0 else 0
+        fixed.append((grid, cap))
+        sel.append(dataclasses.replace(views[i], clusters=synth.cluster_params(hlib, sc, grid=grid, cap=cap)))
+    plan = BatchPlan(ctx, pool, pool_size, sel, autotune=True, max_trials=2)
+    try:
+        for v in sel:
+            v.out.fill_u32(0)
+        plan.launch()
+        ctx.synchronize()
+        for k, (v, i) in enumerate(zip(sel, idx)):
+            cfg = plan.config(k)
+            assert (cfg.grid_x, cfg.grid_y, cfg.grid_z) == fixed[k][0]
+            got = read_view(v, pool_size, hosts[i][1])
+            ref = single_view(ctx, hlib, v, cfg, cap=fixed[k][1])
+            sky = v.scene.host.depth >= 1.0
+            assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
+            if v.adjoint:
+                assert close(got["dpx"], ref["dpx"]), f"view {i} pixel grads"
+                assert close(got["dint"], ref["dint"]), f"view {i} intensity grads"
+    finally:
+        plan.destroy()
+
+
+def test_fixed_cluster_grid_must_match_view(ctx, hlib, mv):
+    pool, pool_size, views, hosts, _ = mv
+    v = views[0]
+    sc = v.scene.host
+    p = synth.cluster_params(hlib, sc)
+    p.width = sc.width - 1  # still covered by the grid, but not this view's size
+    with pytest.raises(abi.HelioError, match="cluster grid is for"):
+        BatchPlan(ctx, pool, pool_size, [dataclasses.replace(v, clusters=p)])
