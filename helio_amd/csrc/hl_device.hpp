@@ -23,16 +23,20 @@ namespace hl {
 
 // ------------------------------------------------------------------ records
 
-// Per-light record the shading kernels read (64 B, one scalar load).  Built
-// from GpuLight by the prep kernel each call:
+// Per-light record the shading kernels read (80 B).  Built from GpuLight by
+// the prep kernel each call:
 //   p0 = (position.xyz, range)          range = +inf for directional lights
 //   p1 = (color.rgb * intensity, cos inner)
 //   p2 = (direction.xyz, cos outer)     cos outer = -2 for non-spot lights
 //   p3 = (1 / range^2, intensity, cone margin, 0)
+//   p4 = (range^2 (1 - 1e-5), range^2 (1 + 1e-5), ci, 3.61e-5 ci)
+//        with ci = max(p1.xyz): the range test's band and the 0.002 test's
+//        scale and error coefficient, so that the per-pixel loop does not
+//        recompute them
 // The cone margin, 1e-5 * |direction|_1, bounds (tenfold) the difference
 // between a fast cos(angle) and the exact one of deferred_lighting.wgsl:739.
 struct alignas(16) HotLight {
-    float4 p0, p1, p2, p3;
+    float4 p0, p1, p2, p3, p4;
 };
 
 // Per-light record the culling kernels read (32 B):
@@ -241,30 +245,68 @@ __device__ __forceinline__ float hl_unorm8(uint32_t v) {
 typedef unsigned int hl_u32x2 __attribute__((ext_vector_type(2)));
 typedef float hl_f32x4 __attribute__((ext_vector_type(4)));
 
+// Global-memory view of a device pointer.  Accesses through it compile to
+// global instructions even when the pointer itself was read from memory (a
+// batched view's record), where the compiler would otherwise emit flat ones,
+// which also count against the LDS/scalar wait counter.  Device memory only.
+template <class T>
+using hl_gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ hl_gptr<T> hl_g(T* p) {
+    return (hl_gptr<T>)p;
+}
+
 __device__ __forceinline__ void hl_store_nt(float4* p, float4 v) {
     hl_f32x4 w = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(w, reinterpret_cast<hl_f32x4*>(p));
+    __builtin_nontemporal_store(w, hl_g(reinterpret_cast<hl_f32x4*>(p)));
+}
+
+__device__ __forceinline__ float4 hl_ld4(const float4* p) {
+    const hl_f32x4 v = *hl_g(reinterpret_cast<const hl_f32x4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Texel fetch and decode are separate so that a caller can issue the loads of
+// every G-buffer texture before the first one is waited for.
+template <int FMT>
+struct TexRaw {
+    using type = hl_f32x4;
+};
+template <>
+struct TexRaw<HL_FMT_RGBA8_UNORM> {
+    using type = uint32_t;
+};
+template <>
+struct TexRaw<HL_FMT_RGBA16_FLOAT> {
+    using type = hl_u32x2;
+};
+
+template <int FMT>
+__device__ __forceinline__ typename TexRaw<FMT>::type hl_fetch(const Tex& t, uint32_t x, uint32_t y) {
+    using R = typename TexRaw<FMT>::type;
+    return __builtin_nontemporal_load(hl_g((const R*)(t.data + (size_t)y * t.pitch)) + x);
 }
 
 template <int FMT>
-__device__ __forceinline__ float4 hl_load_rgba(const Tex& t, uint32_t x, uint32_t y) {
-    const uint8_t* row = t.data + (size_t)y * t.pitch;
+__device__ __forceinline__ float4 hl_decode(typename TexRaw<FMT>::type v) {
     if constexpr (FMT == HL_FMT_RGBA8_UNORM) {
-        uint32_t v = __builtin_nontemporal_load((const uint32_t*)row + x);
         return make_float4(hl_unorm8(v & 0xffu), hl_unorm8((v >> 8) & 0xffu),
                            hl_unorm8((v >> 16) & 0xffu), hl_unorm8(v >> 24));
     } else if constexpr (FMT == HL_FMT_RGBA16_FLOAT) {
-        hl_u32x2 v = __builtin_nontemporal_load((const hl_u32x2*)row + x);
         return make_float4(hl_half((uint16_t)(v.x & 0xffffu)), hl_half((uint16_t)(v.x >> 16)),
                            hl_half((uint16_t)(v.y & 0xffffu)), hl_half((uint16_t)(v.y >> 16)));
     } else {
-        hl_f32x4 v = __builtin_nontemporal_load((const hl_f32x4*)row + x);
         return make_float4(v.x, v.y, v.z, v.w);
     }
 }
 
+template <int FMT>
+__device__ __forceinline__ float4 hl_load_rgba(const Tex& t, uint32_t x, uint32_t y) {
+    return hl_decode<FMT>(hl_fetch<FMT>(t, x, y));
+}
+
 __device__ __forceinline__ float hl_load_depth(const Tex& t, uint32_t x, uint32_t y) {
-    return __builtin_nontemporal_load((const float*)(t.data + (size_t)y * t.pitch) + x);
+    return __builtin_nontemporal_load(hl_g((const float*)(t.data + (size_t)y * t.pitch)) + x);
 }
 
 // -------------------------------------------------------------- wave ops

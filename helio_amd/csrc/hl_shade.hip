@@ -30,7 +30,7 @@ struct Pixel {
     float f0[3];
     float alb[3];
     float rough, metal, ao;
-    float ndv, kk, gv, ev, one_m;
+    float ndv, kk, kk1, gv, ev, one_m;  // kk1 = kk + 0.0001
     float kd[3];     // (1 - metallic) albedo / pi
     float a2, a2gv;  // roughness^4 and roughness^4 G1(NdV)
     bool f0_free[3];
@@ -41,24 +41,23 @@ struct Term {
     float lx, ly, lz;
     float rad[3];
     float ndl;
-    float d, inv_d, att0, fall, s, dsdcos;
+    float d, inv_d, att0, fall, s, dsdcos, atten;
     float hx, hy, hz, inv_h;
     float hdv, u, P, ndh, a2, den, Q, D, el, gl, G, den2;
     float F[3], B[3];
 };
 
 // Range test of deferred_lighting.wgsl:1128-1131 (dist > range rejects) from
-// the fast squared distance; inside the +-1e-5 relative band around range^2,
-// which covers the rounding of both sides many times over, the exact
-// restatement decides.  The band is rare: the exact branch is taken only when
-// some lane of the wave needs it.
-__device__ __forceinline__ bool range_test(float tx, float ty, float tz, float d2, float r) {
-    const float r2 = r * r;
-    bool in = d2 < r2 * (1.0f - 1e-5f);
-    const bool band = !in && !(d2 > r2 * (1.0f + 1e-5f));
+// the fast squared distance; inside the +-1e-5 relative band around range^2
+// (L.p4.xy), which covers the rounding of both sides many times over, the
+// exact restatement decides.  The band is rare: the exact branch is taken only
+// when some lane of the wave needs it.
+__device__ __forceinline__ bool range_test(const HotLight& L, float tx, float ty, float tz, float d2) {
+    bool in = d2 < L.p4.x;
+    const bool band = !in && !(d2 > L.p4.y);
     if (hl_ballot(band)) {
         hl_rare();
-        if (band) in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > r);
+        if (band) in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > L.p0.w);
     }
     return in;
 }
@@ -74,16 +73,18 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
         t.lx = dx * inv;
         t.ly = dy * inv;
         t.lz = dz * inv;
-        t.rad[0] = L.p1.x;
-        t.rad[1] = L.p1.y;
-        t.rad[2] = L.p1.z;
-        t.att0 = t.fall = t.s = 1.0f;
+        if (kFull) {
+            t.rad[0] = L.p1.x;
+            t.rad[1] = L.p1.y;
+            t.rad[2] = L.p1.z;
+        }
+        t.att0 = t.fall = t.s = t.atten = 1.0f;
         t.dsdcos = t.d = t.inv_d = 0.0f;
-        if (fmaxf(fmaxf(t.rad[0], t.rad[1]), t.rad[2]) < kThresh) return false;
+        if (L.p4.z < kThresh) return false;
     } else {
         const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
         const float d2 = tx * tx + ty * ty + tz * tz;
-        if (!range_test(tx, ty, tz, d2, L.p0.w)) return false;
+        if (!range_test(L, tx, ty, tz, d2)) return false;
         const float inv_d = hl_f_rsq(d2);
         const float d = d2 * inv_d;
         t.d = d;
@@ -123,15 +124,20 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
             }
         }
         const float atten = t.att0 * t.fall * t.s;
-        t.rad[0] = L.p1.x * atten;
-        t.rad[1] = L.p1.y * atten;
-        t.rad[2] = L.p1.z * atten;
-        const float m = fmaxf(fmaxf(t.rad[0], t.rad[1]), t.rad[2]);
-        // error bound of the fast radiance (rcp/rsq of att0, |fall| near 0)
-        const float ci = fmaxf(fmaxf(L.p1.x, L.p1.y), L.p1.z);
-        const float err = 8.0f * ci * t.att0 * 4e-6f * t.s + m * 4e-6f + 1e-12f;
-        if (m < kThresh - err) return false;
-        if (m < kThresh + err) {  // ambiguous: decide with the exact restatement
+        t.atten = atten;
+        if (kFull) {
+            t.rad[0] = L.p1.x * atten;
+            t.rad[1] = L.p1.y * atten;
+            t.rad[2] = L.p1.z * atten;
+        }
+        // max(rad) = ci * atten exactly (rounding is monotonic, atten >= 0).
+        // Error bound of the fast radiance (rcp/rsq of att0, |fall| near 0):
+        // 32e-6 ci att0 s + 4e-6 m <= 3.61e-5 ci att0 s, as m <= ci att0 s.
+        const float m = L.p4.z * atten;
+        const float err = L.p4.w * (t.att0 * t.s) + 1e-12f;
+        const float dm = m - kThresh;
+        if (dm < -err) return false;
+        if (fabsf(dm) < err) {  // ambiguous: decide with the exact restatement
             hl_rare();
             bool dummy;
             if (hl_x_radiance_max(L, p.wx, p.wy, p.wz, &dummy) < kThresh) return false;
@@ -139,6 +145,10 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     }
     t.ndl = fmaxf(p.nx * t.lx + p.ny * t.ly + p.nz * t.lz, 0.0f);
     if (t.ndl == 0.0f) return false;
+    // H = normalize(V + L) and N.H in the shader's order: for smooth surfaces
+    // (roughness^4 ~ 1e-5) the GGX denominator near the peak is as small as
+    // one ulp of N.H, so a rearranged N.H (e.g. (N.V + N.L) / |V + L|) moves D
+    // by percents there.
     const float hx = p.vx + t.lx, hy = p.vy + t.ly, hz = p.vz + t.lz;
     const float inv_h = hl_f_rsq(hx * hx + hy * hy + hz * hz);
     t.inv_h = inv_h;
@@ -146,14 +156,14 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     t.hy = hy * inv_h;
     t.hz = hz * inv_h;
     t.hdv = fmaxf(t.hx * p.vx + t.hy * p.vy + t.hz * p.vz, 0.0f);
+    t.ndh = fmaxf(p.nx * t.hx + p.ny * t.hy + p.nz * t.hz, 0.0f);
     t.u = fminf(fmaxf(1.0f - t.hdv, 0.0f), 1.0f);
     const float u2 = t.u * t.u;
     t.P = u2 * u2 * t.u;
     t.a2 = p.a2;
-    t.ndh = fmaxf(p.nx * t.hx + p.ny * t.hy + p.nz * t.hz, 0.0f);
     t.den = t.ndh * t.ndh * (p.a2 - 1.0f) + 1.0f;
     t.Q = kPi * t.den * t.den + 0.0001f;
-    t.el = t.ndl * (1.0f - p.kk) + p.kk + 0.0001f;
+    t.el = t.ndl * (1.0f - p.kk) + p.kk1;
     t.den2 = 4.0f * p.ndv * t.ndl + 0.0001f;
     float spec;
     if (kFull) {
@@ -279,9 +289,9 @@ __device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, u
         const float bx0 = dpp_min_if(mine, p.wx), bx1 = dpp_max_if(mine, p.wx);
         const float by0 = dpp_min_if(mine, p.wy), by1 = dpp_max_if(mine, p.wy);
         const float bz0 = dpp_min_if(mine, p.wz), bz1 = dpp_max_if(mine, p.wz);
-        uint32_t cnt = hl_uniform(a.counts[ucell]);
+        uint32_t cnt = hl_uniform(hl_g(a.counts)[ucell]);
         uint32_t base = GRID == HL_GRID_TILES ? ucell * HL_MAX_LIGHTS_PER_TILE
-                                              : hl_uniform(a.offsets[ucell]);
+                                              : hl_uniform(hl_g(a.offsets)[ucell]);
         if (base >= a.lists_capacity) cnt = 0;
         else if (cnt > a.lists_capacity - base) cnt = a.lists_capacity - base;
         const uint32_t* list = a.lists + base;
@@ -289,10 +299,18 @@ __device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, u
             const uint32_t j = j0 + lane;
             bool pass = false;
             uint32_t idx = 0;
+            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), l1 = q, l2 = q, l3 = q, l4 = q;
             if (j < cnt) {
-                idx = list[j];
+                idx = hl_g(list)[j];
                 if (idx < a.num_lights) {
-                    const float4 q = a.lights[idx].p0;
+                    // the whole record now, so that staging it does not wait
+                    // on a second load after the test
+                    const HotLight* hp = a.lights + idx;
+                    q = hl_ld4(&hp->p0);
+                    l1 = hl_ld4(&hp->p1);
+                    l2 = hl_ld4(&hp->p2);
+                    l3 = hl_ld4(&hp->p3);
+                    l4 = hl_ld4(&hp->p4);
                     const float dx = fmaxf(fmaxf(bx0 - q.x, q.x - bx1), 0.0f);
                     const float dy = fmaxf(fmaxf(by0 - q.y, q.y - by1), 0.0f);
                     const float dz = fmaxf(fmaxf(bz0 - q.z, q.z - bz1), 0.0f);
@@ -303,7 +321,11 @@ __device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, u
             const uint64_t m = hl_ballot(pass);
             if (pass) {
                 const uint32_t pos = hl_prefix(m);
-                s_l[pos] = a.lights[idx];
+                s_l[pos].p0 = q;
+                s_l[pos].p1 = l1;
+                s_l[pos].p2 = l2;
+                s_l[pos].p3 = l3;
+                s_l[pos].p4 = l4;
                 s_i[pos] = idx;
             }
             const uint32_t n = (uint32_t)__popcll(m);
@@ -321,14 +343,38 @@ __device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, u
 // -------------------------------------------------------------- block tile
 // Shades block tile b of a view: 4 wavefronts side by side, each owning a
 // TW x (64/TW) pixel tile (TW = 1 << a.tw_log2: 8x8, 16x4 or 32x2).
+// Per-view constants of the pixel set-up, staged in LDS once per view rather
+// than read per tile: the camera terms and the slice thresholds.
+struct ViewConst {
+    float ivp[16];                   // inv_view_proj
+    float eye[3];                    // position_near.xyz
+    float p22, p32;                  // proj[10], proj[14]
+    float split[HL_MAX_SLICES + 1];  // depth_split
+};
+
 struct ShadeSmem {
     HotLight light[4][kStage];
     uint32_t index[4][kStage];
+    ViewConst vc;
 };
+
+// Fills sm.vc from the view's arguments; the caller synchronises the block
+// before the first tile reads it.
+__device__ __forceinline__ void load_view_const(const ShadeArgs& a, ShadeSmem& sm) {
+    const GpuCameraUniforms* cam = a.cam;
+    const uint32_t t = threadIdx.x;
+    if (t < 16u) sm.vc.ivp[t] = cam->inv_view_proj[t];
+    else if (t < 19u) sm.vc.eye[t - 16u] = cam->position_near[t - 16u];
+    else if (t == 19u) sm.vc.p22 = cam->proj[10];
+    else if (t == 20u) sm.vc.p32 = cam->proj[14];
+    else if (t >= 64u && t <= 64u + HL_MAX_SLICES) sm.vc.split[t - 64u] = a.depth_split[t - 64u];
+}
 
 template <int FA, int FN, int GRID, int MODE>
 __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const ViewConst& vc = sm.vc;
+    b = hl_uniform(b);  // the item is the block's: its tile arithmetic stays scalar
+    const uint32_t lane = threadIdx.x & 63u, wave = hl_uniform(threadIdx.x >> 6);
     const uint32_t twl = a.tw_log2, tw = 1u << twl;
     const uint32_t th = 64u >> twl;
     const uint32_t bx = b % a.block_tiles_x, by = b / a.block_tiles_x;
@@ -348,20 +394,26 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         if (kWrites && (a.flags & HL_SHADE_CLEAR_SKY))
             hl_store_nt(out_px, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         if (kAdj)
-            for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) a.d_pixel[k * plane + pix] = 0.0f;
+            for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) hl_g(a.d_pixel)[k * plane + pix] = 0.0f;
     }
     if (hl_ballot(valid) == 0) return;
 
     // ---- per-pixel set-up (deferred_lighting.wgsl:999-1086)
     Pixel p;
-    float4 A = make_float4(0, 0, 0, 0), Nr = make_float4(0, 1, 0, 0), O = A, E = A;
+    // the four fetches are all issued before the first is waited for
+    typename TexRaw<FA>::type rA{}, rO{};
+    typename TexRaw<FN>::type rN{}, rE{};
     if (valid) {
-        A = hl_load_rgba<FA>(a.albedo, x, y);
-        Nr = hl_load_rgba<FN>(a.normal, x, y);
-        O = hl_load_rgba<FA>(a.orm, x, y);
-        E = hl_load_rgba<FN>(a.emissive, x, y);
+        rA = hl_fetch<FA>(a.albedo, x, y);
+        rN = hl_fetch<FN>(a.normal, x, y);
+        rO = hl_fetch<FA>(a.orm, x, y);
+        rE = hl_fetch<FN>(a.emissive, x, y);
     }
-    const GpuCameraUniforms* cam = a.cam;
+    __builtin_amdgcn_sched_barrier(0);
+    // zero bits decode to 0 in every format; lanes without a pixel keep N = +y
+    const float4 A = hl_decode<FA>(rA), O = hl_decode<FA>(rO), E = hl_decode<FN>(rE);
+    float4 Nr = hl_decode<FN>(rN);
+    if (!valid) Nr = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
     {
         // world position exactly as deferred_lighting.wgsl:1003-1008: the
         // range, cone and threshold decisions depend on it
@@ -369,15 +421,14 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         const float u = ((float)x + 0.5f) / (float)a.width;
         const float v = ((float)y + 0.5f) / (float)a.height;
         const float ndx = u * 2.0f - 1.0f, ndy = 1.0f - v * 2.0f;
-        const float* m = cam->inv_view_proj;
+        const float* m = vc.ivp;
         const float hw = hl_x_mat_row(m, 3, ndx, ndy, depth, 1.0f);
         p.wx = hl_x_mat_row(m, 0, ndx, ndy, depth, 1.0f) / hw;
         p.wy = hl_x_mat_row(m, 1, ndx, ndy, depth, 1.0f) / hw;
         p.wz = hl_x_mat_row(m, 2, ndx, ndy, depth, 1.0f) / hw;
     }
     {
-        const float ex = cam->position_near[0] - p.wx, ey = cam->position_near[1] - p.wy,
-                    ez = cam->position_near[2] - p.wz;
+        const float ex = vc.eye[0] - p.wx, ey = vc.eye[1] - p.wy, ez = vc.eye[2] - p.wz;
         const float in_ = hl_f_rsq(Nr.x * Nr.x + Nr.y * Nr.y + Nr.z * Nr.z);
         const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
         p.nx = Nr.x * in_;
@@ -427,6 +478,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     {
         const float r1 = p.rough + 1.0f;
         p.kk = r1 * r1 * 0.125f;
+        p.kk1 = p.kk + 0.0001f;
         p.ev = p.ndv * (1.0f - p.kk) + p.kk + 0.0001f;
         p.gv = p.ndv * hl_f_rcp(p.ev);
         const float ra = p.rough * p.rough;
@@ -458,13 +510,19 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         uint32_t cz = 0;
         if (valid && a.grid_z > 1u) {
             // guess from the view distance, then settle on the exact thresholds
-            const float p22 = cam->proj[10], p32 = cam->proj[14];
-            const float dist = p32 * hl_f_rcp(depth + p22);
+            const float dist = vc.p32 * hl_f_rcp(depth + vc.p22);
             const float g = __builtin_amdgcn_logf(fmaxf(dist, 1e-30f)) * a.log_scale + a.log_bias;
+            const int kmax = (int)a.grid_z - 1;
             int k = (int)floorf(g);
-            k = k < 0 ? 0 : (k > (int)a.grid_z - 1 ? (int)a.grid_z - 1 : k);
-            while (k > 0 && a.depth_split[k] > depth) --k;
-            while (k < (int)a.grid_z - 1 && a.depth_split[k + 1] <= depth) ++k;
+            k = k < 0 ? 0 : (k > kmax ? kmax : k);
+            // the guess is almost always right: both of its thresholds at once
+            const float* split = vc.split;
+            const float lo = split[k], hi = split[k + 1];
+            if (!((k == 0 || lo <= depth) && (k == kmax || hi > depth))) {
+                hl_rare();
+                while (k > 0 && split[k] > depth) --k;
+                while (k < kmax && split[k + 1] <= depth) ++k;
+            }
             cz = (uint32_t)k;
         }
         cell = cx + a.grid_x * (cy + a.grid_y * cz);
@@ -483,7 +541,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
                                   return;
                               }
                               const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
-                              const bool in = range_test(tx, ty, tz, tx * tx + ty * ty + tz * tz, L.p0.w);
+                              const bool in = range_test(L, tx, ty, tz, tx * tx + ty * ty + tz * tz);
                               pairs += in ? 1u : 0u;
                           });
         // exact: at most 64 pixels x lights per cell per wave, far below 2^24
@@ -507,9 +565,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             if (!mine) return;
             Term t;
             if (eval_term<false>(L, p, t)) {
-                const float w = t.ndl;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) lo[c] += t.B[c] * (t.rad[c] * w);
+                // radiance x NdL = p1 (atten NdL)
+                const float aw = t.atten * t.ndl;
+                lo[0] += t.B[0] * (L.p1.x * aw);
+                lo[1] += t.B[1] * (L.p1.y * aw);
+                lo[2] += t.B[2] * (L.p1.z * aw);
             }
         });
     }
@@ -526,7 +586,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         float l2 = 0.0f;
         if (valid) {
             hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
-            const float4 tg = *((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
+            const float4 tg = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
             const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
             gl[0] = 2.0f * e0;
             gl[1] = 2.0f * e1;
@@ -536,7 +596,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         const float wl = dpp_sum(l2);
         if (lane == 0) atomicAdd(a.loss, (double)wl);
     } else if (valid) {
-        const float4 g = *((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
+        const float4 g = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
         gl[0] = g.x;
         gl[1] = g.y;
         gl[2] = g.z;
@@ -564,11 +624,12 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     if (valid) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) dA[c] += gl[c] * mixv[c] * p.ao;
-        a.d_pixel[0 * plane + pix] = dA[0];
-        a.d_pixel[1 * plane + pix] = dA[1];
-        a.d_pixel[2 * plane + pix] = dA[2];
-        a.d_pixel[3 * plane + pix] = dR;
-        a.d_pixel[4 * plane + pix] = dM;
+        const hl_gptr<float> dp = hl_g(a.d_pixel);
+        dp[0 * plane + pix] = dA[0];
+        dp[1 * plane + pix] = dA[1];
+        dp[2 * plane + pix] = dA[2];
+        dp[3 * plane + pix] = dR;
+        dp[4 * plane + pix] = dM;
     }
 }
 
@@ -580,7 +641,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 // address.  A block claims from its own sub-queue until it is drained, then
 // reads all counters at once (one per thread of the first wave) and moves to
 // the next sub-queue with items left; it calls f(item) for every item it
-// claims and returns when every sub-queue is drained.
+// claims and returns when every sub-queue is drained.  The claim of the next
+// item is issued before f runs on the current one, so its round trip overlaps
+// the tile's own loads.  Claims past a sub-queue's end only raise its counter.
 struct ClaimSmem {
     uint32_t item, q;
 };
@@ -595,18 +658,15 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
                                             ClaimSmem& cs, F&& f) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t q = blockIdx.x % kShadeQueues;
+    uint32_t pending = 0;  // thread 0: the claim in flight on sub-queue q
+    if (threadIdx.x == 0) pending = atomicAdd(queue + q * kShadeQueueStride, 1u);
     for (;;) {
-        const uint32_t nq = queue_len(num_items, q);
-        if (threadIdx.x == 0) {
-            uint32_t* ctr = queue + q * kShadeQueueStride;
-            // a drained sub-queue is seen without claiming from it
-            const uint32_t seen = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            cs.item = seen >= nq ? kNone : atomicAdd(ctr, 1u);
-        }
+        if (threadIdx.x == 0) cs.item = pending;
         __syncthreads();
         const uint32_t j = cs.item;
         __syncthreads();
-        if (j < nq) {
+        if (j < queue_len(num_items, q)) {
+            if (threadIdx.x == 0) pending = atomicAdd(queue + q * kShadeQueueStride, 1u);
             f(q + j * kShadeQueues);
             continue;
         }
@@ -628,6 +688,7 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
         q = cs.q;
         __syncthreads();
         if (q == kNone) return;
+        if (threadIdx.x == 0) pending = atomicAdd(queue + q * kShadeQueueStride, 1u);
     }
 }
 
@@ -635,12 +696,15 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
 // counters of the previous launch on the stream (finished by stream order)
 // are zeroed here for the next one, so consecutive launches alternate
 // between two counter sets without a separate reset.
+// Forward launches are held to 96 VGPRs (5 waves per SIMD): the tile's
+// memory round trips need the waves to hide them.
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
                                                uint32_t* __restrict__ queue_next) {
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
     if (blockIdx.x == 0 && threadIdx.x < kShadeQueues) queue_next[threadIdx.x * kShadeQueueStride] = 0u;
+    load_view_const(a, sm);  // published by the first claim's barrier
     claim_items(a.num_blocks, queue, cs, [&](uint32_t t) { shade_tile<FA, FN, GRID, MODE>(a, t, sm); });
 }
 
@@ -648,17 +712,31 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a, uint32_t* __restrict
 // ordered heaviest first by the host.  One launch covers the views of one
 // G-buffer format set and one mode (forward, or forward + L2 adjoint): the
 // adjoint needs about twice the registers, which would halve the forward's
-// occupancy in a shared kernel.
+// occupancy in a shared kernel.  When a block's item moves to another view
+// (rare: the items of one view are consecutive in each sub-queue) the view's
+// arguments and constants are staged in LDS; read from there they need fewer
+// registers than the same fields read from the view record in memory.
 template <int FA, int FN, int MODE>
-__global__ __launch_bounds__(256) void k_shade_batch(const BatchShadeView* __restrict__ views,
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_batch(const BatchShadeView* __restrict__ views,
                                                      const uint32_t* __restrict__ items,
                                                      uint32_t num_items,
                                                      uint32_t* __restrict__ queue) {
+    static_assert(sizeof(ShadeArgs) % 4 == 0, "copied as words");
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
+    __shared__ ShadeArgs sa;
+    uint32_t cur = 0xFFFFFFFFu;
     claim_items(num_items, queue, cs, [&](uint32_t it) {
-        const uint32_t packed = items[it];
-        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(views[packed >> 24].sa, packed & 0xFFFFFFu, sm);
+        const uint32_t packed = hl_uniform(items[it]);
+        const uint32_t v = packed >> 24;
+        if (v != cur) {  // block-uniform: every thread read the same item
+            cur = v;
+            load_view_const(views[v].sa, sm);
+            const uint32_t* src = (const uint32_t*)&views[v].sa;
+            for (uint32_t i = threadIdx.x; i < sizeof(ShadeArgs) / 4; i += blockDim.x) ((uint32_t*)&sa)[i] = src[i];
+            __syncthreads();
+        }
+        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(sa, packed & 0xFFFFFFu, sm);
     });
 }
 

@@ -104,6 +104,7 @@ def cluster_grid_host(params, off, cnt, pool):
     g.lists_d, g.counts_d, g.offsets_d = pool.ctypes.data, cnt.ctypes.data, off.ctypes.data
     g.clusters = C.pointer(params)
     g.lists_capacity = pool.size
+    g._arrays = (off, cnt, pool)  # the struct holds raw addresses: keep the arrays alive with it
     return g
 
 
