@@ -22,6 +22,7 @@ HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA32_FLOAT, HL_FMT_R32_FLOAT, 
 HL_GRID_TILES, HL_GRID_CLUSTERS = 1, 2
 HL_MAX_SLICES = 64
 HL_SHADE_CLEAR_SKY = 1
+HL_SHADE_LIGHTS_FROM_CULL = 2
 HL_OVERFLOW_CAP, HL_OVERFLOW_POOL = 1, 2
 HL_PIXEL_GRAD_PLANES = 5
 

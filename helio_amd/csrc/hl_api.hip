@@ -230,20 +230,33 @@ hl_status build_shade_args(hl_context* c, const GpuCameraUniforms* camera_d,
 
 namespace {
 
-// Shared set-up of the shading entry points: arguments, then the light records.
+// Shared set-up of the shading entry points: arguments, then the light records
+// (built here, or with reuse those the context's last light cull built from
+// the same lights).
 hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
                         const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights,
-                        const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0) {
+                        const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0,
+                        bool reuse = false) {
     if (!c) return HL_EINVAL;
     if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
     hl_status s = build_shade_args(c, camera_d, g, gb, num_lights, grid, a, set, need_debug0);
     if (s != HL_OK) return s;
-    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(num_lights ? num_lights : 1))) != HL_OK) return s;
     if ((s = ensure(c, c->words, 64)) != HL_OK) return s;
-    a->lights = (const HotLight*)c->hot.p;
     HL_HIP(c, hipSetDevice(c->device));
+    if (reuse) {
+        if (!c->hot.p || c->hot_src != lights_d || c->hot_n != num_lights)
+            return fail(c, HL_EINVAL, "HL_SHADE_LIGHTS_FROM_CULL: the last light cull on this context was "
+                                      "not over these %u lights", num_lights);
+        a->lights = (const HotLight*)c->hot.p;
+        return HL_OK;
+    }
+    c->hot_src = nullptr;  // set again once the records are rebuilt
+    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(num_lights ? num_lights : 1))) != HL_OK) return s;
+    a->lights = (const HotLight*)c->hot.p;
     HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, num_lights, camera_d, (HotLight*)c->hot.p,
                           nullptr, ctx_words(c) + 1));
+    c->hot_src = lights_d;
+    c->hot_n = num_lights;
     return HL_OK;
 }
 
@@ -487,11 +500,14 @@ HL_API hl_status hl_light_cull_tiles(hl_context* c, const GpuCameraUniforms* cam
     if (!camera_d || !lights_d) return fail(c, HL_EINVAL, "null camera or lights");
     if (params->screen_width == 0 || params->screen_height == 0)
         return fail(c, HL_EINVAL, "zero screen size");
+    c->hot_src = nullptr;  // set again once the records are rebuilt
     hl_status s = ensure(c, c->cull, sizeof(CullLight) * (size_t)params->num_lights);
     if (s != HL_OK) return s;
     if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)params->num_lights)) != HL_OK) return s;
     HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, params->num_lights, camera_d,
                           (HotLight*)c->hot.p, (CullLight*)c->cull.p, ctx_words(c) + 1));
+    c->hot_src = lights_d;
+    c->hot_n = params->num_lights;
     HL_HIP(c, launch_cull_tiles(c->stream, camera_d, *params, (const CullLight*)c->cull.p, lists_d,
                                 counts_d));
     return HL_OK;
@@ -511,7 +527,9 @@ HL_API hl_status hl_light_cull_clusters(hl_context* c, const GpuCameraUniforms* 
     const uint32_t ncol = p->grid_x * p->grid_y;
     const size_t ncell = (size_t)ncol * p->grid_z;
     const uint32_t n = num_lights;
+    c->hot_src = nullptr;  // set again once the records are rebuilt
     if ((s = ensure(c, c->cull, sizeof(CullLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
+    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(n ? n : 1))) != HL_OK) return s;
     if ((s = ensure(c, c->geom, cluster_geom_bytes(ncol, (uint32_t)ncell))) != HL_OK) return s;
     if ((s = ensure(c, c->bits, 4 * cluster_bits_words(ncell, n), true)) != HL_OK) return s;
     if ((s = ensure(c, c->summary, 4 * cluster_summary_words(ncell, n), true)) != HL_OK) return s;
@@ -522,13 +540,13 @@ HL_API hl_status hl_light_cull_clusters(hl_context* c, const GpuCameraUniforms* 
     // scan: the caller's word when given, and always the context's copy
     uint32_t* flags = overflow_d ? overflow_d : ctx_words(c);
     uint32_t* flags2 = overflow_d ? ctx_words(c) : nullptr;
-    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, n, camera_d, nullptr, (CullLight*)c->cull.p,
-                          ctx_words(c) + 1));
     ClusterArgs a = make_cluster_args(p, n);
     ClusterScratch w{c->geom.p, (uint32_t*)c->bits.p, (uint32_t*)c->summary.p, (uint32_t*)c->raw.p,
                      mark_queue_at(c->markq.p, n)};
-    HL_HIP(c, launch_cluster_cull(c->stream, a, camera_d, (const CullLight*)c->cull.p, w, offsets_d,
-                                  counts_d, pool_d, pool_cap, flags, flags2));
+    HL_HIP(c, launch_cluster_cull(c->stream, a, camera_d, lights_d, (HotLight*)c->hot.p, (CullLight*)c->cull.p,
+                                  ctx_words(c) + 1, w, offsets_d, counts_d, pool_d, pool_cap, flags, flags2));
+    c->hot_src = lights_d;
+    c->hot_n = n;
     return HL_OK;
 }
 
@@ -548,7 +566,10 @@ HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* cam
     if (!out_d) return fail(c, HL_EINVAL, "output is null");
     ShadeArgs a;
     int set;
-    hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, num_lights, grid, &a, &set, false);
+    if (flags & ~(uint32_t)(HL_SHADE_CLEAR_SKY | HL_SHADE_LIGHTS_FROM_CULL))
+        return fail(c, HL_EINVAL, "unknown shading flags 0x%x", flags);
+    hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, num_lights, grid, &a, &set, false,
+                                (flags & HL_SHADE_LIGHTS_FROM_CULL) != 0);
     if (s != HL_OK) return s;
     if (out_pitch_bytes < 16u * gb->width || out_pitch_bytes % 16 != 0 || ((uintptr_t)out_d) % 16)
         return fail(c, HL_EINVAL, "output pitch %u invalid for width %u (RGBA32F, 16-B aligned)",

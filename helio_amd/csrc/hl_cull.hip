@@ -4,7 +4,8 @@
 //  k_cull_tiles       reference 16-px tile lists (light_cull.wgsl:123-180), one
 //                     wavefront per tile, ballot + prefix-popcount compaction
 //  k_cluster_geom     per column: NDC rectangle and lateral planes; per cell:
-//                     the view-space AABB
+//                     the view-space AABB; its blocks past the cells prepare
+//                     the lights (as k_prep_lights) in the same launch
 //  k_cluster_mark     light-parallel: each light bounds the cells it can reach
 //                     (projected bounding box x slice range); a wavefront tests
 //                     those candidates 64 at a time with the exact membership
@@ -66,6 +67,9 @@ __device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
                              hl_x_mat_row(v, 1, dro.x, dro.y, dro.z, 0.0f),
                              hl_x_mat_row(v, 2, dro.x, dro.y, dro.z, 0.0f),
                              type == HL_LIGHT_SPOT ? dro.w : kNoCone);
+        c.axis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        c.aux = c.axis;
+        if (c.cone.w != kNoCone && !hl_x_cone_light(c.cone, &c.axis, &c.aux.x)) c.cone.w = kNoCone;
         cull[i] = c;
     }
     if (flags) {
@@ -234,7 +238,7 @@ __device__ __forceinline__ bool cell_member(const CullLight& c, const ColumnGeom
     in &= hl_x_plane_pass(g.pl[2], c.vpos, rc);
     in &= hl_x_plane_pass(g.pl[3], c.vpos, rc);
     in &= hl_x_sphere_aabb(c.vpos, rc, lo, hi);
-    if (c.cone.w != kNoCone) in &= hl_x_cone_aabb(c.vpos, c.cone, pad, lo, hi);
+    if (c.cone.w != kNoCone) in &= hl_x_cone_cell(c.vpos, c.axis, c.aux.x, pad, lo, hi);
     return in;
 }
 
@@ -274,20 +278,22 @@ __device__ __forceinline__ bool light_cells(const ClusterArgs& a,
             // A lateral plane through the origin at angle alpha from the view
             // axis passes a centre up to rr / cos(alpha) beyond the column's
             // edge at the centre's own depth; 1/cos(alpha) <= sqrt(1 + 1/p00^2).
+            // fast arithmetic: a bound of candidates, widened by a whole cell
+            // on each side below, far beyond its rounding
             const float p00 = cam->proj[0], p11 = cam->proj[5];
-            const float fx = sqrtf(1.0f + 1.0f / (p00 * p00));
-            const float fy = sqrtf(1.0f + 1.0f / (p11 * p11));
-            const float iz = 1.0f / zc;
+            const float fx = __builtin_amdgcn_sqrtf(1.0f + hl_f_rcp(p00 * p00));
+            const float fy = __builtin_amdgcn_sqrtf(1.0f + hl_f_rcp(p11 * p11));
+            const float iz = hl_f_rcp(zc);
             const float nx0 = p00 * (L.vpos.x - rr * fx) * iz, nx1 = p00 * (L.vpos.x + rr * fx) * iz;
             const float ny0 = p11 * (L.vpos.y - rr * fy) * iz, ny1 = p11 * (L.vpos.y + rr * fy) * iz;
             const float fw = (float)a.width, fh = (float)a.height;
             const float px0 = (nx0 + 1.0f) * 0.5f * fw, px1 = (nx1 + 1.0f) * 0.5f * fw;
             const float py0 = (1.0f - ny1) * 0.5f * fh, py1 = (1.0f - ny0) * 0.5f * fh;
-            const float cw = (float)a.cell_w, ch = (float)a.cell_h;
-            x0 = (int)fmaxf(floorf(px0 / cw) - 1.0f, -1.0f);
-            x1 = (int)fminf(floorf(px1 / cw) + 1.0f, (float)gx);
-            y0 = (int)fmaxf(floorf(py0 / ch) - 1.0f, -1.0f);
-            y1 = (int)fminf(floorf(py1 / ch) + 1.0f, (float)gy);
+            const float icw = hl_f_rcp((float)a.cell_w), ich = hl_f_rcp((float)a.cell_h);
+            x0 = (int)fmaxf(floorf(px0 * icw) - 1.0f, -1.0f);
+            x1 = (int)fminf(floorf(px1 * icw) + 1.0f, (float)gx);
+            y0 = (int)fmaxf(floorf(py0 * ich) - 1.0f, -1.0f);
+            y1 = (int)fminf(floorf(py1 * ich) + 1.0f, (float)gy);
         }
     }
     x0 = clampi(x0, 0, gx - 1);
@@ -358,6 +364,9 @@ __device__ __forceinline__ CullLight load_light_uniform(const CullLight* __restr
                          hl_uniform(L.vpos.w));
     L.cone = make_float4(hl_uniform(L.cone.x), hl_uniform(L.cone.y), hl_uniform(L.cone.z),
                          hl_uniform(L.cone.w));
+    L.axis = make_float4(hl_uniform(L.axis.x), hl_uniform(L.axis.y), hl_uniform(L.axis.z),
+                         hl_uniform(L.axis.w));
+    L.aux.x = hl_uniform(L.aux.x);
     return L;
 }
 
@@ -649,7 +658,9 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
 
 // ------------------------------------------------------ one view per launch
 // The geometry kernel also resets the mark queue and the overflow words (it
-// runs before every kernel that reads them).
+// runs before every kernel that reads them).  Its blocks past the cells'
+// prepare the lights' records (independent work that the mark kernel needs
+// too), so that both take one launch.
 __global__ __launch_bounds__(256) void k_cluster_geom(ClusterArgs a,
                                                       const GpuCameraUniforms* __restrict__ cam,
                                                       ColumnGeom* __restrict__ geom,
@@ -658,7 +669,17 @@ __global__ __launch_bounds__(256) void k_cluster_geom(ClusterArgs a,
                                                       uint32_t* __restrict__ flags,
                                                       uint32_t* __restrict__ flags2,
                                                       unsigned long long* __restrict__ tile_sums,
-                                                      uint32_t ntiles) {
+                                                      uint32_t ntiles,
+                                                      const GpuLight* __restrict__ lights,
+                                                      HotLight* __restrict__ hot,
+                                                      CullLight* __restrict__ cl,
+                                                      uint32_t* __restrict__ feature_flags,
+                                                      uint32_t geom_blocks) {
+    if (blockIdx.x >= geom_blocks) {
+        prep_one(lights, nullptr, a.num_lights, cam, hot, cl, feature_flags,
+                 (blockIdx.x - geom_blocks) * blockDim.x + threadIdx.x);
+        return;
+    }
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < kMarkQueues) qcount[g * kMarkCountStride] = 0u;
     if (g == 0) {
@@ -759,20 +780,22 @@ __device__ __forceinline__ CellBox* view_boxes(const BatchCullView& v) {
     return (CellBox*)((uint8_t*)v.geom + cluster_geom_offset_boxes_dev(v.ca.grid_x * v.ca.grid_y));
 }
 
-__global__ __launch_bounds__(256) void k_batch_prep(const GpuLight* __restrict__ pool,
-                                                    const BatchCullView* __restrict__ views,
-                                                    const uint32_t* __restrict__ light_base,
-                                                    uint32_t nviews) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= light_base[nviews]) return;
-    const uint32_t vi = find_view(light_base, nviews, g);
-    const BatchCullView& v = views[vi];
-    prep_one(pool, v.subset, v.ca.num_lights, v.cam, v.hot, v.cull, nullptr, g - light_base[vi]);
-}
-
+// Cells of every view, then (blocks past geom_blocks) the lights of every
+// view gathered from the pool, in one launch.
 __global__ __launch_bounds__(256) void k_batch_geom(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ cell_base,
-                                                    uint32_t nviews, uint32_t* __restrict__ qcount) {
+                                                    uint32_t nviews, uint32_t* __restrict__ qcount,
+                                                    const GpuLight* __restrict__ pool,
+                                                    const uint32_t* __restrict__ light_base,
+                                                    uint32_t geom_blocks) {
+    if (blockIdx.x >= geom_blocks) {
+        const uint32_t l = (blockIdx.x - geom_blocks) * blockDim.x + threadIdx.x;
+        if (l >= light_base[nviews]) return;
+        const uint32_t vi = find_view(light_base, nviews, l);
+        const BatchCullView& v = views[vi];
+        prep_one(pool, v.subset, v.ca.num_lights, v.cam, v.hot, v.cull, nullptr, l - light_base[vi]);
+        return;
+    }
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < kMarkQueues) qcount[g * kMarkCountStride] = 0u;
     const uint32_t vi = find_view(cell_base, nviews, blockIdx.x * blockDim.x);
@@ -865,11 +888,9 @@ hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCul
                              uint32_t total_cells, uint32_t total_tiles, const MarkQueue& mq) {
     if (total_cells % kBatchCellAlign) return hipErrorInvalidValue;
     hipError_t e;
-    if (total_lights) {
-        k_batch_prep<<<(total_lights + 255) / 256, 256, 0, s>>>(pool, views, light_base, nviews);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    k_batch_geom<<<(total_cells + 255) / 256, 256, 0, s>>>(views, cell_base, nviews, mq.count);
+    const uint32_t geom_blocks = total_cells / 256u;  // total_cells: kBatchCellAlign multiple
+    k_batch_geom<<<geom_blocks + (total_lights + 255) / 256, 256, 0, s>>>(views, cell_base, nviews, mq.count,
+                                                                          pool, light_base, geom_blocks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (total_lights) {
         k_batch_mark<<<(total_lights + 3) / 4, 256, 0, s>>>(views, light_base, nviews, mq);
@@ -917,7 +938,8 @@ size_t cluster_geom_bytes(uint32_t ncol, uint32_t ncell) {
 }
 
 hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCameraUniforms* cam,
-                               const CullLight* cl, const ClusterScratch& w, uint32_t* offsets,
+                               const GpuLight* lights, HotLight* hot, CullLight* cl, uint32_t* feature_flags,
+                               const ClusterScratch& w, uint32_t* offsets,
                                uint32_t* counts, uint32_t* pool, uint32_t pool_cap,
                                uint32_t* flags, uint32_t* flags2) {
     const uint32_t ncol = a.grid_x * a.grid_y;
@@ -927,8 +949,10 @@ hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCam
     CellBox* boxes = (CellBox*)((uint8_t*)w.geom + cluster_geom_offset_boxes(ncol));
     const uint32_t ntiles = cluster_scan_tiles(ncell);
     unsigned long long* tile_sums = (unsigned long long*)((uint8_t*)w.raw + cluster_tile_sums_offset(ncell));
-    k_cluster_geom<<<(ncell + 255) / 256, 256, 0, s>>>(a, cam, geom, boxes, w.mq.count, flags, flags2,
-                                                       tile_sums, ntiles);
+    const uint32_t geom_blocks = (ncell + 255) / 256;
+    k_cluster_geom<<<geom_blocks + (a.num_lights + 255) / 256, 256, 0, s>>>(
+        a, cam, geom, boxes, w.mq.count, flags, flags2, tile_sums, ntiles, lights, hot, cl, feature_flags,
+        geom_blocks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.num_lights) {

@@ -42,8 +42,13 @@ struct alignas(16) HotLight {
 // Per-light record the culling kernels read (32 B):
 //   vpos = (view-space position, range)  (range = +inf: directional, always listed)
 //   cone = (view-space direction (not normalized), cos outer)  (cos outer = -2: no cone)
+//   axis = (unit view-space axis, cos of the outer half-angle clamped to 1),
+//   aux  = (sin of that angle, 0, 0, 0): the light-only terms of the cone
+//          test, computed once by the prep kernel; a spot light whose cone
+//          cannot reject anything (zero axis or half-angle >= 90 degrees)
+//          has cone.w = -2 like the other lights
 struct alignas(16) CullLight {
-    float4 vpos, cone;
+    float4 vpos, cone, axis, aux;
 };
 
 constexpr float kNoCone = -2.0f;
@@ -160,25 +165,33 @@ __device__ __forceinline__ bool hl_x_sphere_aabb(float4 c, float r, float3 lo, f
     return d2 <= r * r;
 }
 
-// Cone against the AABB's bounding sphere (see the cluster membership rule).
-__device__ __forceinline__ bool hl_x_cone_aabb(float4 c, float4 cone, float pad, float3 lo,
-                                               float3 hi) {
+// Cone against the AABB's bounding sphere (see the cluster membership rule),
+// in two parts.  The light part (axis length, unit axis, cos and sin of the
+// half-angle) returns false when the cone cannot reject anything.
+__device__ __forceinline__ bool hl_x_cone_light(float4 cone, float4* axis, float* sin_t) {
 #pragma clang fp contract(off)
     float alen = sqrtf(hl_x_dot3(cone.x, cone.y, cone.z, cone.x, cone.y, cone.z));
-    if (!(alen > 0.0f)) return true;
+    if (!(alen > 0.0f)) return false;
     float cos_t = cone.w / alen;
-    if (!(cos_t > 0.0f)) return true;
+    if (!(cos_t > 0.0f)) return false;
     if (cos_t > 1.0f) cos_t = 1.0f;
-    float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    *sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    *axis = make_float4(cone.x / alen, cone.y / alen, cone.z / alen, cos_t);
+    return true;
+}
+
+// The cell part, from the light part's axis = (unit axis, cos) and sin.
+__device__ __forceinline__ bool hl_x_cone_cell(float4 c, float4 axis, float sin_t, float pad, float3 lo,
+                                               float3 hi) {
+#pragma clang fp contract(off)
     float mx = (lo.x + hi.x) * 0.5f, my = (lo.y + hi.y) * 0.5f, mz = (lo.z + hi.z) * 0.5f;
     float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
     float R = sqrtf(hl_x_dot3(ex, ey, ez, ex, ey, ez)) * 0.5f + pad;
     float vx = mx - c.x, vy = my - c.y, vz = mz - c.z;
-    float ax = cone.x / alen, ay = cone.y / alen, az = cone.z / alen;
     float vlen2 = hl_x_dot3(vx, vy, vz, vx, vy, vz);
-    float v1 = hl_x_dot3(vx, vy, vz, ax, ay, az);
+    float v1 = hl_x_dot3(vx, vy, vz, axis.x, axis.y, axis.z);
     float perp = sqrtf(fmaxf(0.0f, vlen2 - v1 * v1));
-    float dcp = cos_t * perp - v1 * sin_t;
+    float dcp = axis.w * perp - v1 * sin_t;
     if (dcp > R) return false;
     if (v1 < -R) return false;
     return true;

@@ -35,6 +35,9 @@ struct hl_context {
     std::string err;
     DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum;
     uint32_t shade_parity = 0;  // which half of shadeq the next shading launch claims from
+    // the lights the records in `hot` were last built from (stream order)
+    const GpuLight* hot_src = nullptr;
+    uint32_t hot_n = 0;
     // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
 };
 
@@ -156,8 +159,11 @@ hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCul
                              const uint32_t* tile_base, uint32_t nviews, uint32_t total_lights,
                              uint32_t total_cells, uint32_t total_tiles, const MarkQueue& mq);
 
+// Prepares the lights' records (hot: shading, may be null; cl: culling; feature
+// flags) and builds the lists.
 hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCameraUniforms* cam,
-                               const CullLight* cl, const ClusterScratch& w, uint32_t* offsets,
+                               const GpuLight* lights, HotLight* hot, CullLight* cl, uint32_t* feature_flags,
+                               const ClusterScratch& w, uint32_t* offsets,
                                uint32_t* counts, uint32_t* pool, uint32_t pool_cap,
                                uint32_t* flags, uint32_t* flags2);
 

@@ -212,7 +212,12 @@ HL_API hl_status hl_query_overflow(hl_context* ctx, uint32_t* flags);
 
 /* ---------------------------------------------------------------- shading */
 enum {
-    HL_SHADE_CLEAR_SKY = 1u   /* write (0,0,0,0) to sky pixels (LoadOp::Clear); default leaves them */
+    HL_SHADE_CLEAR_SKY = 1u,        /* write (0,0,0,0) to sky pixels (LoadOp::Clear); default leaves them */
+    HL_SHADE_LIGHTS_FROM_CULL = 2u  /* use the light records that this context's last light cull built
+                                       (hl_light_cull_clusters / hl_light_cull_tiles over the same lights_d
+                                       and num_lights, the buffer unchanged since in stream order) instead
+                                       of building them again; HL_EINVAL when the last cull was over other
+                                       lights.  The records depend on the lights only, not the camera. */
 };
 
 /* DeferredLight fs_main over the G-buffer (deferred_lighting.wgsl:991-1259 with

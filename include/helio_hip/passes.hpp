@@ -269,7 +269,15 @@ public:
     std::vector<const char*> writes() const override { return {"pre_aa"}; }
 
     void set_debug_mode(uint32_t mode) { debug_mode_ = mode; }
-    void set_clear_sky(bool on) { flags_ = on ? (uint32_t)HL_SHADE_CLEAR_SKY : 0u; }
+    void set_clear_sky(bool on) {
+        flags_ = on ? (flags_ | (uint32_t)HL_SHADE_CLEAR_SKY) : (flags_ & ~(uint32_t)HL_SHADE_CLEAR_SKY);
+    }
+    // Use the light records the frame's LightCull (same context, same lights)
+    // already built, instead of building them again.
+    void set_lights_from_cull(bool on) {
+        flags_ = on ? (flags_ | (uint32_t)HL_SHADE_LIGHTS_FROM_CULL)
+                    : (flags_ & ~(uint32_t)HL_SHADE_LIGHTS_FROM_CULL);
+    }
 
     // DeferredGlobals as DeferredLightPass::prepare writes them (lib.rs:836-878)
     void prepare(const PrepareContext& pc) override {

@@ -168,6 +168,35 @@ def test_zero_lights(ctx):
     check_radiance(out, ref)
 
 
+def test_shade_lights_from_cull(ctx):
+    """HL_SHADE_LIGHTS_FROM_CULL: shading with the records the cluster cull
+    built equals shading that builds them (bit for bit); it is refused after a
+    cull over other lights."""
+    sc = scene("c1_720p_64")
+    ds = upload_scene(ctx, sc)
+    params = synth.cluster_params(ctx.lib, sc)
+    cull = LightCullPass(ctx, sc.width, sc.height, clusters=params, pool_cap=1 << 20)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    cull.prepare(ds)
+    shade.prepare(ds)
+    cull.execute(ds)
+    grid = cull.publish()
+    shade.execute(ds, grid)
+    ref = shade.output().copy()
+    cull.execute(ds)
+    shade.out.fill_u32(0)
+    shade.execute(ds, grid, flags=abi.HL_SHADE_LIGHTS_FROM_CULL)
+    got = shade.output()
+    lit = sc.depth < 1.0
+    assert np.array_equal(got[lit], ref[lit])
+    # a shading call that builds records for other lights, then reuse: refused
+    few = synth.make_scene("c1_720p_64", num_lights=3)
+    ds_few = upload_scene(ctx, few)
+    shade.execute(ds_few, grid)
+    with pytest.raises(abi.HelioError, match="LIGHTS_FROM_CULL"):
+        shade.execute(ds, grid, flags=abi.HL_SHADE_LIGHTS_FROM_CULL)
+
+
 @pytest.mark.parametrize("mode", [1, 2, 4, 5, 20, 21])
 def test_debug_views_parity(ctx, mode):
     """Debug views of fs_main (deferred_lighting.wgsl:1032-1038): albedo for
