@@ -63,14 +63,14 @@ struct hl_plan {
     uint32_t* d_light_base = nullptr;
     uint32_t* d_cell_base = nullptr;
     uint32_t* d_tile_base = nullptr;
-    uint32_t* d_items = nullptr;
+    uint2* d_spans = nullptr;  // per group: (first item, view) per view + (item count, 0)
     uint32_t* d_queue = nullptr;
     uint32_t* d_flags = nullptr;
     void* d_markq = nullptr;  // mark_queue_bytes(total_lights)
     uint32_t total_lights = 0, total_cells = 0, total_tiles = 0, num_items = 0;
     // shading items grouped by (G-buffer format set, mode): one persistent
     // launch per non-empty group; group = set * 2 + (adjoint ? 1 : 0)
-    uint32_t grp_first[6] = {}, grp_count[6] = {}, grp_grid[6] = {};
+    uint32_t grp_first[6] = {}, grp_spans[6] = {}, grp_count[6] = {}, grp_grid[6] = {};
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -103,12 +103,13 @@ void free_plan_device(hl_plan* p) {
     for (auto& b : p->bufs) free_bufs(b);
     for (void* q : {(void*)p->d_cull, (void*)p->d_shade, (void*)p->d_light_base, (void*)p->d_cell_base,
                     (void*)p->d_tile_base,
-                    (void*)p->d_items, (void*)p->d_queue, (void*)p->d_flags, p->d_markq})
+                    (void*)p->d_spans, (void*)p->d_queue, (void*)p->d_flags, p->d_markq})
         if (q) (void)hipFree(q);
     p->d_markq = nullptr;
     p->d_cull = nullptr;
     p->d_shade = nullptr;
-    p->d_light_base = p->d_cell_base = p->d_tile_base = p->d_items = p->d_queue = p->d_flags = nullptr;
+    p->d_light_base = p->d_cell_base = p->d_tile_base = p->d_queue = p->d_flags = nullptr;
+    p->d_spans = nullptr;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     p->exec = nullptr;
@@ -141,7 +142,7 @@ hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
     HL_HIP(c, launch_fill_u32(s, p->d_queue, 0, 6 * (size_t)kShadeQueueWords));
     for (int g = 0; g < 6; ++g)
         HL_HIP(c, launch_shade_batch(s, g / 2, (g & 1) ? kModeFusedL2 : kModeForward, p->d_shade,
-                                     p->d_items + p->grp_first[g], p->grp_count[g],
+                                     p->d_spans + p->grp_first[g], p->grp_spans[g], p->grp_count[g],
                                      p->d_queue + (size_t)g * kShadeQueueWords, p->grp_grid[g]));
     for (uint32_t v = 0; v < nv; ++v) {
         const hl_view_desc& d = p->views[v];
@@ -300,24 +301,34 @@ hl_status build_plan(hl_plan* p) {
                      [](const std::pair<double, uint32_t>& a, const std::pair<double, uint32_t>& b) {
                          return a.first > b.first;
                      });
-    std::vector<uint32_t> items;
+    // Items of a group: the block tiles of its views, view after view in the
+    // order above.  A group is described by spans (first item, view), one per
+    // view plus a closing (item count, 0), so an item's view and tile follow
+    // from its index.
+    std::vector<uint2> spans;
     int cus = 0;
     HL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    uint64_t all_items = 0;
     for (int g = 0; g < 6; ++g) {
-        p->grp_first[g] = (uint32_t)items.size();
+        p->grp_first[g] = (uint32_t)spans.size();
+        uint64_t n = 0;
         for (const auto& o : order) {
             const uint32_t v = o.second;
             if (vset[v] * 2 + (is_adjoint(p->views[v]) ? 1 : 0) != g) continue;
-            for (uint32_t blk = 0; blk < p->hshade[v].sa.num_blocks; ++blk) items.push_back((v << 24) | blk);
+            spans.push_back(make_uint2((uint32_t)n, v));
+            n += p->hshade[v].sa.num_blocks;
         }
-        p->grp_count[g] = (uint32_t)items.size() - p->grp_first[g];
+        if (n >= 0x80000000ull) return fail(c, HL_ECAPACITY, "%llu block tiles in one launch", (unsigned long long)n);
+        p->grp_spans[g] = (uint32_t)spans.size() - p->grp_first[g];
+        spans.push_back(make_uint2((uint32_t)n, 0u));
+        p->grp_count[g] = (uint32_t)n;
+        all_items += n;
         // 8 blocks of 4 waves per CU, fewer when there are fewer items
         p->grp_grid[g] = std::min((uint32_t)std::max(1, cus) * 8u, std::max(1u, p->grp_count[g]));
     }
-    p->num_items = (uint32_t)items.size();
-    if ((s = plan_alloc(p, (void**)&p->d_items, 4 * (items.empty() ? 1 : items.size()))) != HL_OK) return s;
-    if (!items.empty())
-        HL_HIP(c, hipMemcpy(p->d_items, items.data(), 4 * items.size(), hipMemcpyHostToDevice));
+    p->num_items = (uint32_t)all_items;
+    if ((s = plan_alloc(p, (void**)&p->d_spans, sizeof(uint2) * spans.size())) != HL_OK) return s;
+    HL_HIP(c, hipMemcpy(p->d_spans, spans.data(), sizeof(uint2) * spans.size(), hipMemcpyHostToDevice));
     if ((s = plan_alloc(p, (void**)&p->d_queue, 4 * 6 * (size_t)kShadeQueueWords)) != HL_OK) return s;
     return HL_OK;
 }

@@ -185,8 +185,9 @@ inline void set_tile_shape(ShadeArgs& a, uint32_t tw_log2) {
     a.block_tiles_x = (a.width + bw - 1) / bw;
     a.num_blocks = a.block_tiles_x * ((a.height + bh - 1) / bh);
 }
+// spans: nspans (first item, view) pairs in item order, then (num_items, 0).
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
-                              const uint32_t* items, uint32_t num_items, uint32_t* queue,
+                              const uint2* spans, uint32_t nspans, uint32_t num_items, uint32_t* queue,
                               uint32_t grid_blocks);
 // Per-light FP64 gradient accumulators: kGradSlots slot-major copies of 4
 // doubles (dpos.xyz, dI) per light, summed by launch_light_grads_to_f32.

@@ -708,64 +708,71 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade(Sha
     claim_items(a.num_blocks, queue, cs, [&](uint32_t t) { shade_tile<FA, FN, GRID, MODE>(a, t, sm); });
 }
 
-// Many views, one persistent launch: the items are (view, block tile) pairs
-// ordered heaviest first by the host.  One launch covers the views of one
-// G-buffer format set and one mode (forward, or forward + L2 adjoint): the
-// adjoint needs about twice the registers, which would halve the forward's
-// occupancy in a shared kernel.  When a block's item moves to another view
-// (rare: the items of one view are consecutive in each sub-queue) the view's
-// arguments and constants are staged in LDS; read from there they need fewer
-// registers than the same fields read from the view record in memory.
+// Many views, one persistent launch: the items are the block tiles of the
+// views, view after view (heaviest first by the host), described by spans
+// (first item, view); an item's view and tile follow from its index.  One
+// launch covers the views of one G-buffer format set and one mode (forward,
+// or forward + L2 adjoint): the adjoint needs about twice the registers,
+// which would halve the forward's occupancy in a shared kernel.  A block
+// keeps the item range of its current view; when an item falls outside it
+// (rare: each sub-queue's items run through one view at a time) the block
+// finds the span by bisection and stages that view's arguments and constants
+// in LDS, where they need fewer registers than read from the view record.
 template <int FA, int FN, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_batch(const BatchShadeView* __restrict__ views,
-                                                     const uint32_t* __restrict__ items,
-                                                     uint32_t num_items,
-                                                     uint32_t* __restrict__ queue) {
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_batch(
+    const BatchShadeView* __restrict__ views, const uint2* __restrict__ spans, uint32_t nspans,
+    uint32_t num_items, uint32_t* __restrict__ queue) {
     static_assert(sizeof(ShadeArgs) % 4 == 0, "copied as words");
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
     __shared__ ShadeArgs sa;
-    uint32_t cur = 0xFFFFFFFFu;
+    uint32_t first = 1u, end = 0u;  // item range of the staged view (none yet)
     claim_items(num_items, queue, cs, [&](uint32_t it) {
-        const uint32_t packed = hl_uniform(items[it]);
-        const uint32_t v = packed >> 24;
-        if (v != cur) {  // block-uniform: every thread read the same item
-            cur = v;
+        it = hl_uniform(it);  // block-uniform: every thread read the same claim
+        if (it < first || it >= end) {
+            uint32_t lo = 0, hi = nspans;  // spans[lo].x <= it < spans[hi].x
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (hl_uniform(spans[mid].x) <= it) lo = mid;
+                else hi = mid;
+            }
+            first = hl_uniform(spans[lo].x);
+            end = hl_uniform(spans[lo + 1].x);
+            const uint32_t v = hl_uniform(spans[lo].y);
             load_view_const(views[v].sa, sm);
             const uint32_t* src = (const uint32_t*)&views[v].sa;
             for (uint32_t i = threadIdx.x; i < sizeof(ShadeArgs) / 4; i += blockDim.x) ((uint32_t*)&sa)[i] = src[i];
             __syncthreads();
         }
-        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(sa, packed & 0xFFFFFFu, sm);
+        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(sa, it - first, sm);
     });
 }
 
 template <int FA, int FN>
-static void launch_batch_fmt(hipStream_t s, int mode, const BatchShadeView* views,
-                             const uint32_t* items, uint32_t num_items, uint32_t* queue,
-                             uint32_t grid_blocks) {
+static void launch_batch_fmt(hipStream_t s, int mode, const BatchShadeView* views, const uint2* spans,
+                             uint32_t nspans, uint32_t num_items, uint32_t* queue, uint32_t grid_blocks) {
     if (mode == kModeFusedL2)
-        k_shade_batch<FA, FN, kModeFusedL2><<<grid_blocks, 256, 0, s>>>(views, items, num_items, queue);
+        k_shade_batch<FA, FN, kModeFusedL2><<<grid_blocks, 256, 0, s>>>(views, spans, nspans, num_items, queue);
     else
-        k_shade_batch<FA, FN, kModeForward><<<grid_blocks, 256, 0, s>>>(views, items, num_items, queue);
+        k_shade_batch<FA, FN, kModeForward><<<grid_blocks, 256, 0, s>>>(views, spans, nspans, num_items, queue);
 }
 
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
-                              const uint32_t* items, uint32_t num_items, uint32_t* queue,
+                              const uint2* spans, uint32_t nspans, uint32_t num_items, uint32_t* queue,
                               uint32_t grid_blocks) {
     if (num_items == 0) return hipSuccess;
     switch (gbuf_set) {
         case kSetRef:
-            launch_batch_fmt<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT>(s, mode, views, items, num_items, queue,
-                                                                    grid_blocks);
+            launch_batch_fmt<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT>(s, mode, views, spans, nspans, num_items,
+                                                                    queue, grid_blocks);
             break;
         case kSet16:
-            launch_batch_fmt<HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA16_FLOAT>(s, mode, views, items, num_items, queue,
-                                                                     grid_blocks);
+            launch_batch_fmt<HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA16_FLOAT>(s, mode, views, spans, nspans, num_items,
+                                                                     queue, grid_blocks);
             break;
         default:
-            launch_batch_fmt<HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA32_FLOAT>(s, mode, views, items, num_items, queue,
-                                                                     grid_blocks);
+            launch_batch_fmt<HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA32_FLOAT>(s, mode, views, spans, nspans, num_items,
+                                                                     queue, grid_blocks);
             break;
     }
     return hipGetLastError();
