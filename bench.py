@@ -40,9 +40,21 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
-FLOP_PER_PIXEL, FLOP_PER_PAIR = 60, 130
+
+
+def roofline_constants() -> dict[str, float]:
+    """The #defines of include/helio_hip/roofline.h (the one place the
+    algorithmic-work constants and the peaks are fixed)."""
+    import re
+    text = (ROOT / "include" / "helio_hip" / "roofline.h").read_text()
+    return {m.group(1): float(m.group(2)) for m in re.finditer(r"#define\s+(HL_\w+)\s+([0-9.]+)", text)}
+
+
+RL = roofline_constants()
+HBM_PEAK_GBS = RL["HL_PEAK_HBM_GBS"]
+FP32_PEAK_TFLOPS = RL["HL_PEAK_FP32_TFLOPS"]
+FP32_SCALAR_PEAK_TFLOPS = RL["HL_PEAK_FP32_SCALAR_TFLOPS"]
+FLOP_PER_PIXEL, FLOP_PER_PAIR = int(RL["HL_RL_FLOP_PER_PIXEL"]), int(RL["HL_RL_FLOP_PER_PAIR"])
 
 
 def log(msg: str) -> None:
@@ -55,7 +67,11 @@ def shade_bytes(scene, ncell: int, nlist: int, texel_sum: int) -> int:
     pixel + 4 per cell count and list entry + 56 per light."""
     sky = int((scene.depth >= 1.0).sum())
     shaded = scene.width * scene.height - sky
-    return 4 * sky + (texel_sum + 4 + 16) * shaded + 4 * (ncell + nlist) + 56 * len(scene.lights)
+    r = {k: int(v) for k, v in RL.items() if k.startswith("HL_RL_")}
+    return (r["HL_RL_SKY_PIXEL_BYTES"] * sky
+            + (texel_sum + r["HL_RL_DEPTH_BYTES"] + r["HL_RL_OUTPUT_BYTES"]) * shaded
+            + r["HL_RL_CELL_BYTES"] * ncell + r["HL_RL_LIST_ENTRY_BYTES"] * nlist
+            + r["HL_RL_LIGHT_BYTES"] * len(scene.lights))
 
 
 def time_steps(ctx, steps: int, warmup: int, *parts):
@@ -337,6 +353,7 @@ def main() -> int:
                      "bytes_per_launch": bytes_launch},
         "roofline_valu": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS,
+                          "frac_of_scalar_fp32": tflops / FP32_SCALAR_PEAK_TFLOPS,
                           "flops_per_launch": flops},
     }
     if "batched" in r4k:

@@ -149,6 +149,10 @@ struct ColumnGeom {
 struct CellBox {
     float lo[3], hi[3];
 };
+// chunk_fetch reads both records as words at these offsets
+static_assert(sizeof(float3) == 12 && sizeof(ColumnGeom) == 80 && offsetof(ColumnGeom, on) == 64,
+              "ColumnGeom layout");
+static_assert(sizeof(CellBox) == 24, "CellBox layout");
 
 // geometry scratch: ColumnGeom per column, then (256-B aligned) CellBox per cell
 __host__ __device__ inline size_t cluster_geom_offset_boxes_dev(uint32_t ncol) {
@@ -328,18 +332,25 @@ __device__ __forceinline__ MarkBox unpack_box(uint4 v) {
 
 // Candidate cells chunk*64 .. chunk*64+63 of light li (one per lane, slice
 // fastest): the exact membership rule, then the member bit and its summary
-// bit (no-return atomics).  The column, box and slice-padding loads of a lane
-// are independent and issued together.
-__device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight& L,
-                                           const MarkBox& m, uint32_t li,
-                                           const ColumnGeom* __restrict__ geom,
-                                           const CellBox* __restrict__ boxes,
-                                           uint32_t* __restrict__ bits,
-                                           uint32_t* __restrict__ summary, uint32_t W,
-                                           uint32_t SW, uint32_t chunk) {
+// bit (no-return atomics).  Split in a fetch (the lane's cell and its column,
+// box and slice-padding records, loaded together) and a test-and-mark, so
+// that a wave can have the loads of two chunks in flight at once.
+struct ChunkCell {
+    ColumnGeom g;
+    CellBox bx;
+    size_t cell;
+    float pad;
+    bool on;
+};
+
+__device__ __forceinline__ ChunkCell chunk_fetch(const ClusterArgs& a, const MarkBox& m,
+                                                 const ColumnGeom* __restrict__ geom,
+                                                 const CellBox* __restrict__ boxes, uint32_t chunk) {
+    ChunkCell r;
     const uint32_t work = m.nx * m.ny * m.nz;
     const uint32_t t = chunk * 64u + (threadIdx.x & 63u);
-    if (t >= work) return;
+    r.on = t < work;
+    if (!r.on) return r;
     uint32_t rest, iy;
     if (work <= (1u << 21)) {
         rest = hl_floor_div(t, m.nz);
@@ -351,11 +362,48 @@ __device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight
     const uint32_t kz = t - rest * m.nz, ix = rest - iy * m.nx;
     const uint32_t col = (m.x0 + ix) + a.grid_x * (m.y0 + iy);
     const uint32_t k = m.s0 + kz;
-    const size_t cell = (size_t)col + (size_t)(a.grid_x * a.grid_y) * k;
-    if (!cell_member(L, geom[col], boxes[cell], a.slice_r_pad[k])) return;
+    r.cell = (size_t)col + (size_t)(a.grid_x * a.grid_y) * k;
+    // records of a batched view come through pointers read from memory:
+    // hl_g keeps their loads global rather than flat
+    {
+        const hl_gptr<const hl_f32x4> gq = hl_g(reinterpret_cast<const hl_f32x4*>(geom + col));
+        const hl_f32x4 q0 = gq[0], q1 = gq[1], q2 = gq[2], q4 = gq[4];
+        r.g.pl[0] = make_float3(q0.x, q0.y, q0.z);
+        r.g.pl[1] = make_float3(q0.w, q1.x, q1.y);
+        r.g.pl[2] = make_float3(q1.z, q1.w, q2.x);
+        r.g.pl[3] = make_float3(q2.y, q2.z, q2.w);
+        r.g.on = __float_as_uint(q4.x);
+        const hl_gptr<const float> bq = hl_g(&boxes[r.cell].lo[0]);
+        r.bx.lo[0] = bq[0];
+        r.bx.lo[1] = bq[1];
+        r.bx.lo[2] = bq[2];
+        r.bx.hi[0] = bq[3];
+        r.bx.hi[1] = bq[4];
+        r.bx.hi[2] = bq[5];
+    }
+    r.pad = a.slice_r_pad[k];
+    return r;
+}
+
+__device__ __forceinline__ void chunk_mark(const CullLight& L, const ChunkCell& r, uint32_t li,
+                                           uint32_t* __restrict__ bits, uint32_t* __restrict__ summary,
+                                           uint32_t W, uint32_t SW) {
+    if (!r.on || !cell_member(L, r.g, r.bx, r.pad)) return;
     const uint32_t word = li >> 5;
-    atomicOr(&bits[cell * W + word], 1u << (li & 31u));
-    atomicOr(&summary[cell * SW + (word >> 5)], 1u << (word & 31u));
+    __hip_atomic_fetch_or(hl_g(&bits[r.cell * W + word]), 1u << (li & 31u), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(hl_g(&summary[r.cell * SW + (word >> 5)]), 1u << (word & 31u), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight& L,
+                                           const MarkBox& m, uint32_t li,
+                                           const ColumnGeom* __restrict__ geom,
+                                           const CellBox* __restrict__ boxes,
+                                           uint32_t* __restrict__ bits,
+                                           uint32_t* __restrict__ summary, uint32_t W,
+                                           uint32_t SW, uint32_t chunk) {
+    chunk_mark(L, chunk_fetch(a, m, geom, boxes, chunk), li, bits, summary, W, SW);
 }
 
 __device__ __forceinline__ CullLight load_light_uniform(const CullLight* __restrict__ cl, uint32_t li) {
@@ -390,6 +438,12 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
     MarkBox m;
     if (!light_cells(a, cam, L, sp, m)) return;
     const uint32_t nchunk = (m.nx * m.ny * m.nz + 63u) / 64u;
+    // chunks 0 and 1 always stay with this wave: their loads go out first,
+    // both together, ahead of the queueing of the rest
+    static_assert(kMarkInline == 2, "two head chunks fetched together");
+    const ChunkCell r0 = chunk_fetch(a, m, geom, boxes, 0u);
+    ChunkCell r1;
+    if (nchunk > 1u) r1 = chunk_fetch(a, m, geom, boxes, 1u);
     uint32_t qlo = nchunk, qhi = nchunk;  // chunks [qlo, qhi) are queued
     if (nchunk > kMarkInline) {
         const uint32_t extra = nchunk - kMarkInline;
@@ -407,10 +461,11 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
             qhi = kMarkInline + fit;
         }
     }
-    for (uint32_t c = 0; c < nchunk; ++c) {
-        if (c >= qlo && c < qhi) continue;
-        mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, c);
-    }
+    // chunks [0, qlo) and [qhi, nchunk) stay here (qlo >= 2 whenever nchunk >= 2)
+    chunk_mark(L, r0, li, bits, summary, W, SW);
+    if (nchunk > 1u) chunk_mark(L, r1, li, bits, summary, W, SW);
+    for (uint32_t c = 2u; c < qlo; ++c) mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, c);
+    for (uint32_t c = qhi; c < nchunk; ++c) mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, c);
 }
 
 // Count and fill work on 16 lanes per cluster (four clusters per wavefront).
