@@ -17,6 +17,8 @@ Further legs, each reported in its own object of the same line:
                   shading launch), eager and replayed from its HIP graph
   fwd_adjoint_512 512x512 fp32 G-buffer, 256 point lights: cluster cull +
                   fused forward / L2 loss / adjoint per step (and forward only)
+  gbuffer_pack_4k the G-buffer producer's format contract (hl_gbuffer_pack) at
+                  3840x2160 into the reference formats
   multiview       (opt-in, --multiview) 48 probe faces (16 each at 128^2,
                   256^2, 512^2) + 2 XR eyes at 2064x2208 over an 8192-light
                   pool, adjoint on the eyes: one batched frame (autotuned, graph
@@ -175,6 +177,32 @@ def bench_4k(ctx, args):
     return res
 
 
+def bench_gbuffer_pack(ctx, args):
+    """hl_gbuffer_pack at 3840x2160 into the reference formats (RGBA8 albedo/orm,
+    RGBA16F normal/emissive), metallic workflow: 64 B read + 24 B written per pixel."""
+    from helio_amd.device import pack_gbuffer, surface_desc
+    from helio_amd import abi
+    W, H = 3840, 2160
+    rng = np.random.default_rng(5)
+    plane = rng.uniform(0.0, 1.0, size=(H, W, 4)).astype(np.float32)
+    planes = [ctx.upload(plane) for _ in range(4)]
+    outs = [ctx.alloc(W * H * tb) for tb in (4, 8, 4, 8)]
+    g = abi.hl_gbuffer()
+    g.width, g.height = W, H
+    for t, o, fmt, tb in zip((g.albedo, g.normal, g.orm, g.emissive), outs,
+                             (abi.HL_FMT_RGBA8_UNORM, abi.HL_FMT_RGBA16_FLOAT, abi.HL_FMT_RGBA8_UNORM,
+                              abi.HL_FMT_RGBA16_FLOAT), (4, 8, 4, 8)):
+        t.data_d, t.pitch_bytes, t.format = o.ptr, W * tb, fmt
+    s = surface_desc(W, H, *planes)
+    ms, _ = time_steps(ctx, args.steps, args.warmup, lambda: pack_gbuffer(ctx, s, g))
+    nbytes = W * H * (4 * 16 + 4 + 8 + 4 + 8)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"G-buffer pack 4K: {ms * 1e3:.1f} us, {gbs:.0f} GB/s")
+    return {"ms_per_pack": ms, "bytes_per_pack": nbytes, "achieved_GBs": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
+            "workload": "3840x2160 RGBA32F surface planes -> RGBA8 albedo/orm + RGBA16F normal/emissive "
+                        "(metallic-workflow F0), hl_gbuffer_pack"}
+
+
 def bench_adjoint(ctx, args):
     from helio_amd import synth
     from helio_amd.device import AdjointBuffers, DeferredLightPass, LightCullPass, upload_scene
@@ -275,6 +303,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the one-view batched 4K leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 512x512 forward+adjoint leg")
+    ap.add_argument("--no-pack", action="store_true", help="skip the 4K G-buffer pack leg")
     ap.add_argument("--multiview", action="store_true",
                     help="also run the full-size 50-view batched leg (opt-in: its autotuned plan hit a GPU "
                          "memory fault once, cause not yet found)")
@@ -306,6 +335,7 @@ def main() -> int:
     with Context(0) as ctx:
         r4k = bench_4k(ctx, args)
         adj = None if args.no_adjoint else bench_adjoint(ctx, args)
+        pack = None if args.no_pack else bench_gbuffer_pack(ctx, args)
         mv = bench_multiview(ctx, args) if args.multiview else None
 
     sc, q = r4k["scene"], r4k["q"]
@@ -360,6 +390,8 @@ def main() -> int:
         res["batched_4k"] = r4k["batched"]
     if adj is not None:
         res["fwd_adjoint_512"] = adj
+    if pack is not None:
+        res["gbuffer_pack_4k"] = pack
     res["multiview"] = mv if mv is not None else {
         "skipped": "full-size 50-view leg is opt-in (--multiview): its autotuned plan faulted the GPU once "
                    "and stays off the GPU until the cause is found; the 1/8-scale 50-view plan is "

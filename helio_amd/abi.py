@@ -90,6 +90,12 @@ class hl_cluster_params(C.Structure):
                 ("slice_r_pad", C.c_float * HL_MAX_SLICES)]
 
 
+class hl_surface(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("pitch_bytes", C.c_uint32),
+                ("reserved", C.c_uint32), ("albedo_d", C.c_void_p), ("normal_d", C.c_void_p),
+                ("orm_d", C.c_void_p), ("emissive_d", C.c_void_p), ("specular_f0_d", C.c_void_p)]
+
+
 class hl_light_grid(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("num_tiles_x", C.c_uint32), ("num_tiles_y", C.c_uint32),
                 ("lists_d", C.c_void_p), ("counts_d", C.c_void_p), ("offsets_d", C.c_void_p),
@@ -163,6 +169,7 @@ SIGNATURES: dict[str, tuple] = {
     "hl_light_cull_tiles": (None, [vp, vp, P(LightCullParams), vp, vp, vp]),
     "hl_light_cull_clusters": (None, [vp, vp, P(hl_cluster_params), vp, u32, vp, vp, vp, u32, vp]),
     "hl_query_overflow": (None, [vp, P(u32)]),
+    "hl_gbuffer_pack": (None, [vp, P(hl_surface), P(hl_gbuffer)]),
     "hl_deferred_forward": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                    P(hl_light_grid), vp, u32, u32]),
     "hl_count_pixel_light_pairs": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,

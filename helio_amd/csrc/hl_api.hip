@@ -82,6 +82,8 @@ hl_status check_tex(hl_context* c, const hl_texture& t, uint32_t w, const char* 
     return HL_OK;
 }
 
+hl_status color_set(hl_context* c, const hl_gbuffer* g, int* set);
+
 hl_status check_gbuffer(hl_context* c, const hl_gbuffer* g, int* set) {
     if (!g) return fail(c, HL_EINVAL, "gbuffer is null");
     if (g->width == 0 || g->height == 0) return fail(c, HL_EINVAL, "gbuffer has zero size");
@@ -98,6 +100,11 @@ hl_status check_gbuffer(hl_context* c, const hl_gbuffer* g, int* set) {
     if (g->lightmap_uv.data_d)
         return fail(c, HL_EUNSUPPORTED,
                     "gbuffer.lightmap_uv must be NULL: baked lightmaps are outside the hot path");
+    return color_set(c, g, set);
+}
+
+// The format set of the four colour targets.
+hl_status color_set(hl_context* c, const hl_gbuffer* g, int* set) {
     const uint32_t fa = g->albedo.format, fo = g->orm.format, fn = g->normal.format,
                    fe = g->emissive.format;
     if (fa != fo || fn != fe)
@@ -554,6 +561,39 @@ HL_API hl_status hl_query_overflow(hl_context* c, uint32_t* flags) {
     if (!c || !flags) return HL_EINVAL;
     HL_HIP(c, hipMemcpyAsync(flags, ctx_words(c), 4, hipMemcpyDeviceToHost, c->stream));
     HL_HIP(c, hipStreamSynchronize(c->stream));
+    return HL_OK;
+}
+
+HL_API hl_status hl_gbuffer_pack(hl_context* c, const hl_surface* sf, const hl_gbuffer* g) {
+    if (!c) return HL_EINVAL;
+    if (!sf || !g) return fail(c, HL_EINVAL, "surface or gbuffer is null");
+    if (sf->width != g->width || sf->height != g->height)
+        return fail(c, HL_EINVAL, "surface %ux%u differs from gbuffer %ux%u", sf->width, sf->height, g->width,
+                    g->height);
+    if (g->width > 32768 || g->height > 32768)
+        return fail(c, HL_EINVAL, "gbuffer %ux%u exceeds 32768", g->width, g->height);
+    if (sf->pitch_bytes % 16 || (uint64_t)sf->pitch_bytes < 16ull * sf->width)
+        return fail(c, HL_EINVAL, "surface pitch %u invalid for width %u (RGBA32F planes)", sf->pitch_bytes,
+                    sf->width);
+    const float* planes[5] = {sf->albedo_d, sf->normal_d, sf->orm_d, sf->emissive_d, sf->specular_f0_d};
+    for (int i = 0; i < 5; ++i) {
+        if (!planes[i] && i < 4) return fail(c, HL_EINVAL, "surface plane %d is null", i);
+        if (((uintptr_t)planes[i]) % 16) return fail(c, HL_EINVAL, "surface plane %d not 16-B aligned", i);
+    }
+    hl_status s;
+    if ((s = check_tex(c, g->albedo, g->width, "albedo")) != HL_OK) return s;
+    if ((s = check_tex(c, g->normal, g->width, "normal")) != HL_OK) return s;
+    if ((s = check_tex(c, g->orm, g->width, "orm")) != HL_OK) return s;
+    if ((s = check_tex(c, g->emissive, g->width, "emissive")) != HL_OK) return s;
+    int set = 0;
+    if ((s = color_set(c, g, &set)) != HL_OK) return s;
+    if (g->lightmap_uv.data_d) {
+        if (g->lightmap_uv.format != HL_FMT_RG16_FLOAT)
+            return fail(c, HL_EINVAL, "gbuffer.lightmap_uv must be RG16_FLOAT");
+        if ((s = check_tex(c, g->lightmap_uv, g->width, "lightmap_uv")) != HL_OK) return s;
+    }
+    HL_HIP(c, hipSetDevice(c->device));
+    HL_HIP(c, launch_gbuffer_pack(c->stream, set, *sf, *g));
     return HL_OK;
 }
 

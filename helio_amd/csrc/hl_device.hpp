@@ -245,6 +245,20 @@ __device__ __forceinline__ float hl_half(uint16_t h) {
     return (float)v;
 }
 
+// Encoders of the G-buffer formats (the inverse direction of the decoders
+// below): unorm8 = round-to-nearest-even of clamp(v, 0, 1) * 255, NaN -> 0;
+// fp16 = IEEE round-to-nearest-even (v_cvt_f16_f32 in the default mode).
+__device__ __forceinline__ uint32_t hl_enc_unorm8(float v) {
+    const float c = fminf(fmaxf(v, 0.0f), 1.0f);  // maxNum: NaN -> 0
+    return (uint32_t)__builtin_rintf(c * 255.0f);
+}
+__device__ __forceinline__ uint32_t hl_enc_half(float v) {
+    const _Float16 h = (_Float16)v;
+    uint16_t u;
+    __builtin_memcpy(&u, &h, 2);
+    return u;
+}
+
 // v / 255 exactly: v * (1/255) with the reciprocal split into its float part
 // and the float of the remainder, fma(v, hi, v * lo); equal to the IEEE
 // quotient for every v in 0..255 (checked in exact rational arithmetic).

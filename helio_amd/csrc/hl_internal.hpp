@@ -185,6 +185,9 @@ inline void set_tile_shape(ShadeArgs& a, uint32_t tw_log2) {
     a.block_tiles_x = (a.width + bw - 1) / bw;
     a.num_blocks = a.block_tiles_x * ((a.height + bh - 1) / bh);
 }
+// Packs surface attributes into the colour targets of gb (format set gbuf_set).
+hipError_t launch_gbuffer_pack(hipStream_t s, int gbuf_set, const hl_surface& sf, const hl_gbuffer& gb);
+
 // spans: nspans (first item, view) pairs in item order, then (num_items, 0).
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
                               const uint2* spans, uint32_t nspans, uint32_t num_items, uint32_t* queue,

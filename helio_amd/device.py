@@ -144,6 +144,25 @@ def upload_scene(ctx: Context, scene) -> DeviceScene:
     return DeviceScene(scene, cam, lights, planes, g)
 
 
+def surface_desc(width: int, height: int, albedo, normal, orm, emissive, specular_f0=None) -> abi.hl_surface:
+    """hl_surface over RGBA32F planes of width x height (DeviceArray or host numpy arrays)."""
+    def ptr(p):
+        if p is None:
+            return None
+        return p.ptr if isinstance(p, DeviceArray) else p.ctypes.data
+    s = abi.hl_surface()
+    s.width, s.height, s.pitch_bytes = width, height, 16 * width
+    s.albedo_d, s.normal_d, s.orm_d, s.emissive_d = ptr(albedo), ptr(normal), ptr(orm), ptr(emissive)
+    s.specular_f0_d = ptr(specular_f0)
+    return s
+
+
+def pack_gbuffer(ctx: Context, surface: abi.hl_surface, gbuffer: abi.hl_gbuffer) -> None:
+    """The G-buffer pass's target writes (hl_gbuffer_pack): surface attributes
+    into the G-buffer textures, F0 resolved into their alpha channels."""
+    ctx.lib.call("hl_gbuffer_pack", ctx.h, C.byref(surface), C.byref(gbuffer), ctx=ctx.h)
+
+
 class TileGrid:
     """tile_light_lists / tile_light_counts (crates/libhelio/src/frame.rs:230-234)."""
 

@@ -210,6 +210,37 @@ HL_API hl_status hl_light_cull_clusters(hl_context* ctx, const GpuCameraUniforms
  * this context (synchronizes the stream).  0 = no truncation. */
 HL_API hl_status hl_query_overflow(hl_context* ctx, uint32_t* flags);
 
+/* --------------------------------------------------------- G-buffer pack */
+/* Per-pixel surface attributes as the G-buffer pass's fragment shader has them
+ * before it writes its targets (crates/passes/3d/helio-pass-gbuffer/shaders/
+ * gbuffer.wgsl:480-493): RGBA32F planes of width x height, all with the same
+ * row pitch. */
+typedef struct hl_surface {
+    uint32_t width, height;
+    uint32_t pitch_bytes;            /* >= 16 * width, multiple of 16 */
+    uint32_t reserved;               /* 0 */
+    const float* albedo_d;           /* rgb albedo, a alpha                            */
+    const float* normal_d;           /* xyz world normal (written as given), w ignored */
+    const float* orm_d;              /* ao, roughness, metallic, w ignored              */
+    const float* emissive_d;         /* rgb emissive, w ignored                         */
+    const float* specular_f0_d;      /* rgb F0 of the specular workflow, w ignored;
+                                        NULL = metallic workflow                        */
+} hl_surface;
+
+/* Writes the G-buffer the deferred pass reads, in the formats of `gbuffer`
+ * (one of the format sets above; gbuffer's width/height must equal the
+ * surface's; depth is not written):
+ *   albedo   = (albedo.rgb, alpha)
+ *   normal   = (N, F0.r)
+ *   orm      = (ao, roughness, metallic, F0.g)
+ *   emissive = (emissive.rgb, F0.b)             (gbuffer.wgsl:10-15, 488-491)
+ * with F0 = clamp(mix(0.04, albedo, metallic), 0, 0.999) in the metallic
+ * workflow (gbuffer.wgsl:324-328), or the given specular F0.  When
+ * gbuffer->lightmap_uv.data_d is set (RG16_FLOAT) it is filled with the
+ * (-1, -1) "no lightmap" sentinel.  Conversions: unorm8 = round-to-nearest-even
+ * of clamp(v, 0, 1) * 255 (NaN -> 0); fp16 = IEEE round-to-nearest-even. */
+HL_API hl_status hl_gbuffer_pack(hl_context* ctx, const hl_surface* surface, const hl_gbuffer* gbuffer);
+
 /* ---------------------------------------------------------------- shading */
 enum {
     HL_SHADE_CLEAR_SKY = 1u,        /* write (0,0,0,0) to sky pixels (LoadOp::Clear); default leaves them */

@@ -15,6 +15,7 @@
  *    membership rule is defined in helio_hip.h and restated here.
  *  - FP64 differentiable forward, analytic adjoint and central finite differences
  *    (or64_*), all lights evaluated per pixel (no grid).
+ *  - The G-buffer producer's format contract (or_gbuffer_pack, gbuffer_pack.c).
  *
  * WGSL leaves a few operation orders to the backend; this oracle fixes them:
  *    dot(a,b)      = (a.x*b.x + a.y*b.y) + a.z*b.z
@@ -137,6 +138,13 @@ void or64_backward(const or64_pixel* px, uint32_t width, uint32_t height,
 double or64_fd(const or64_pixel* px, uint32_t width, uint32_t height, const DeferredGlobals* g,
                const GpuLight* lights, uint32_t num_lights, const float* target,
                uint32_t target_pitch, int kind, uint32_t index, double h, int* crosses);
+
+/* G-buffer producer contract (gbuffer_pack.c): hl_gbuffer_pack on host
+ * memory, and its pieces. */
+void or_gbuffer_pack(const hl_surface* surface, const hl_gbuffer* gbuffer);
+uint16_t or_float_to_half(float f);
+uint32_t or_unorm8_encode(float v);
+float or_f0_metallic(float albedo, float metallic);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,10 @@ _SIGS = {
     "or_light_cull_clusters": (u32, [P(abi.GpuCameraUniforms), P(abi.hl_cluster_params), vp, u32, vp, vp, vp, u32]),
     "or_cluster_hits": (u32, [P(abi.GpuCameraUniforms), P(abi.hl_cluster_params), vp, u32, u32]),
     "or_slice_of_depth": (u32, [P(abi.hl_cluster_params), C.c_float]),
+    "or_gbuffer_pack": (None, [P(abi.hl_surface), P(abi.hl_gbuffer)]),
+    "or_float_to_half": (C.c_uint16, [C.c_float]),
+    "or_unorm8_encode": (u32, [C.c_float]),
+    "or_f0_metallic": (C.c_float, [C.c_float, C.c_float]),
     "or_shade": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
                               P(abi.hl_light_grid), vp, u32, u32, u32, u32]),
     "or_count_pairs": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
