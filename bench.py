@@ -19,6 +19,11 @@ Further legs, each reported in its own object of the same line:
                   fused forward / L2 loss / adjoint per step (and forward only)
   gbuffer_pack_4k the G-buffer producer's format contract (hl_gbuffer_pack) at
                   3840x2160 into the reference formats
+  forward_lit_4k  forward-lit fragment shading (hl_forward_lit) of the 4K
+                  frame's 8.3 M surfaces as fragments (8x8-pixel blocks in
+                  raster order) under the same 16384 lights read through
+                  entity indices and transforms, on the reference's 16-px tile
+                  lists (cap 64) and on the 32x18x48 cluster lists
   multiview       (opt-in, --multiview) 48 probe faces (16 each at 128^2,
                   256^2, 512^2) + 2 XR eyes at 2064x2208 over an 8192-light
                   pool, adjoint on the eyes: one batched frame (autotuned, graph
@@ -203,6 +208,42 @@ def bench_gbuffer_pack(ctx, args):
                         "(metallic-workflow F0), hl_gbuffer_pack"}
 
 
+def bench_forward_lit(ctx, args):
+    """hl_forward_lit over every surface of the 4K/16k scene as one fragment
+    (80-byte records), lights positioned through entity indices and
+    transforms, on the tile lists and on the cluster lists."""
+    from helio_amd import synth
+    from helio_amd.device import ForwardLitPass, LightCullPass, upload_scene
+    sc = synth.make_scene(args.workload)
+    sc.lights, entity, transforms = synth.make_light_transforms(sc.lights, seed=4)
+    ds = upload_scene(ctx, sc)
+    frags = synth.make_fragments(sc, "tiles")
+    n = len(frags)
+    d_fr, d_ent, d_tr = ctx.upload(frags), ctx.upload(entity), ctx.upload(transforms)
+    fwd = ForwardLitPass(ctx, sc.width, sc.height)
+    fwd.prepare(ds)
+    res = {"workload": f"{n} fragments (every surface of the {sc.width}x{sc.height} frame, 8x8-pixel blocks "
+                       f"in raster order, 80 B each) under {len(sc.lights)} lights read through entity "
+                       f"indices and transforms; RGBA32F per fragment",
+           "fragments": n}
+    params = synth.cluster_params(ctx.lib, sc)
+    for mode, cull in (("tiles", LightCullPass(ctx, sc.width, sc.height)),
+                       ("clusters", LightCullPass(ctx, sc.width, sc.height, clusters=params, pool_cap=1 << 24))):
+        cull.prepare(ds)
+        cull.execute(ds)
+        grid = cull.publish()
+        kw = dict(entity=d_ent, transforms=d_tr, num_transforms=len(transforms))
+        pairs = fwd.count_pairs(ds, d_fr, n, grid, **kw)
+        ms, _ = time_steps(ctx, args.steps, args.warmup, lambda: fwd.execute(ds, d_fr, n, grid, **kw))
+        res[mode] = {"ms_per_call": ms, "pairs": pairs, "evals_per_s": pairs / (ms * 1e-3),
+                     "fragments_per_s": n / (ms * 1e-3),
+                     "bytes_fragment_io": 96 * n, "fragment_io_GBs": 96 * n / (ms * 1e-3) / 1e9}
+        log(f"forward-lit 4K {mode}: {ms * 1e3:.1f} us, {pairs / (ms * 1e-3) / 1e9:.1f} G evals/s")
+    res["tiles"]["grid"] = "reference tile_light_lists (16 px, cap 64, ascending)"
+    res["clusters"]["grid"] = f"{params.grid_x}x{params.grid_y}x{params.grid_z} log-depth clusters, uncapped"
+    return res
+
+
 def bench_adjoint(ctx, args):
     from helio_amd import synth
     from helio_amd.device import AdjointBuffers, DeferredLightPass, LightCullPass, upload_scene
@@ -304,6 +345,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the one-view batched 4K leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 512x512 forward+adjoint leg")
     ap.add_argument("--no-pack", action="store_true", help="skip the 4K G-buffer pack leg")
+    ap.add_argument("--no-forward", action="store_true", help="skip the 4K forward-lit leg")
     ap.add_argument("--multiview", action="store_true",
                     help="also run the full-size 50-view batched leg (opt-in: its autotuned plan hit a GPU "
                          "memory fault once, cause not yet found)")
@@ -336,6 +378,7 @@ def main() -> int:
         r4k = bench_4k(ctx, args)
         adj = None if args.no_adjoint else bench_adjoint(ctx, args)
         pack = None if args.no_pack else bench_gbuffer_pack(ctx, args)
+        fwd = None if args.no_forward else bench_forward_lit(ctx, args)
         mv = bench_multiview(ctx, args) if args.multiview else None
 
     sc, q = r4k["scene"], r4k["q"]
@@ -392,6 +435,8 @@ def main() -> int:
         res["fwd_adjoint_512"] = adj
     if pack is not None:
         res["gbuffer_pack_4k"] = pack
+    if fwd is not None:
+        res["forward_lit_4k"] = fwd
     res["multiview"] = mv if mv is not None else {
         "skipped": "full-size 50-view leg is opt-in (--multiview): its autotuned plan faulted the GPU once "
                    "and stays off the GPU until the cause is found; the 1/8-scale 50-view plan is "

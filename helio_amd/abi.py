@@ -70,6 +70,31 @@ class GpuMaterial(C.Structure):
                 ("flags", C.c_uint32), ("material_class", C.c_uint32), ("class_params", C.c_float * 4)]
 
 
+class ForwardLitGlobals(C.Structure):
+    _fields_ = [("frame", C.c_uint32), ("delta_time", C.c_float), ("light_count", C.c_uint32),
+                ("ambient_intensity", C.c_float), ("ambient_color", C.c_float * 4),
+                ("num_tiles_x", C.c_uint32), ("num_tiles_y", C.c_uint32), ("screen_width", C.c_float),
+                ("screen_height", C.c_float)]
+
+
+class hl_transform(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("rotation", C.c_float * 3), ("scale", C.c_float * 3)]
+
+
+class hl_fragment(C.Structure):
+    _fields_ = [("world_pos", C.c_float * 3), ("pixel", C.c_uint32), ("normal", C.c_float * 3),
+                ("alpha", C.c_float), ("albedo", C.c_float * 3), ("ao", C.c_float),
+                ("specular_f0", C.c_float * 3), ("roughness", C.c_float), ("emissive", C.c_float * 3),
+                ("metallic", C.c_float)]
+
+
+# numpy views of the same records (fragments and transforms are built as arrays)
+FRAGMENT_DTYPE = np.dtype([("world_pos", "<f4", 3), ("pixel", "<u4"), ("normal", "<f4", 3), ("alpha", "<f4"),
+                           ("albedo", "<f4", 3), ("ao", "<f4"), ("specular_f0", "<f4", 3),
+                           ("roughness", "<f4"), ("emissive", "<f4", 3), ("metallic", "<f4")])
+TRANSFORM_DTYPE = np.dtype([("position", "<f4", 3), ("rotation", "<f4", 3), ("scale", "<f4", 3)])
+
+
 class hl_texture(C.Structure):
     _fields_ = [("data_d", C.c_void_p), ("pitch_bytes", C.c_uint32), ("format", C.c_uint32)]
 
@@ -127,6 +152,9 @@ def check_layouts() -> None:
     assert C.sizeof(LightCullParams) == 32
     assert C.sizeof(DeferredGlobals) == 112
     assert C.sizeof(GpuMaterial) == 96
+    assert C.sizeof(ForwardLitGlobals) == 48
+    assert C.sizeof(hl_transform) == 36 == TRANSFORM_DTYPE.itemsize
+    assert C.sizeof(hl_fragment) == 80 == FRAGMENT_DTYPE.itemsize
 
 
 check_layouts()
@@ -173,6 +201,10 @@ SIGNATURES: dict[str, tuple] = {
     "hl_deferred_forward": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                    P(hl_light_grid), vp, u32, u32]),
     "hl_count_pixel_light_pairs": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
+                                          P(hl_light_grid), P(C.c_uint64)]),
+    "hl_forward_lit": (None, [vp, vp, P(ForwardLitGlobals), vp, u32, vp, u32, vp, vp, u32,
+                              P(hl_light_grid), vp, u32]),
+    "hl_forward_lit_count_pairs": (None, [vp, vp, P(ForwardLitGlobals), vp, u32, vp, u32, vp, vp, u32,
                                           P(hl_light_grid), P(C.c_uint64)]),
     "hl_deferred_backward": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                     P(hl_light_grid), vp, u32, vp, vp, vp]),

@@ -21,7 +21,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 ORACLE = ROOT / "oracle"
 
-HIP_SOURCES = ["hl_api.hip", "hl_cull.hip", "hl_shade.hip", "hl_batch.hip", "hl_gbuffer.hip"]
+HIP_SOURCES = ["hl_api.hip", "hl_cull.hip", "hl_shade.hip", "hl_batch.hip", "hl_gbuffer.hip",
+               "hl_forward.hip"]
 ARCH = os.environ.get("HELIO_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -91,7 +92,8 @@ def build_synth() -> Path:
 def build_oracle() -> Path:
     out = ORACLE / "liboracle.so"
     _run(["gcc", *CC_FLAGS, f"-I{ROOT / 'include'}", str(ORACLE / "oracle.c"),
-          str(ORACLE / "oracle64.c"), str(ORACLE / "gbuffer_pack.c"), "-o", str(out), "-lm"])
+          str(ORACLE / "oracle64.c"), str(ORACLE / "gbuffer_pack.c"),
+          str(ORACLE / "forward_lit.c"), "-o", str(out), "-lm"])
     return out
 
 

@@ -132,6 +132,10 @@ hl_status check_cluster_params(hl_context* c, const hl_cluster_params* p) {
     if ((uint64_t)p->grid_x * p->grid_y * p->grid_z > (1ull << 26) ||
         (uint64_t)p->grid_x * p->grid_y > 65535u)
         return fail(c, HL_EINVAL, "cluster grid too large (at most 65535 columns, 2^26 cells)");
+    // the shading kernels search the slice thresholds as a sorted table
+    for (uint32_t k = 1; k < p->grid_z; ++k)
+        if (!(p->depth_split[k] >= p->depth_split[k - 1]))
+            return fail(c, HL_EINVAL, "cluster depth_split is not non-decreasing at slice %u", k);
     return HL_OK;
 }
 
@@ -713,6 +717,124 @@ HL_API hl_status hl_deferred_forward_backward_l2(
     return run_adjoint(c, camera_d, g, gb, lights_d, num_lights, grid, target_d, target_pitch_bytes,
                        out_d, out_pitch_bytes, loss_d, d_pixel_d, d_light_pos_d, d_light_intensity_d,
                        true);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- forward-lit
+namespace {
+
+// Arguments of a forward-lit launch, and the light records it reads (built on
+// the context stream from the lights and, when given, their transforms).
+hl_status prepare_forward(hl_context* c, const GpuCameraUniforms* camera_d, const ForwardLitGlobals* g,
+                          const hl_fragment* frags_d, uint32_t n, const GpuLight* lights_d,
+                          uint32_t num_lights, const uint32_t* entity_d, const hl_transform* transforms_d,
+                          uint32_t num_transforms, const hl_light_grid* grid, ForwardArgs* a) {
+    if (!c) return HL_EINVAL;
+    if (!camera_d) return fail(c, HL_EINVAL, "camera is null");
+    if (!g) return fail(c, HL_EINVAL, "globals are null");
+    if (!grid) return fail(c, HL_EINVAL, "light grid is null");
+    if (n && !frags_d) return fail(c, HL_EINVAL, "fragments are null");
+    if (((uintptr_t)frags_d) % 16) return fail(c, HL_EINVAL, "fragments must be 16-byte aligned");
+    if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
+    if (!entity_d != !transforms_d)
+        return fail(c, HL_EINVAL, "light_entity_indices and transforms must both be given or both be null");
+    memset(a, 0, sizeof *a);
+    hl_status s;
+    if (grid->mode == HL_GRID_TILES) {
+        if (!grid->lists_d || !grid->counts_d) return fail(c, HL_EINVAL, "tile lists/counts are null");
+        if (g->num_tiles_x != grid->num_tiles_x)
+            return fail(c, HL_EINVAL, "globals.num_tiles_x (%u) != grid.num_tiles_x (%u)", g->num_tiles_x,
+                        grid->num_tiles_x);
+        a->grid_mode = HL_GRID_TILES;
+        a->grid_x = grid->num_tiles_x;
+        a->grid_y = grid->num_tiles_y;
+        a->grid_z = 1;
+        a->cell_w = a->cell_h = HL_TILE_SIZE;
+        a->tile_stride_x = g->num_tiles_x;
+        uint64_t cap = (uint64_t)grid->num_tiles_x * grid->num_tiles_y * HL_MAX_LIGHTS_PER_TILE;
+        if (grid->lists_capacity && grid->lists_capacity < cap) cap = grid->lists_capacity;
+        a->lists_capacity = cap > 0xffffffffull ? 0xffffffffu : (uint32_t)cap;
+    } else if (grid->mode == HL_GRID_CLUSTERS) {
+        const hl_cluster_params* p = grid->clusters;
+        if ((s = check_cluster_params(c, p)) != HL_OK) return s;
+        if (!grid->lists_d || !grid->counts_d || !grid->offsets_d)
+            return fail(c, HL_EINVAL, "cluster offsets/counts/pool are null");
+        a->grid_mode = HL_GRID_CLUSTERS;
+        a->grid_x = p->grid_x;
+        a->grid_y = p->grid_y;
+        a->grid_z = p->grid_z;
+        a->cell_w = p->cell_w;
+        a->cell_h = p->cell_h;
+        a->lists_capacity = grid->lists_capacity;
+        memcpy(a->depth_split, p->depth_split, sizeof a->depth_split);
+    } else {
+        return fail(c, HL_EINVAL, "unknown grid mode %u", grid->mode);
+    }
+    a->lists = grid->lists_d;
+    a->counts = grid->counts_d;
+    a->offsets = grid->offsets_d;
+    a->cam = camera_d;
+    a->frags = frags_d;
+    a->num_frags = n;
+    a->num_lights = num_lights;
+    for (int k = 0; k < 3; ++k) {
+#pragma clang fp contract(off)
+        a->amb[k] = g->ambient_color[k] * g->ambient_intensity;
+    }
+    if ((s = ensure(c, c->words, 64)) != HL_OK) return s;
+    HL_HIP(c, hipSetDevice(c->device));
+    c->hot_src = nullptr;  // the records no longer mirror a light cull's
+    if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(num_lights ? num_lights : 1))) != HL_OK) return s;
+    a->lights = (const HotLight*)c->hot.p;
+    HL_HIP(c, launch_forward_prep(c->stream, lights_d, num_lights, entity_d, transforms_d, num_transforms,
+                                  (HotLight*)c->hot.p));
+    return HL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+HL_API hl_status hl_forward_lit(hl_context* c, const GpuCameraUniforms* camera_d, const ForwardLitGlobals* g,
+                                const hl_fragment* fragments_d, uint32_t num_fragments, const GpuLight* lights_d,
+                                uint32_t num_lights, const uint32_t* light_entity_indices_d,
+                                const hl_transform* transforms_d, uint32_t num_transforms,
+                                const hl_light_grid* grid, float* out_d, uint32_t flags) {
+    if (!c) return HL_EINVAL;
+    if (flags) return fail(c, HL_EINVAL, "unknown forward-lit flags 0x%x", flags);
+    if (num_fragments && !out_d) return fail(c, HL_EINVAL, "output is null");
+    if (((uintptr_t)out_d) % 16) return fail(c, HL_EINVAL, "output must be 16-byte aligned");
+    ForwardArgs a;
+    hl_status s = prepare_forward(c, camera_d, g, fragments_d, num_fragments, lights_d, num_lights,
+                                  light_entity_indices_d, transforms_d, num_transforms, grid, &a);
+    if (s != HL_OK) return s;
+    a.out = out_d;
+    HL_HIP(c, launch_forward_lit(c->stream, a));
+    return HL_OK;
+}
+
+HL_API hl_status hl_forward_lit_count_pairs(hl_context* c, const GpuCameraUniforms* camera_d,
+                                            const ForwardLitGlobals* g, const hl_fragment* fragments_d,
+                                            uint32_t num_fragments, const GpuLight* lights_d, uint32_t num_lights,
+                                            const uint32_t* light_entity_indices_d,
+                                            const hl_transform* transforms_d, uint32_t num_transforms,
+                                            const hl_light_grid* grid, uint64_t* pairs) {
+    if (!c) return HL_EINVAL;
+    if (!pairs) return fail(c, HL_EINVAL, "pairs is null");
+    ForwardArgs a;
+    hl_status s = prepare_forward(c, camera_d, g, fragments_d, num_fragments, lights_d, num_lights,
+                                  light_entity_indices_d, transforms_d, num_transforms, grid, &a);
+    if (s != HL_OK) return s;
+    unsigned long long* ctr = (unsigned long long*)(ctx_words(c) + 2);
+    HL_HIP(c, launch_fill_u32(c->stream, (uint32_t*)ctr, 0, 2));
+    a.pair_counter = ctr;
+    HL_HIP(c, launch_forward_lit(c->stream, a));
+    unsigned long long v = 0;
+    HL_HIP(c, hipMemcpyAsync(&v, ctr, 8, hipMemcpyDeviceToHost, c->stream));
+    HL_HIP(c, hipStreamSynchronize(c->stream));
+    *pairs = v;
+    return HL_OK;
 }
 
 }  // extern "C"

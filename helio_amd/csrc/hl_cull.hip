@@ -39,24 +39,7 @@ __device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
     const float inner = l->inner_angle;
     const float inf = __builtin_inff();
     const float range = type == HL_LIGHT_DIRECTIONAL ? inf : pr.w;
-    if (hot) {
-        HotLight h;
-        h.p0 = make_float4(pr.x, pr.y, pr.z, range);
-        {
-#pragma clang fp contract(off)
-            h.p1 = make_float4(ci.x * ci.w, ci.y * ci.w, ci.z * ci.w, inner);
-        }
-        h.p2 = make_float4(dro.x, dro.y, dro.z, type == HL_LIGHT_SPOT ? dro.w : kNoCone);
-        const float margin = 1e-5f * (fabsf(dro.x) + fabsf(dro.y) + fabsf(dro.z));
-        h.p3 = make_float4(1.0f / (range * range), ci.w, margin, 0.0f);
-        {
-#pragma clang fp contract(off)
-            const float r2 = range * range;
-            const float cmax = fmaxf(fmaxf(h.p1.x, h.p1.y), h.p1.z);
-            h.p4 = make_float4(r2 * (1.0f - 1e-5f), r2 * (1.0f + 1e-5f), cmax, 3.61e-5f * cmax);
-        }
-        hot[i] = h;
-    }
+    if (hot) hot[i] = make_hot_light(pr.x, pr.y, pr.z, dro.x, dro.y, dro.z, ci, pr.w, dro.w, inner, type);
     if (cull) {
         const float* v = cam->view;
         CullLight c;

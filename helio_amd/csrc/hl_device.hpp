@@ -39,7 +39,7 @@ struct alignas(16) HotLight {
     float4 p0, p1, p2, p3, p4;
 };
 
-// Per-light record the culling kernels read (32 B):
+// Per-light record the culling kernels read (64 B):
 //   vpos = (view-space position, range)  (range = +inf: directional, always listed)
 //   cone = (view-space direction (not normalized), cos outer)  (cos outer = -2: no cone)
 //   axis = (unit view-space axis, cos of the outer half-angle clamped to 1),
@@ -52,6 +52,32 @@ struct alignas(16) CullLight {
 };
 
 constexpr float kNoCone = -2.0f;
+
+// The shading record of one light from its fields: position (px, py, pz),
+// direction (dx, dy, dz) as stored (a spot light's axis, the direction a
+// directional light shines along), colour and intensity ci, range, the cosines
+// of the outer and inner cone angles, and the light type.
+__device__ __forceinline__ HotLight make_hot_light(float px, float py, float pz, float dx, float dy, float dz,
+                                                   float4 ci, float range_field, float outer, float inner,
+                                                   uint32_t type) {
+    const float range = type == HL_LIGHT_DIRECTIONAL ? __builtin_inff() : range_field;
+    HotLight h;
+    h.p0 = make_float4(px, py, pz, range);
+    {
+#pragma clang fp contract(off)
+        h.p1 = make_float4(ci.x * ci.w, ci.y * ci.w, ci.z * ci.w, inner);
+    }
+    h.p2 = make_float4(dx, dy, dz, type == HL_LIGHT_SPOT ? outer : kNoCone);
+    const float margin = 1e-5f * (fabsf(dx) + fabsf(dy) + fabsf(dz));
+    h.p3 = make_float4(1.0f / (range * range), ci.w, margin, 0.0f);
+    {
+#pragma clang fp contract(off)
+        const float r2 = range * range;
+        const float cmax = fmaxf(fmaxf(h.p1.x, h.p1.y), h.p1.z);
+        h.p4 = make_float4(r2 * (1.0f - 1e-5f), r2 * (1.0f + 1e-5f), cmax, 3.61e-5f * cmax);
+    }
+    return h;
+}
 
 // Texel formats the shading kernels are instantiated for.
 enum GbufSet : int { kSetRef = 0, kSet16 = 1, kSet32 = 2 };

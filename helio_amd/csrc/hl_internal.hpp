@@ -188,6 +188,34 @@ inline void set_tile_shape(ShadeArgs& a, uint32_t tw_log2) {
 // Packs surface attributes into the colour targets of gb (format set gbuf_set).
 hipError_t launch_gbuffer_pack(hipStream_t s, int gbuf_set, const hl_surface& sf, const hl_gbuffer& gb);
 
+// launchers (hl_forward.hip)
+// Everything a forward-lit launch reads (passed by value); the list walk reads
+// the grid and light fields by these names.
+struct ForwardArgs {
+    const HotLight* lights;
+    uint32_t num_lights;
+    const uint32_t* lists;
+    const uint32_t* counts;
+    const uint32_t* offsets;
+    uint32_t lists_capacity;
+    uint32_t grid_mode;               // HL_GRID_TILES / HL_GRID_CLUSTERS
+    uint32_t grid_x, grid_y, grid_z;  // tiles: num_tiles_x, num_tiles_y, 1
+    uint32_t cell_w, cell_h;          // clusters: pixels per cell
+    uint32_t tile_stride_x;           // tiles: ForwardLitGlobals.num_tiles_x
+    const GpuCameraUniforms* cam;
+    const hl_fragment* frags;
+    uint32_t num_frags;
+    float amb[3];                     // ambient_color.rgb * ambient_intensity
+    float* out;                       // RGBA32F per fragment
+    unsigned long long* pair_counter; // set: count pairs instead of shading
+    float depth_split[HL_MAX_SLICES + 1];
+};
+// Shading records of the lights with the positions and directions of their
+// transforms (entity == nullptr: the lights' own).
+hipError_t launch_forward_prep(hipStream_t s, const GpuLight* lights, uint32_t n, const uint32_t* entity,
+                               const hl_transform* transforms, uint32_t num_transforms, HotLight* hot);
+hipError_t launch_forward_lit(hipStream_t s, const ForwardArgs& a);
+
 // spans: nspans (first item, view) pairs in item order, then (num_items, 0).
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
                               const uint2* spans, uint32_t nspans, uint32_t num_items, uint32_t* queue,

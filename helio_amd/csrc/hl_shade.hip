@@ -14,175 +14,9 @@
 // Every pixel therefore sums exactly the lights of its own cell list that pass
 // the range test, in list order, as deferred_lighting.wgsl:1125-1145 does.
 // The adjoint re-derives the same per-wave lists instead of storing them.
-#include "hl_internal.hpp"
+#include "hl_lighting.hpp"
 
 namespace hl {
-
-constexpr int kStage = 64;  // LDS light records per wave and chunk
-constexpr float kPi = 3.14159265359f;
-constexpr float kInvPi = 1.0f / 3.14159265359f;
-constexpr float kThresh = 0.002f;  // deferred_lighting.wgsl:781
-
-struct Pixel {
-    float wx, wy, wz;
-    float nx, ny, nz;
-    float vx, vy, vz;
-    float f0[3];
-    float alb[3];
-    float rough, metal, ao;
-    float ndv, kk, kk1, gv, ev, one_m;  // kk1 = kk + 0.0001
-    float kd[3];     // (1 - metallic) albedo / pi
-    float a2, a2gv;  // roughness^4 and roughness^4 G1(NdV)
-    bool f0_free[3];
-};
-
-// Per-(pixel, light) quantities of pbr_direct_light (deferred_lighting.wgsl:699-820).
-struct Term {
-    float lx, ly, lz;
-    float rad[3];
-    float ndl;
-    float d, inv_d, att0, fall, s, dsdcos, atten;
-    float hx, hy, hz, inv_h;
-    float hdv, u, P, ndh, a2, den, Q, D, el, gl, G, den2;
-    float F[3], B[3];
-};
-
-// Range test of deferred_lighting.wgsl:1128-1131 (dist > range rejects) from
-// the fast squared distance; inside the +-1e-5 relative band around range^2
-// (L.p4.xy), which covers the rounding of both sides many times over, the
-// exact restatement decides.  The band is rare: the exact branch is taken only
-// when some lane of the wave needs it.
-__device__ __forceinline__ bool range_test(const HotLight& L, float tx, float ty, float tz, float d2) {
-    bool in = d2 < L.p4.x;
-    const bool band = !in && !(d2 > L.p4.y);
-    if (hl_ballot(band)) {
-        hl_rare();
-        if (band) in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > L.p0.w);
-    }
-    return in;
-}
-
-// Fast arithmetic, exact decisions.  Returns false when the term is exactly
-// zero in the restated shader.
-template <bool kFull>
-__device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Term& t) {
-    const bool dir = __builtin_isinf(L.p0.w);
-    if (dir) {
-        const float dx = -L.p2.x, dy = -L.p2.y, dz = -L.p2.z;
-        const float inv = hl_f_rsq(dx * dx + dy * dy + dz * dz);
-        t.lx = dx * inv;
-        t.ly = dy * inv;
-        t.lz = dz * inv;
-        if (kFull) {
-            t.rad[0] = L.p1.x;
-            t.rad[1] = L.p1.y;
-            t.rad[2] = L.p1.z;
-        }
-        t.att0 = t.fall = t.s = t.atten = 1.0f;
-        t.dsdcos = t.d = t.inv_d = 0.0f;
-        if (L.p4.z < kThresh) return false;
-    } else {
-        const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
-        const float d2 = tx * tx + ty * ty + tz * tz;
-        if (!range_test(L, tx, ty, tz, d2)) return false;
-        const float inv_d = hl_f_rsq(d2);
-        const float d = d2 * inv_d;
-        t.d = d;
-        t.inv_d = inv_d;
-        t.lx = tx * inv_d;
-        t.ly = ty * inv_d;
-        t.lz = tz * inv_d;
-        t.att0 = hl_f_rcp(d2 + 0.0001f);
-        const float nd2 = d2 * L.p3.x;  // (dist / range)^2
-        t.fall = fmaxf(0.0f, 1.0f - nd2 * nd2);
-        t.s = 1.0f;
-        t.dsdcos = 0.0f;
-        if (L.p2.w != kNoCone) {
-            // smoothstep(cos outer, cos inner, cos_a) (deferred_lighting.wgsl:
-            // 739-742) divides by cos(inner) - cos(outer) (0.005 for a 10-degree
-            // penumbra), which would amplify a last-bit error of a fast L ~200x.
-            // Outside the penumbra by more than the margin the factor is exactly
-            // 0 (the term vanishes) or 1; inside, it is evaluated exactly.
-            const float cf = -(t.lx * L.p2.x + t.ly * L.p2.y + t.lz * L.p2.z);
-            const bool ordered = L.p2.w < L.p1.w;
-            if (ordered && cf < L.p2.w - L.p3.z) return false;
-            if (!(ordered && cf > L.p1.w + L.p3.z)) {
-                hl_rare();
-                float cos_a;
-                {
-#pragma clang fp contract(off)
-                    const float dist = sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz));
-                    const float ex = tx / dist, ey = ty / dist, ez = tz / dist;
-                    cos_a = hl_x_dot3(-ex, -ey, -ez, L.p2.x, L.p2.y, L.p2.z);
-                }
-                t.s = hl_x_smoothstep(L.p2.w, L.p1.w, cos_a);
-                if (kFull) {
-                    const float inv_w = hl_f_rcp(L.p1.w - L.p2.w);
-                    const float tt = (cos_a - L.p2.w) * inv_w;
-                    t.dsdcos = (tt > 0.0f && tt < 1.0f) ? 6.0f * tt * (1.0f - tt) * inv_w : 0.0f;
-                }
-            }
-        }
-        const float atten = t.att0 * t.fall * t.s;
-        t.atten = atten;
-        if (kFull) {
-            t.rad[0] = L.p1.x * atten;
-            t.rad[1] = L.p1.y * atten;
-            t.rad[2] = L.p1.z * atten;
-        }
-        // max(rad) = ci * atten exactly (rounding is monotonic, atten >= 0).
-        // Error bound of the fast radiance (rcp/rsq of att0, |fall| near 0):
-        // 32e-6 ci att0 s + 4e-6 m <= 3.61e-5 ci att0 s, as m <= ci att0 s.
-        const float m = L.p4.z * atten;
-        const float err = L.p4.w * (t.att0 * t.s) + 1e-12f;
-        const float dm = m - kThresh;
-        if (dm < -err) return false;
-        if (fabsf(dm) < err) {  // ambiguous: decide with the exact restatement
-            hl_rare();
-            bool dummy;
-            if (hl_x_radiance_max(L, p.wx, p.wy, p.wz, &dummy) < kThresh) return false;
-        }
-    }
-    t.ndl = fmaxf(p.nx * t.lx + p.ny * t.ly + p.nz * t.lz, 0.0f);
-    if (t.ndl == 0.0f) return false;
-    // H = normalize(V + L) and N.H in the shader's order: for smooth surfaces
-    // (roughness^4 ~ 1e-5) the GGX denominator near the peak is as small as
-    // one ulp of N.H, so a rearranged N.H (e.g. (N.V + N.L) / |V + L|) moves D
-    // by percents there.
-    const float hx = p.vx + t.lx, hy = p.vy + t.ly, hz = p.vz + t.lz;
-    const float inv_h = hl_f_rsq(hx * hx + hy * hy + hz * hz);
-    t.inv_h = inv_h;
-    t.hx = hx * inv_h;
-    t.hy = hy * inv_h;
-    t.hz = hz * inv_h;
-    t.hdv = fmaxf(t.hx * p.vx + t.hy * p.vy + t.hz * p.vz, 0.0f);
-    t.ndh = fmaxf(p.nx * t.hx + p.ny * t.hy + p.nz * t.hz, 0.0f);
-    t.u = fminf(fmaxf(1.0f - t.hdv, 0.0f), 1.0f);
-    const float u2 = t.u * t.u;
-    t.P = u2 * u2 * t.u;
-    t.a2 = p.a2;
-    t.den = t.ndh * t.ndh * (p.a2 - 1.0f) + 1.0f;
-    t.Q = kPi * t.den * t.den + 0.0001f;
-    t.el = t.ndl * (1.0f - p.kk) + p.kk1;
-    t.den2 = 4.0f * p.ndv * t.ndl + 0.0001f;
-    float spec;
-    if (kFull) {
-        t.D = p.a2 * hl_f_rcp(t.Q);
-        t.gl = t.ndl * hl_f_rcp(t.el);
-        t.G = p.gv * t.gl;
-        spec = t.D * t.G * hl_f_rcp(t.den2);
-    } else {
-        // D G / den2 with one reciprocal: a2 gv NdL / (Q el den2); for
-        // roughness in [0, 1] each factor lies in [1e-4, 4.1]
-        spec = p.a2gv * t.ndl * hl_f_rcp(t.Q * t.el * t.den2);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        t.F[c] = p.f0[c] + (1.0f - p.f0[c]) * t.P;
-        t.B[c] = p.kd[c] + t.F[c] * (spec - p.kd[c]);  // (1 - F) kd + spec F
-    }
-    return true;
-}
 
 // Reverse-mode derivative of one active term given dL/dout (gl).  Per-pixel
 // gradients accumulate into dA (albedo), dR (roughness), dM (metallic); the
@@ -270,74 +104,6 @@ __device__ __forceinline__ void adjoint_term(const HotLight& L, const Pixel& p, 
     g.x = (alx - t.lx * ld) * t.inv_d + adj_d * t.lx;
     g.y = (aly - t.ly * ld) * t.inv_d + adj_d * t.ly;
     g.z = (alz - t.lz * ld) * t.inv_d + adj_d * t.lz;
-}
-
-// Walks the lights of every distinct cell among the wave's valid pixels:
-// per cell, refines its list against the AABB of those pixels, compacts the
-// survivors into LDS and calls f(light, index, mine) for each, uniformly
-// across the wave.
-template <int GRID, class F>
-__device__ __forceinline__ void walk_lights(const ShadeArgs& a, HotLight* s_l, uint32_t* s_i,
-                                            const Pixel& p, bool valid, uint32_t cell, F&& f) {
-    const uint32_t lane = hl_lane();
-    uint64_t todo = hl_ballot(valid);
-    while (todo) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
-        const uint32_t ucell = hl_readlane(cell, leader);
-        const bool mine = valid && cell == ucell;
-        todo &= ~hl_ballot(mine);
-        const float bx0 = dpp_min_if(mine, p.wx), bx1 = dpp_max_if(mine, p.wx);
-        const float by0 = dpp_min_if(mine, p.wy), by1 = dpp_max_if(mine, p.wy);
-        const float bz0 = dpp_min_if(mine, p.wz), bz1 = dpp_max_if(mine, p.wz);
-        uint32_t cnt = hl_uniform(hl_g(a.counts)[ucell]);
-        uint32_t base = GRID == HL_GRID_TILES ? ucell * HL_MAX_LIGHTS_PER_TILE
-                                              : hl_uniform(hl_g(a.offsets)[ucell]);
-        if (base >= a.lists_capacity) cnt = 0;
-        else if (cnt > a.lists_capacity - base) cnt = a.lists_capacity - base;
-        const uint32_t* list = a.lists + base;
-        for (uint32_t j0 = 0; j0 < cnt; j0 += 64u) {
-            const uint32_t j = j0 + lane;
-            bool pass = false;
-            uint32_t idx = 0;
-            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), l1 = q, l2 = q, l3 = q, l4 = q;
-            if (j < cnt) {
-                idx = hl_g(list)[j];
-                if (idx < a.num_lights) {
-                    // the whole record now, so that staging it does not wait
-                    // on a second load after the test
-                    const HotLight* hp = a.lights + idx;
-                    q = hl_ld4(&hp->p0);
-                    l1 = hl_ld4(&hp->p1);
-                    l2 = hl_ld4(&hp->p2);
-                    l3 = hl_ld4(&hp->p3);
-                    l4 = hl_ld4(&hp->p4);
-                    const float dx = fmaxf(fmaxf(bx0 - q.x, q.x - bx1), 0.0f);
-                    const float dy = fmaxf(fmaxf(by0 - q.y, q.y - by1), 0.0f);
-                    const float dz = fmaxf(fmaxf(bz0 - q.z, q.z - bz1), 0.0f);
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    pass = d2 <= q.w * q.w * 1.0001f + 1e-12f;
-                }
-            }
-            const uint64_t m = hl_ballot(pass);
-            if (pass) {
-                const uint32_t pos = hl_prefix(m);
-                s_l[pos].p0 = q;
-                s_l[pos].p1 = l1;
-                s_l[pos].p2 = l2;
-                s_l[pos].p3 = l3;
-                s_l[pos].p4 = l4;
-                s_i[pos] = idx;
-            }
-            const uint32_t n = (uint32_t)__popcll(m);
-            hl_wave_sync();
-            for (uint32_t i = 0; i < n; ++i) {
-                const HotLight L = s_l[i];
-                const uint32_t li = s_i[i];
-                f(L, li, mine);
-            }
-            hl_wave_sync();
-        }
-    }
 }
 
 // -------------------------------------------------------------- block tile

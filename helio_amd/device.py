@@ -310,6 +310,64 @@ class DeferredLightPass:
         return self.out.download(np.float32, (self.height, self.width, 4))
 
 
+class ForwardLitPass:
+    """ForwardLit's fragment stage (hl_forward_lit): fs_main of forward_lit.wgsl
+    over a batch of fragments, lit from the light grid LightCull published --
+    the second consumer of that grid
+    (crates/passes/3d/helio-pass-forward-lit/src/lib.rs:328-615).  The
+    rasteriser that produces the fragments and the blend into pre_aa stay with
+    the caller; one RGBA value per fragment comes back."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self.globals = None
+        self.out = None
+
+    def name(self) -> str:
+        return "ForwardLit"
+
+    def reads(self) -> tuple:
+        return ("main_scene", "depth", "pre_aa", "cluster_light_grid")
+
+    def writes(self) -> tuple:
+        return ("pre_aa",)
+
+    def prepare(self, scene: DeviceScene) -> None:
+        from .synth import forward_globals
+        self.globals = forward_globals(scene.host)
+
+    def _args(self, scene, fragments, num_fragments, grid, lights, entity, transforms, num_transforms):
+        lights = lights if lights is not None else scene.lights
+        return (self.ctx.h, scene.camera.ptr, C.byref(self.globals), fragments.ptr if num_fragments else None,
+                num_fragments, lights.ptr, scene.num_lights, entity.ptr if entity is not None else None,
+                transforms.ptr if transforms is not None else None, num_transforms, C.byref(grid))
+
+    def execute(self, scene: DeviceScene, fragments: DeviceArray, num_fragments: int, grid: abi.hl_light_grid,
+                lights: DeviceArray | None = None, entity: DeviceArray | None = None,
+                transforms: DeviceArray | None = None, num_transforms: int = 0) -> DeviceArray:
+        """Shades num_fragments fragments (abi.FRAGMENT_DTYPE records) into
+        self.out (RGBA32F per fragment).  lights defaults to the scene's."""
+        if self.out is None or self.out.nbytes < 16 * max(num_fragments, 1):
+            self.out = self.ctx.alloc(16 * max(num_fragments, 1))
+        self.ctx.lib.call("hl_forward_lit", *self._args(scene, fragments, num_fragments, grid, lights, entity,
+                                                        transforms, num_transforms),
+                          self.out.ptr, 0, ctx=self.ctx.h)
+        return self.out
+
+    def count_pairs(self, scene: DeviceScene, fragments: DeviceArray, num_fragments: int,
+                    grid: abi.hl_light_grid, lights: DeviceArray | None = None, entity: DeviceArray | None = None,
+                    transforms: DeviceArray | None = None, num_transforms: int = 0) -> int:
+        n = C.c_uint64()
+        self.ctx.lib.call("hl_forward_lit_count_pairs", *self._args(scene, fragments, num_fragments, grid, lights,
+                                                                    entity, transforms, num_transforms),
+                          C.byref(n), ctx=self.ctx.h)
+        return n.value
+
+    def output(self, num_fragments: int) -> np.ndarray:
+        return self.out.download(np.float32, (num_fragments, 4), nbytes=16 * num_fragments)
+
+
 class AdjointBuffers:
     """Outputs of the adjoint: loss (FP64), 5 per-pixel gradient planes
     (albedo r, g, b, roughness, metallic) and per-light position / intensity

@@ -314,3 +314,108 @@ def multiview_specs(seed: int = SEED_BASE + 16 * 5, small: bool = False) -> tupl
         n = int(round(math.exp(rng.uniform(math.log(32), math.log(4096)))))
         subsets.append(light_subset(seed, i, pool.num_lights, n))
     return views, subsets, pool
+
+
+# ------------------------------------------------------------ forward-lit
+def _decode_plane(arr: np.ndarray, fmt: int, H: int, W: int) -> np.ndarray:
+    if fmt == abi.HL_FMT_RGBA8_UNORM:
+        return arr.reshape(H, W, 4).astype(np.float32) / np.float32(255.0)
+    if fmt == abi.HL_FMT_RGBA16_FLOAT:
+        return arr.view(np.float16).reshape(H, W, 4).astype(np.float32)
+    return arr.view(np.float32).reshape(H, W, 4)
+
+
+def make_fragments(scene: HostScene, order: str = "tiles", seed: int = 0) -> np.ndarray:
+    """One forward-shaded fragment (abi.FRAGMENT_DTYPE) per non-sky pixel of
+    the scene: world position at the pixel centre from its depth (computed in
+    double, so it projects into its pixel), the unit normal, albedo and alpha,
+    ao, roughness clamped to [0.045, 1] and metallic of the G-buffer, F0 of the
+    metallic workflow and the emissive colour -- what a rasteriser plus the
+    material evaluation would hand the forward-lit shader for that surface.
+    order: "tiles" (8x8-pixel blocks in raster order, as a tiled rasteriser
+    emits them), "rows" (raster order) or "shuffled" (a random permutation)."""
+    H, W = scene.height, scene.width
+    fa, _, fn, _ = scene.formats
+    ys, xs = np.nonzero(scene.depth < 1.0)
+    if order == "tiles":
+        key = ((ys // 8) * ((W + 7) // 8) + xs // 8) * 64 + (ys % 8) * 8 + xs % 8
+        o = np.argsort(key, kind="stable")
+        ys, xs = ys[o], xs[o]
+    elif order == "shuffled":
+        o = np.random.default_rng(seed).permutation(len(ys))
+        ys, xs = ys[o], xs[o]
+    elif order != "rows":
+        raise ValueError(f"unknown fragment order {order!r}")
+    n = len(ys)
+    f = np.zeros(n, dtype=abi.FRAGMENT_DTYPE)
+    ivp = np.array(scene.camera.inv_view_proj, dtype=np.float64).reshape(4, 4).T  # column-major
+    d = scene.depth[ys, xs].astype(np.float64)
+    ndc = np.stack([((xs + 0.5) / W) * 2.0 - 1.0, 1.0 - ((ys + 0.5) / H) * 2.0, d, np.ones(n)])
+    h = ivp @ ndc
+    f["world_pos"] = (h[:3] / h[3]).T.astype(np.float32)
+    f["pixel"] = xs.astype(np.uint32) | (ys.astype(np.uint32) << 16)
+    alb = _decode_plane(scene.albedo, fa, H, W)[ys, xs]
+    nrm = _decode_plane(scene.normal, fn, H, W)[ys, xs].astype(np.float64)
+    orm = _decode_plane(scene.orm, fa, H, W)[ys, xs]
+    emi = _decode_plane(scene.emissive, fn, H, W)[ys, xs]
+    nn = nrm[:, :3] / np.linalg.norm(nrm[:, :3], axis=1, keepdims=True)
+    f["normal"] = nn.astype(np.float32)
+    f["alpha"] = alb[:, 3]
+    f["albedo"] = alb[:, :3]
+    f["ao"] = orm[:, 0]
+    f["roughness"] = np.clip(orm[:, 1], np.float32(0.045), np.float32(1.0))
+    m = orm[:, 2:3]
+    f["metallic"] = m[:, 0]
+    f["specular_f0"] = np.clip(np.float32(0.04) * (np.float32(1.0) - m) + alb[:, :3] * m,
+                               np.float32(0.0), np.float32(0.999))
+    f["emissive"] = emi[:, :3]
+    return f
+
+
+def direction_from_rotation_f64(rot_deg: np.ndarray) -> np.ndarray:
+    """-Z rotated by Ry(yaw) Rx(pitch) Rz(roll), rotation = [pitch, yaw, roll]
+    in degrees, in double (rotation matrices, not the shader's quaternions)."""
+    r = np.radians(np.asarray(rot_deg, dtype=np.float64))
+    p, y, z = r[..., 0], r[..., 1], r[..., 2]
+    # R = Ry Rx Rz applied to (0, 0, -1): minus the third column of R
+    cx, sx, cy, sy = np.cos(p), np.sin(p), np.cos(y), np.sin(y)
+    col = np.stack([sy * cx, -sx, cy * cx], axis=-1)  # Ry Rx e_z (Rz leaves e_z)
+    return -col
+
+
+def make_light_transforms(lights: np.ndarray, seed: int = 0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """SceneDB-style light transforms for a light array: a transform table of
+    2n + 3 entries of which the lights own a random subset (entity indices),
+    each with the light's position and a random rotation.  Returns (lights
+    with direction_outer.xyz set to the rotated -Z, so that the clustered
+    cull's cone test sees the direction the forward shader uses; entity
+    indices; transforms)."""
+    n = len(lights)
+    rng = np.random.default_rng(seed)
+    rot = rng.uniform(-180.0, 180.0, size=(n, 3)).astype(np.float32)
+    ntr = 2 * n + 3
+    entity = rng.permutation(ntr)[:n].astype(np.uint32)
+    tr = np.zeros(ntr, dtype=abi.TRANSFORM_DTYPE)
+    tr["scale"] = 1.0
+    tr["position"][entity] = lights["position_range"][:, :3]
+    tr["rotation"][entity] = rot
+    out = lights.copy()
+    out["direction_outer"][:, :3] = direction_from_rotation_f64(rot).astype(np.float32)
+    return out, entity, tr
+
+
+def forward_globals(scene: HostScene) -> abi.ForwardLitGlobals:
+    """ForwardLitGlobals as ForwardLitPass::prepare fills them
+    (crates/passes/3d/helio-pass-forward-lit/src/lib.rs:382-406)."""
+    g = abi.ForwardLitGlobals()
+    hg = scene.globals
+    g.frame = hg.frame
+    g.delta_time = hg.delta_time
+    g.light_count = len(scene.lights)
+    g.ambient_intensity = hg.ambient_intensity
+    g.ambient_color[:] = (hg.ambient_color[0], hg.ambient_color[1], hg.ambient_color[2], 1.0)
+    g.num_tiles_x = (scene.width + 15) // 16
+    g.num_tiles_y = (scene.height + 15) // 16
+    g.screen_width = float(scene.width)
+    g.screen_height = float(scene.height)
+    return g

@@ -280,6 +280,70 @@ HL_API hl_status hl_count_pixel_light_pairs(hl_context* ctx, const GpuCameraUnif
                                             uint32_t num_lights, const hl_light_grid* grid,
                                             uint64_t* pairs);
 
+/* ------------------------------------------------------------ forward-lit */
+/* One fragment of a forward-shaded surface: what ForwardLit's fs_main holds
+ * after its surface evaluation (forward_lit.wgsl:245-282, 440-456), 80 bytes.
+ * pixel is the framebuffer pixel the fragment covers (x | y << 16), whose
+ * 16-px tile (or cluster) selects the light list (forward_lit.wgsl:463-466). */
+typedef struct hl_fragment {
+    float world_pos[3];
+    uint32_t pixel;          /* x | y << 16                                     */
+    float normal[3];         /* N as the surface evaluation returns it (unit)   */
+    float alpha;             /* albedo.a; <= 0.001 discards the fragment        */
+    float albedo[3];
+    float ao;
+    float specular_f0[3];    /* F0 as resolved by the surface evaluation        */
+    float roughness;
+    float emissive[3];
+    float metallic;
+} hl_fragment;
+
+/* ForwardLit fs_main over a batch of fragments (forward_lit.wgsl:440-493), the
+ * second consumer of the light grid (crates/libhelio/src/frame.rs:382-383).
+ * Replaces the fragment stage of ForwardLitPass::execute
+ * (crates/passes/3d/helio-pass-forward-lit/src/lib.rs:408-615); rasterisation,
+ * depth test and blending stay with the caller.
+ * Light world positions and directions come from
+ * transforms_d[light_entity_indices_d[i]] (position; direction = -Z rotated by
+ * the YXZ Euler rotation, forward_lit.wgsl:337-372), as the shader reads them;
+ * range, cone angles, colour and type from lights_d.  With
+ * light_entity_indices_d == NULL (and transforms_d NULL) the lights' own
+ * position_range.xyz / direction_outer.xyz are used.  A light whose entity
+ * index is >= num_transforms contributes nothing.
+ * globals: ForwardLitGlobals as ForwardLitPass::prepare writes it
+ *   (lib.rs:382-406); ambient_color, ambient_intensity, num_tiles_x/y and
+ *   screen_width/height are read.
+ * grid: the light lists -- tiles (the reference's tile_light_lists /
+ *   tile_light_counts, indexed with globals.num_tiles_x) or clusters (the
+ *   fragment's slice from the NDC depth of its world position).
+ * out_d: one RGBA32F value per fragment, in fragment order:
+ *   (Lo + ambient + emissive, alpha); a discarded fragment (alpha <= 0.001)
+ *   gets (0, 0, 0, 0).  A fragment whose pixel lies outside the grid is lit by
+ *   no light (ambient and emissive only).
+ * fragments_d and out_d are 16-byte aligned.  flags must be 0.
+ * Fragments in screen-coherent order (as a rasteriser emits them, or sorted
+ * by tile) share light lists within a wavefront; any order is correct.
+ * Radiance is within 1e-4 relative of the FP32 restatement evaluating the same
+ * lists. */
+HL_API hl_status hl_forward_lit(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                const ForwardLitGlobals* globals, const hl_fragment* fragments_d,
+                                uint32_t num_fragments, const GpuLight* lights_d, uint32_t num_lights,
+                                const uint32_t* light_entity_indices_d,
+                                const hl_transform* transforms_d, uint32_t num_transforms,
+                                const hl_light_grid* grid, float* out_d, uint32_t flags);
+
+/* Counts the (fragment, light) pairs hl_forward_lit evaluates past the range
+ * test (kept fragments only) -- its "fragment-light evals".  Same inputs;
+ * writes *pairs (host). */
+HL_API hl_status hl_forward_lit_count_pairs(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                            const ForwardLitGlobals* globals,
+                                            const hl_fragment* fragments_d, uint32_t num_fragments,
+                                            const GpuLight* lights_d, uint32_t num_lights,
+                                            const uint32_t* light_entity_indices_d,
+                                            const hl_transform* transforms_d,
+                                            uint32_t num_transforms, const hl_light_grid* grid,
+                                            uint64_t* pairs);
+
 /* ---------------------------------------------------------------- adjoint */
 /* Per-pixel gradient planes: d_pixel_d holds 5 planes of width*height floats,
  * plane-major: dL/d albedo.r, .g, .b, dL/d roughness, dL/d metallic. */

@@ -122,6 +122,30 @@ typedef struct GpuMaterial {
     float class_params[4];       /* 80 */
 } GpuMaterial;
 
+/* ForwardLitGlobals, the forward-lit pass's uniform
+ * (crates/passes/3d/helio-pass-forward-lit/src/lib.rs:10-22,
+ * shaders/forward_lit.wgsl:17-27). */
+typedef struct ForwardLitGlobals {
+    uint32_t frame;                /*  0 */
+    float delta_time;              /*  4 */
+    uint32_t light_count;          /*  8 */
+    float ambient_intensity;       /* 12 */
+    float ambient_color[4];        /* 16 */
+    uint32_t num_tiles_x;          /* 32 */
+    uint32_t num_tiles_y;          /* 36 */
+    float screen_width;            /* 40 */
+    float screen_height;           /* 44 */
+} ForwardLitGlobals;
+
+/* SceneDB's Transform component as the forward-lit shader binds it: three
+ * packed [f32; 3] (forward_lit.wgsl:110-122), rotation in degrees
+ * [pitch_x, yaw_y, roll_z]. */
+typedef struct hl_transform {
+    float position[3];             /*  0 */
+    float rotation[3];             /* 12 */
+    float scale[3];                /* 24 */
+} hl_transform;
+
 #ifdef __cplusplus
 #define HL_STATIC_ASSERT static_assert
 #else
@@ -164,6 +188,11 @@ HL_STATIC_ASSERT(sizeof(GpuMaterial) == 96, "GpuMaterial must be 96 bytes");
 HL_STATIC_ASSERT(offsetof(GpuMaterial, roughness_metallic) == 32, "material.roughness_metallic");
 HL_STATIC_ASSERT(offsetof(GpuMaterial, workflow) == 68, "material.workflow");
 HL_STATIC_ASSERT(offsetof(GpuMaterial, class_params) == 80, "material.class_params");
+
+HL_STATIC_ASSERT(sizeof(ForwardLitGlobals) == 48, "ForwardLitGlobals must be 48 bytes");
+HL_STATIC_ASSERT(offsetof(ForwardLitGlobals, num_tiles_x) == 32, "forward globals.num_tiles_x");
+HL_STATIC_ASSERT(sizeof(hl_transform) == 36, "Transform must be 36 bytes (three packed [f32; 3])");
+HL_STATIC_ASSERT(offsetof(hl_transform, rotation) == 12, "transform.rotation");
 
 #ifdef __cplusplus
 }

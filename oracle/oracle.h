@@ -16,6 +16,8 @@
  *  - FP64 differentiable forward, analytic adjoint and central finite differences
  *    (or64_*), all lights evaluated per pixel (no grid).
  *  - The G-buffer producer's format contract (or_gbuffer_pack, gbuffer_pack.c).
+ *  - The forward-lit fragment shader (or_forward_lit, forward_lit.c), the
+ *    second consumer of the light grid.
  *
  * WGSL leaves a few operation orders to the backend; this oracle fixes them:
  *    dot(a,b)      = (a.x*b.x + a.y*b.y) + a.z*b.z
@@ -145,6 +147,18 @@ void or_gbuffer_pack(const hl_surface* surface, const hl_gbuffer* gbuffer);
 uint16_t or_float_to_half(float f);
 uint32_t or_unorm8_encode(float v);
 float or_f0_metallic(float albedo, float metallic);
+
+/* Forward-lit fragment shading (forward_lit.c, forward_lit.wgsl:337-493):
+ * hl_forward_lit on host memory (out may be NULL: count only; returns the
+ * (fragment, light) pairs past the range test), and its pieces. */
+uint64_t or_forward_lit(const GpuCameraUniforms* cam, const ForwardLitGlobals* g, const hl_fragment* frags,
+                        uint32_t n, const GpuLight* lights, uint32_t num_lights, const uint32_t* entity,
+                        const hl_transform* transforms, uint32_t num_transforms, const hl_light_grid* grid,
+                        float* out);
+void or_light_direction_from_rotation(const float rot_deg[3], float out[3]);
+void or_forward_pbr_direct_light(const GpuLight* light, const float light_pos[3], const float light_dir[3],
+                                 const float wp[3], const float N[3], const float V[3], const float F0[3],
+                                 const float albedo[3], float roughness, float metallic, float out[3]);
 
 #ifdef __cplusplus
 }

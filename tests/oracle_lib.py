@@ -30,6 +30,11 @@ _SIGS = {
     "or_float_to_half": (C.c_uint16, [C.c_float]),
     "or_unorm8_encode": (u32, [C.c_float]),
     "or_f0_metallic": (C.c_float, [C.c_float, C.c_float]),
+    "or_forward_lit": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.ForwardLitGlobals), vp, u32, vp, u32, vp, vp,
+                                    u32, P(abi.hl_light_grid), vp]),
+    "or_light_direction_from_rotation": (None, [P(C.c_float), P(C.c_float)]),
+    "or_forward_pbr_direct_light": (None, [vp, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
+                                           P(C.c_float), P(C.c_float), C.c_float, C.c_float, P(C.c_float)]),
     "or_shade": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
                               P(abi.hl_light_grid), vp, u32, u32, u32, u32]),
     "or_count_pairs": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
@@ -124,3 +129,32 @@ def shade(scene, grid, globals_=None, flags=0, rows=None, out=None):
                               scene.lights.ctypes.data if len(scene.lights) else None, len(scene.lights),
                               C.byref(grid), out.ctypes.data, W * 16, flags, r0, r1)
     return out, pairs
+
+
+def forward_lit(camera, globals_, frags, lights, grid, entity=None, transforms=None, count_only=False):
+    """FP32 restatement of forward_lit.wgsl fs_main; returns (rgba [n, 4] or None, pairs)."""
+    n = len(frags)
+    out = None if count_only else np.zeros((n, 4), dtype=np.float32)
+    pairs = oracle().or_forward_lit(C.byref(camera), C.byref(globals_), frags.ctypes.data if n else None, n,
+                                    lights.ctypes.data if len(lights) else None, len(lights),
+                                    entity.ctypes.data if entity is not None else None,
+                                    transforms.ctypes.data if transforms is not None else None,
+                                    len(transforms) if transforms is not None else 0, C.byref(grid),
+                                    out.ctypes.data if out is not None else None)
+    return out, pairs
+
+
+def all_lights_grid(num_lights, width, height):
+    """One cluster covering the screen that lists every light in order (a grid
+    that cannot miss a light)."""
+    p = abi.hl_cluster_params()
+    p.grid_x = p.grid_y = p.grid_z = 1
+    p.cell_w, p.cell_h, p.width, p.height = width, height, width, height
+    p.near_plane, p.far_plane = 0.1, 1000.0
+    off = np.zeros(1, np.uint32)
+    cnt = np.array([num_lights], np.uint32)
+    pool = np.arange(max(num_lights, 1), dtype=np.uint32)
+    g = cluster_grid_host(p, off, cnt, pool)
+    g._params = p
+    g.clusters = C.pointer(p)
+    return g

@@ -66,15 +66,17 @@ def test_c_header_static_asserts_compile(tmp_path):
 def test_ctypes_structs_match_c_sizes(tmp_path):
     src = tmp_path / "s.c"
     src.write_text('#include <stdio.h>\n#include "helio_hip/helio_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(hl_gbuffer),'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(hl_gbuffer),'
                    'sizeof(hl_cluster_params), sizeof(hl_light_grid), sizeof(hl_view_desc),'
-                   'sizeof(hl_texture), sizeof(hl_surface));return 0;}\n')
+                   'sizeof(hl_texture), sizeof(hl_surface), sizeof(hl_fragment), sizeof(hl_transform),'
+                   'sizeof(ForwardLitGlobals));return 0;}\n')
     exe = tmp_path / "s"
     subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     assert sizes == [C.sizeof(abi.hl_gbuffer), C.sizeof(abi.hl_cluster_params),
                      C.sizeof(abi.hl_light_grid), C.sizeof(abi.hl_view_desc), C.sizeof(abi.hl_texture),
-                     C.sizeof(abi.hl_surface)]
+                     C.sizeof(abi.hl_surface), C.sizeof(abi.hl_fragment), C.sizeof(abi.hl_transform),
+                     C.sizeof(abi.ForwardLitGlobals)]
 
 
 def declared_symbols():
