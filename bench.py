@@ -20,7 +20,7 @@ Further legs, each reported in its own object of the same line:
   gbuffer_pack_4k the G-buffer producer's format contract (hl_gbuffer_pack) at
                   3840x2160 into the reference formats
   forward_lit_4k  forward-lit fragment shading (hl_forward_lit) of the 4K
-                  frame's 8.3 M surfaces as fragments (8x8-pixel blocks in
+                  frame's 4.9 M surfaces as fragments (8x8-pixel blocks in
                   raster order) under the same 16384 lights read through
                   entity indices and transforms, on the reference's 16-px tile
                   lists (cap 64) and on the 32x18x48 cluster lists
