@@ -19,6 +19,10 @@ Further legs, each reported in its own object of the same line:
                   fused forward / L2 loss / adjoint per step (and forward only)
   gbuffer_pack_4k the G-buffer producer's format contract (hl_gbuffer_pack) at
                   3840x2160 into the reference formats
+  shadowed_4k     the 4K/16k frame with shadow_factor (hl_deferred_forward_
+                  shadowed): a directional sun (4 cascades) and 8 point + 8
+                  spot shadow casters, 1024^2 atlases (60 layers, dynamic and
+                  static), PCF (medium preset) and PCSS (high preset)
   forward_lit_4k  forward-lit fragment shading (hl_forward_lit) of the 4K
                   frame's 4.9 M surfaces as fragments (8x8-pixel blocks in
                   raster order) under the same 16384 lights read through
@@ -208,6 +212,40 @@ def bench_gbuffer_pack(ctx, args):
                         "(metallic-workflow F0), hl_gbuffer_pack"}
 
 
+def bench_shadowed(ctx, args):
+    """Cluster cull + shadowed shading of the 4K/16k frame with 17 shadow
+    casters; whole frames timed as in the headline, per preset."""
+    from helio_amd import synth
+    from helio_amd.device import DeferredLightPass, DeviceShadows, LightCullPass, upload_scene
+    sc = synth.make_scene(args.workload)
+    sh = synth.add_shadow_casters(sc, n_point=8, n_spot=8, directional=True, size=1024, quality="medium")
+    ds = upload_scene(ctx, sc)
+    dsh = DeviceShadows(ctx, sh)
+    params = synth.cluster_params(ctx.lib, sc)
+    cull = LightCullPass(ctx, sc.width, sc.height, clusters=params, pool_cap=1 << 24)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    cull.prepare(ds)
+    shade.prepare(ds)
+    cull.execute(ds)
+    grid = cull.publish()
+    q = shade.count_pairs(ds, grid)
+    res = {"workload": f"{args.workload} with a directional sun (4 cascades) and 8 point + 8 spot shadow casters: "
+                       f"{sh.layers} layers of {sh.size}^2 in each atlas (dynamic, static)",
+           "pairs_q": q, "layers": sh.layers, "atlas_size": sh.size}
+    ms_plain, parts = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
+                                 lambda: shade.execute(ds, grid))
+    res["unshadowed"] = {"ms_per_frame": ms_plain, "shade_ms": parts[1]}
+    for quality in ("medium", "high"):
+        dsh.desc.config = synth.shadow_config(quality)
+        ms, parts = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
+                               lambda: shade.execute(ds, grid, shadows=dsh))
+        res[quality] = {"ms_per_frame": ms, "shade_ms": parts[1], "evals_per_s": q / (ms * 1e-3),
+                        "pcss": bool(dsh.desc.config.enable_pcss),
+                        "pcf_samples": int(dsh.desc.config.pcf_sample_count)}
+        log(f"shadowed 4K {quality}: {ms:.3f} ms/frame (shade {parts[1] * 1e3:.1f} us)")
+    return res
+
+
 def bench_forward_lit(ctx, args):
     """hl_forward_lit over every surface of the 4K/16k scene as one fragment
     (80-byte records), lights positioned through entity indices and
@@ -346,6 +384,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 512x512 forward+adjoint leg")
     ap.add_argument("--no-pack", action="store_true", help="skip the 4K G-buffer pack leg")
     ap.add_argument("--no-forward", action="store_true", help="skip the 4K forward-lit leg")
+    ap.add_argument("--no-shadows", action="store_true", help="skip the 4K shadowed leg")
     ap.add_argument("--multiview", action="store_true",
                     help="also run the full-size 50-view batched leg (opt-in: its autotuned plan hit a GPU "
                          "memory fault once, cause not yet found)")
@@ -379,6 +418,7 @@ def main() -> int:
         adj = None if args.no_adjoint else bench_adjoint(ctx, args)
         pack = None if args.no_pack else bench_gbuffer_pack(ctx, args)
         fwd = None if args.no_forward else bench_forward_lit(ctx, args)
+        shd = None if args.no_shadows else bench_shadowed(ctx, args)
         mv = bench_multiview(ctx, args) if args.multiview else None
 
     sc, q = r4k["scene"], r4k["q"]
@@ -437,6 +477,8 @@ def main() -> int:
         res["gbuffer_pack_4k"] = pack
     if fwd is not None:
         res["forward_lit_4k"] = fwd
+    if shd is not None:
+        res["shadowed_4k"] = shd
     res["multiview"] = mv if mv is not None else {
         "skipped": "full-size 50-view leg is opt-in (--multiview): its autotuned plan faulted the GPU once "
                    "and stays off the GPU until the cause is found; the 1/8-scale 50-view plan is "

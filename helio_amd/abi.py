@@ -95,6 +95,21 @@ FRAGMENT_DTYPE = np.dtype([("world_pos", "<f4", 3), ("pixel", "<u4"), ("normal",
 TRANSFORM_DTYPE = np.dtype([("position", "<f4", 3), ("rotation", "<f4", 3), ("scale", "<f4", 3)])
 
 
+class CascadeConfig(C.Structure):
+    _fields_ = [("split_distance", C.c_float), ("depth_bias", C.c_float), ("filter_radius", C.c_float),
+                ("pcss_light_size", C.c_float)]
+
+
+class ShadowConfig(C.Structure):
+    _fields_ = [("cascades", CascadeConfig * 4), ("enable_pcss", C.c_uint32), ("pcss_blocker_samples", C.c_uint32),
+                ("pcss_filter_samples", C.c_uint32), ("pcf_sample_count", C.c_uint32)]
+
+
+class hl_shadow_maps(C.Structure):
+    _fields_ = [("atlas_d", C.c_void_p), ("static_atlas_d", C.c_void_p), ("matrices_d", C.c_void_p),
+                ("num_layers", C.c_uint32), ("atlas_size", C.c_uint32), ("config", ShadowConfig)]
+
+
 class hl_texture(C.Structure):
     _fields_ = [("data_d", C.c_void_p), ("pitch_bytes", C.c_uint32), ("format", C.c_uint32)]
 
@@ -155,6 +170,7 @@ def check_layouts() -> None:
     assert C.sizeof(ForwardLitGlobals) == 48
     assert C.sizeof(hl_transform) == 36 == TRANSFORM_DTYPE.itemsize
     assert C.sizeof(hl_fragment) == 80 == FRAGMENT_DTYPE.itemsize
+    assert C.sizeof(ShadowConfig) == 80
 
 
 check_layouts()
@@ -200,6 +216,8 @@ SIGNATURES: dict[str, tuple] = {
     "hl_gbuffer_pack": (None, [vp, P(hl_surface), P(hl_gbuffer)]),
     "hl_deferred_forward": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                    P(hl_light_grid), vp, u32, u32]),
+    "hl_deferred_forward_shadowed": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
+                                            P(hl_light_grid), P(hl_shadow_maps), vp, u32, u32]),
     "hl_count_pixel_light_pairs": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                           P(hl_light_grid), P(C.c_uint64)]),
     "hl_forward_lit": (None, [vp, vp, P(ForwardLitGlobals), vp, u32, vp, u32, vp, vp, u32,

@@ -601,17 +601,32 @@ HL_API hl_status hl_gbuffer_pack(hl_context* c, const hl_surface* sf, const hl_g
     return HL_OK;
 }
 
-HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* camera_d,
-                                     const DeferredGlobals* g, const hl_gbuffer* gb,
-                                     const GpuLight* lights_d, uint32_t num_lights,
-                                     const hl_light_grid* grid, float* out_d,
-                                     uint32_t out_pitch_bytes, uint32_t flags) {
+}  // extern "C"
+
+namespace {
+
+// hl_deferred_forward and its shadowed variant (sh == nullptr: unshadowed).
+hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
+                      const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights, const hl_light_grid* grid,
+                      const hl_shadow_maps* sh, float* out_d, uint32_t out_pitch_bytes, uint32_t flags) {
     if (!c) return HL_EINVAL;
     if (!out_d) return fail(c, HL_EINVAL, "output is null");
     ShadeArgs a;
     int set;
     if (flags & ~(uint32_t)(HL_SHADE_CLEAR_SKY | HL_SHADE_LIGHTS_FROM_CULL))
         return fail(c, HL_EINVAL, "unknown shading flags 0x%x", flags);
+    if (sh) {
+        if (sh->atlas_size == 0 || sh->atlas_size > 16384)
+            return fail(c, HL_EINVAL, "shadow atlas_size %u out of range (1..16384)", sh->atlas_size);
+        if (sh->num_layers > 65536) return fail(c, HL_EINVAL, "%u shadow layers (at most 65536)", sh->num_layers);
+        if (sh->num_layers && (!sh->atlas_d || !sh->static_atlas_d || !sh->matrices_d))
+            return fail(c, HL_EINVAL, "shadow atlases or matrices are null");
+        if (((uintptr_t)sh->atlas_d | (uintptr_t)sh->static_atlas_d | (uintptr_t)sh->matrices_d) % 4)
+            return fail(c, HL_EINVAL, "shadow atlases and matrices must be 4-byte aligned");
+        const ShadowConfig& k = sh->config;
+        if (k.pcf_sample_count > 256 || k.pcss_blocker_samples > 256 || k.pcss_filter_samples > 256)
+            return fail(c, HL_EINVAL, "shadow sample counts above 256");
+    }
     hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, num_lights, grid, &a, &set, false,
                                 (flags & HL_SHADE_LIGHTS_FROM_CULL) != 0);
     if (s != HL_OK) return s;
@@ -621,8 +636,46 @@ HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* cam
     a.out = out_d;
     a.out_pitch = out_pitch_bytes;
     a.flags = flags;
-    if ((s = shade_launch(c, set, kModeForward, a)) != HL_OK) return s;
-    return HL_OK;
+    int mode = kModeForward;
+    if (sh) {
+        ShadowView& v = a.shadow;
+        v.atlas = sh->atlas_d;
+        v.static_atlas = sh->static_atlas_d;
+        v.matrices = sh->matrices_d;
+        v.layers = sh->num_layers;
+        v.size = sh->atlas_size;
+        v.pcf_count = sh->config.pcf_sample_count;
+        v.enable_pcss = sh->config.enable_pcss;
+        v.blocker_samples = sh->config.pcss_blocker_samples;
+        v.filter_samples = sh->config.pcss_filter_samples;
+        for (int k = 0; k < 4; ++k) {
+            v.filter_radius[k] = sh->config.cascades[k].filter_radius;
+            v.light_size[k] = sh->config.cascades[k].pcss_light_size;
+            v.csm[k] = g->csm_splits[k];
+        }
+        mode = kModeShadowed;
+    }
+    return shade_launch(c, set, mode, a);
+}
+
+}  // namespace
+
+extern "C" {
+
+HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* camera_d,
+                                     const DeferredGlobals* g, const hl_gbuffer* gb,
+                                     const GpuLight* lights_d, uint32_t num_lights,
+                                     const hl_light_grid* grid, float* out_d,
+                                     uint32_t out_pitch_bytes, uint32_t flags) {
+    return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, nullptr, out_d, out_pitch_bytes, flags);
+}
+
+HL_API hl_status hl_deferred_forward_shadowed(hl_context* c, const GpuCameraUniforms* camera_d,
+                                              const DeferredGlobals* g, const hl_gbuffer* gb,
+                                              const GpuLight* lights_d, uint32_t num_lights,
+                                              const hl_light_grid* grid, const hl_shadow_maps* shadows,
+                                              float* out_d, uint32_t out_pitch_bytes, uint32_t flags) {
+    return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, shadows, out_d, out_pitch_bytes, flags);
 }
 
 HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUniforms* camera_d,

@@ -39,7 +39,9 @@ __device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
     const float inner = l->inner_angle;
     const float inf = __builtin_inff();
     const float range = type == HL_LIGHT_DIRECTIONAL ? inf : pr.w;
-    if (hot) hot[i] = make_hot_light(pr.x, pr.y, pr.z, dro.x, dro.y, dro.z, ci, pr.w, dro.w, inner, type);
+    if (hot)
+        hot[i] = make_hot_light(pr.x, pr.y, pr.z, dro.x, dro.y, dro.z, ci, pr.w, dro.w, inner, type,
+                                shadow_word(i, type, l->shadow_index));
     if (cull) {
         const float* v = cam->view;
         CullLight c;

@@ -28,7 +28,7 @@ namespace hl {
 //   p0 = (position.xyz, range)          range = +inf for directional lights
 //   p1 = (color.rgb * intensity, cos inner)
 //   p2 = (direction.xyz, cos outer)     cos outer = -2 for non-spot lights
-//   p3 = (1 / range^2, intensity, cone margin, 0)
+//   p3 = (1 / range^2, intensity, cone margin, shadow word as bits)
 //   p4 = (range^2 (1 - 1e-5), range^2 (1 + 1e-5), ci, 3.61e-5 ci)
 //        with ci = max(p1.xyz): the range test's band and the 0.002 test's
 //        scale and error coefficient, so that the per-pixel loop does not
@@ -53,13 +53,38 @@ struct alignas(16) CullLight {
 
 constexpr float kNoCone = -2.0f;
 
+// Shadow word of a shading record (p3.w as bits): the light's first shadow
+// layer (28 bits; a larger shadow_index saturates, past any atlas) and its
+// kind in the top bits (0 directional, 1 point, 2 any other type: the spot
+// path of shadow_factor); all ones when the light casts no shadow.  Only
+// lights below MAX_SHADOW_LIGHTS = 42 (deferred_lighting.wgsl:20) carry one.
+constexpr uint32_t kMaxShadowLights = 42u;
+constexpr uint32_t kNoShadowWord = 0xFFFFFFFFu;
+__host__ __device__ inline uint32_t shadow_word(uint32_t index, uint32_t type, uint32_t shadow_index) {
+    if (index >= kMaxShadowLights || shadow_index == HL_NO_SHADOW) return kNoShadowWord;
+    const uint32_t kind = type == HL_LIGHT_DIRECTIONAL ? 0u : (type == HL_LIGHT_POINT ? 1u : 2u);
+    return (shadow_index < 0x0FFFFFFFu ? shadow_index : 0x0FFFFFFFu) | kind << 28;
+}
+
+// The shadow inputs of one shading launch (hl_shadow_maps): the two atlases,
+// the light matrices and the ShadowConfig fields shadow_factor reads.
+struct ShadowView {
+    const float* atlas;
+    const float* static_atlas;
+    const float* matrices;  // 16 per layer
+    uint32_t layers, size;
+    uint32_t pcf_count, enable_pcss, blocker_samples, filter_samples;
+    float filter_radius[4], light_size[4];
+    float csm[4];
+};
+
 // The shading record of one light from its fields: position (px, py, pz),
 // direction (dx, dy, dz) as stored (a spot light's axis, the direction a
 // directional light shines along), colour and intensity ci, range, the cosines
 // of the outer and inner cone angles, and the light type.
 __device__ __forceinline__ HotLight make_hot_light(float px, float py, float pz, float dx, float dy, float dz,
                                                    float4 ci, float range_field, float outer, float inner,
-                                                   uint32_t type) {
+                                                   uint32_t type, uint32_t shadow) {
     const float range = type == HL_LIGHT_DIRECTIONAL ? __builtin_inff() : range_field;
     HotLight h;
     h.p0 = make_float4(px, py, pz, range);
@@ -69,7 +94,7 @@ __device__ __forceinline__ HotLight make_hot_light(float px, float py, float pz,
     }
     h.p2 = make_float4(dx, dy, dz, type == HL_LIGHT_SPOT ? outer : kNoCone);
     const float margin = 1e-5f * (fabsf(dx) + fabsf(dy) + fabsf(dz));
-    h.p3 = make_float4(1.0f / (range * range), ci.w, margin, 0.0f);
+    h.p3 = make_float4(1.0f / (range * range), ci.w, margin, __uint_as_float(shadow));
     {
 #pragma clang fp contract(off)
         const float r2 = range * range;
@@ -123,6 +148,7 @@ struct ShadeArgs {
     uint32_t num_blocks;     // block tiles of the view
     uint32_t tw_log2;        // wavefront tile: (1 << tw_log2) x (64 >> tw_log2) pixels
     uint32_t _pad0;
+    ShadowView shadow;       // kModeShadowed only
 };
 
 // One view of a batched persistent launch (device memory).

@@ -72,6 +72,7 @@ __global__ __launch_bounds__(256) void k_forward_prep(const GpuLight* __restrict
             h.p0 = make_float4(inf, inf, inf, 0.0f);
             h.p1 = h.p2 = h.p3 = h.p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             h.p2.w = kNoCone;
+            h.p3.w = __uint_as_float(kNoShadowWord);
             hot[i] = h;
             return;
         }
@@ -80,7 +81,9 @@ __global__ __launch_bounds__(256) void k_forward_prep(const GpuLight* __restrict
         const float* r = transforms[e].rotation;
         dir = direction_from_rotation(r[0], r[1], r[2]);
     }
-    hot[i] = make_hot_light(pos.x, pos.y, pos.z, dir.x, dir.y, dir.z, ci, pr.w, dro.w, l->inner_angle, type);
+    // forward_lit.wgsl samples no shadow map
+    hot[i] = make_hot_light(pos.x, pos.y, pos.z, dir.x, dir.y, dir.z, ci, pr.w, dro.w, l->inner_angle, type,
+                            kNoShadowWord);
 }
 
 constexpr int kFwdModeShade = 0, kFwdModeCount = 1;

@@ -15,6 +15,7 @@
 // the range test, in list order, as deferred_lighting.wgsl:1125-1145 does.
 // The adjoint re-derives the same per-wave lists instead of storing them.
 #include "hl_lighting.hpp"
+#include "hl_shadow.hpp"
 
 namespace hl {
 
@@ -150,7 +151,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
     const bool valid = inside && depth < 1.0f;
     constexpr bool kAdj = MODE == kModeBackward || MODE == kModeFusedL2;
-    constexpr bool kWrites = MODE == kModeForward || MODE == kModeFusedL2;
+    constexpr bool kShadow = MODE == kModeShadowed;
+    constexpr bool kFwd = MODE == kModeForward || kShadow;
+    constexpr bool kWrites = kFwd || MODE == kModeFusedL2;
     float4* out_px = (kWrites && inside)
                          ? (float4*)(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x
                          : nullptr;
@@ -197,9 +200,18 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         const float ex = vc.eye[0] - p.wx, ey = vc.eye[1] - p.wy, ez = vc.eye[2] - p.wz;
         const float in_ = hl_f_rsq(Nr.x * Nr.x + Nr.y * Nr.y + Nr.z * Nr.z);
         const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
-        p.nx = Nr.x * in_;
-        p.ny = Nr.y * in_;
-        p.nz = Nr.z * in_;
+        if (kShadow) {
+            // shadow_factor's normal offset and N.L take the shader's exact N
+#pragma clang fp contract(off)
+            const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
+            p.nx = n.x;
+            p.ny = n.y;
+            p.nz = n.z;
+        } else {
+            p.nx = Nr.x * in_;
+            p.ny = Nr.y * in_;
+            p.nz = Nr.z * in_;
+        }
         p.vx = ex * iv;
         p.vy = ey * iv;
         p.vz = ez * iv;
@@ -255,7 +267,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     }
 
     // ---- debug views (deferred_lighting.wgsl:1032-1038)
-    if (MODE == kModeForward && a.debug_mode != 0u) {
+    if (kFwd && a.debug_mode != 0u) {
         if (valid) {
             float4 o;
             if (a.debug_mode == 5u)
@@ -332,7 +344,15 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             Term t;
             if (eval_term<false>(L, p, t)) {
                 // radiance x NdL = p1 (atten NdL)
-                const float aw = t.atten * t.ndl;
+                float aw = t.atten * t.ndl;
+                if (kShadow) {
+                    // pbr_direct_light's sf (deferred_lighting.wgsl:1136-1143); the
+                    // word is the light's, so the branch is uniform
+                    const uint32_t w = __float_as_uint(L.p3.w);
+                    if (w != kNoShadowWord)
+                        aw *= shadow_factor(a.shadow, L, w >> 28, w & 0x0FFFFFFFu, p.wx, p.wy, p.wz, p.nx, p.ny,
+                                            p.nz, (float)x + 0.5f, (float)y + 0.5f, vc.eye);
+                }
                 lo[0] += t.B[0] * (L.p1.x * aw);
                 lo[1] += t.B[1] * (L.p1.y * aw);
                 lo[2] += t.B[2] * (L.p1.z * aw);
@@ -341,7 +361,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = lo[c] + amb[c];
-    if (MODE == kModeForward) {
+    if (kFwd) {
         if (valid) hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
         return;
     }
@@ -465,7 +485,7 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
 // Forward launches are held to 96 VGPRs (5 waves per SIMD): the tile's
 // memory round trips need the waves to hide them.
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeShadowed ? 4 : 1)) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
                                                uint32_t* __restrict__ queue_next) {
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
@@ -591,6 +611,7 @@ hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& 
         case kModeCount: return launch_fmt<kModeCount>(s, gbuf_set, a, queue, queue_next);
         case kModeBackward: return launch_fmt<kModeBackward>(s, gbuf_set, a, queue, queue_next);
         case kModeFusedL2: return launch_fmt<kModeFusedL2>(s, gbuf_set, a, queue, queue_next);
+        case kModeShadowed: return launch_fmt<kModeShadowed>(s, gbuf_set, a, queue, queue_next);
         default: return hipErrorInvalidValue;
     }
 }

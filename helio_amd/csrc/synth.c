@@ -296,17 +296,8 @@ static double ray_box(const double o[3], const double d[3], const box* b) {
     return t0 > 0.0 ? t0 : -1.0;
 }
 
-/* Writes tightly packed attachments (pitch = width * texel size).
- * Formats: gbuf_format 0 -> albedo/orm RGBA8, normal/emissive RGBA16F;
- * 1 -> all RGBA16F; 2 -> all RGBA32F.  depth is f32. */
-SYN_API int hl_synth_gbuffer(const hl_synth_config* c, void* albedo, void* normal, void* orm,
-                             void* emissive, float* depth) {
-    if (!c || !albedo || !normal || !orm || !emissive || !depth) return -1;
-    cam64 k;
-    make_cam(c, &k);
-    GpuCameraUniforms cf;
-    hl_synth_camera(c, &cf);
-    const double p22 = cf.proj[10], p32 = cf.proj[14];
+/* The scene's boxes on the ground plane y = 0 (caller frees). */
+static box* make_boxes(const hl_synth_config* c) {
     const uint32_t nb = c->num_boxes;
     box* boxes = (box*)malloc(sizeof(box) * (nb ? nb : 1));
     for (uint32_t b = 0; b < nb; ++b) {
@@ -322,6 +313,22 @@ SYN_API int hl_synth_gbuffer(const hl_synth_config* c, void* albedo, void* norma
         boxes[b].lo[2] = cz - 0.5 * sz;
         boxes[b].hi[2] = cz + 0.5 * sz;
     }
+    return boxes;
+}
+
+/* Writes tightly packed attachments (pitch = width * texel size).
+ * Formats: gbuf_format 0 -> albedo/orm RGBA8, normal/emissive RGBA16F;
+ * 1 -> all RGBA16F; 2 -> all RGBA32F.  depth is f32. */
+SYN_API int hl_synth_gbuffer(const hl_synth_config* c, void* albedo, void* normal, void* orm,
+                             void* emissive, float* depth) {
+    if (!c || !albedo || !normal || !orm || !emissive || !depth) return -1;
+    cam64 k;
+    make_cam(c, &k);
+    GpuCameraUniforms cf;
+    hl_synth_camera(c, &cf);
+    const double p22 = cf.proj[10], p32 = cf.proj[14];
+    const uint32_t nb = c->num_boxes;
+    box* boxes = make_boxes(c);
     const uint32_t W = c->width, H = c->height, fmt = c->gbuf_format;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t yy = 0; yy < (int64_t)H; ++yy) {
@@ -441,6 +448,132 @@ SYN_API int hl_synth_gbuffer(const hl_synth_config* c, void* albedo, void* norma
                 pe[0] = pe[1] = pe[2] = 0.0f;
                 pe[3] = F0[2];
             }
+        }
+    }
+    free(boxes);
+    return 0;
+}
+
+/* ---------------------------------------------------------- shadow maps */
+/* glam look_at_rh from eye along unit f */
+static void look_along(const double eye[3], const double f_in[3], double v[16]) {
+    double f[3] = {f_in[0], f_in[1], f_in[2]};
+    norm3(f);
+    double upw[3] = {0, 1, 0};
+    if (fabs(f[1]) > 0.999) { upw[1] = 0; upw[2] = -1; }
+    double s[3], u[3];
+    cross3(f, upw, s);
+    norm3(s);
+    cross3(s, f, u);
+    const double m[16] = {s[0], u[0], -f[0], 0,  s[1], u[1], -f[1], 0,  s[2], u[2], -f[2], 0,
+                          -(s[0] * eye[0] + s[1] * eye[1] + s[2] * eye[2]),
+                          -(u[0] * eye[0] + u[1] * eye[1] + u[2] * eye[2]),
+                          (f[0] * eye[0] + f[1] * eye[1] + f[2] * eye[2]), 1};
+    memcpy(v, m, sizeof m);
+}
+
+/* glam perspective_rh, depth [0, 1] */
+static void persp(double fov, double n, double fa, double p[16]) {
+    const double h = cos(0.5 * fov) / sin(0.5 * fov), r = fa / (n - fa);
+    const double m[16] = {h, 0, 0, 0, 0, h, 0, 0, 0, 0, r, -1, 0, 0, r * n, 0};
+    memcpy(p, m, sizeof m);
+}
+
+/* glam orthographic_rh, depth [0, 1] */
+static void ortho(double hw, double n, double fa, double p[16]) {
+    const double m[16] = {1.0 / hw, 0, 0, 0, 0, 1.0 / hw, 0, 0, 0, 0, 1.0 / (n - fa), 0, 0, 0, n / (n - fa), 1};
+    memcpy(p, m, sizeof m);
+}
+
+/* Depth of the first surface of the scene along the texel ray of a layer,
+ * as NDC z of the layer's matrix; 1 when nothing is hit. dynamic selects the
+ * boxes of the dynamic atlas (odd) or the static one (even + ground). */
+static float shadow_texel(const double vp[16], const double ivp[16], uint32_t size, uint32_t x, uint32_t y,
+                          const box* boxes, uint32_t nb, int dynamic) {
+    const double ndx = ((x + 0.5) / size) * 2.0 - 1.0, ndy = 1.0 - ((y + 0.5) / size) * 2.0;
+    double a[3], b[3];
+    for (int k = 0; k < 2; ++k) {
+        const double z = k == 0 ? 0.0 : 1.0;
+        double h[4];
+        for (int r = 0; r < 4; ++r) h[r] = ivp[r] * ndx + ivp[4 + r] * ndy + ivp[8 + r] * z + ivp[12 + r];
+        double* o = k == 0 ? a : b;
+        for (int r = 0; r < 3; ++r) o[r] = h[r] / h[3];
+    }
+    double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int r = 0; r < 3; ++r) d[r] /= len;
+    double t = -1.0;
+    if (!dynamic && fabs(d[1]) > 1e-12) {
+        const double tg = -a[1] / d[1];
+        if (tg > 0.0) t = tg;
+    }
+    for (uint32_t i = (uint32_t)dynamic; i < nb; i += 2) {
+        const double tb = ray_box(a, d, &boxes[i]);
+        if (tb > 0.0 && (t < 0.0 || tb < t)) t = tb;
+    }
+    if (t < 0.0 || t > len) return 1.0f;
+    const double p[4] = {a[0] + t * d[0], a[1] + t * d[1], a[2] + t * d[2], 1.0};
+    double cz = 0.0, cw = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        cz += vp[c * 4 + 2] * p[c];
+        cw += vp[c * 4 + 3] * p[c];
+    }
+    const double z = cz / cw;
+    return z < 0.0 ? 0.0f : (z >= 1.0 ? 1.0f : (float)z);
+}
+
+/* Shadow atlases (dynamic: odd boxes; static: even boxes and the ground) and
+ * one light matrix per layer for the lights with a shadow_index: point lights
+ * 6 cube faces (+X, -X, +Y, -Y, +Z, -Z; 90 degrees, near 0.05, far = range),
+ * spot lights one perspective layer (the outer cone plus 5 degrees), the
+ * directional light 4 orthographic cascades centred on the camera eye with
+ * half-widths csm[c].  Layers are size x size, layer-major. */
+SYN_API int hl_synth_shadow_maps(const hl_synth_config* c, const GpuLight* lights, uint32_t num_lights,
+                                 const float csm[4], uint32_t size, uint32_t num_layers, float* atlas,
+                                 float* static_atlas, float* matrices) {
+    if (!c || !atlas || !static_atlas || !matrices || size == 0) return -1;
+    box* boxes = make_boxes(c);
+    const uint32_t nb = c->num_boxes;
+    const size_t texels = (size_t)size * size;
+    for (size_t i = 0; i < texels * num_layers; ++i) atlas[i] = static_atlas[i] = 1.0f;
+    memset(matrices, 0, sizeof(float) * 16 * num_layers);
+    static const double faces[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (uint32_t li = 0; li < num_lights; ++li) {
+        const GpuLight* l = &lights[li];
+        if (l->shadow_index == HL_NO_SHADOW) continue;
+        const double pos[3] = {l->position_range[0], l->position_range[1], l->position_range[2]};
+        const double dir[3] = {l->direction_outer[0], l->direction_outer[1], l->direction_outer[2]};
+        uint32_t nl = l->light_type == HL_LIGHT_POINT ? 6u : (l->light_type == HL_LIGHT_DIRECTIONAL ? 4u : 1u);
+        for (uint32_t k = 0; k < nl; ++k) {
+            const uint32_t layer = l->shadow_index + k;
+            if (layer >= num_layers) break;
+            double v[16], p[16], vp[16], ivp[16];
+            if (l->light_type == HL_LIGHT_POINT) {
+                look_along(pos, faces[k], v);
+                persp(3.141592653589793 / 2.0, 0.05, l->position_range[3], p);
+            } else if (l->light_type == HL_LIGHT_DIRECTIONAL) {
+                const double hw = csm[k];
+                const double eye[3] = {c->eye[0] - dir[0] * 500.0, c->eye[1] - dir[1] * 500.0,
+                                       c->eye[2] - dir[2] * 500.0};
+                look_along(eye, dir, v);
+                ortho(hw, 0.0, 1000.0, p);
+            } else {
+                double fov = 2.0 * acos(l->direction_outer[3]) + 10.0 * 3.141592653589793 / 180.0;
+                if (fov > 3.0) fov = 3.0;
+                look_along(pos, dir, v);
+                persp(fov, 0.05, l->position_range[3], p);
+            }
+            mul44(p, v, vp);
+            inv44(vp, ivp);
+            for (int e = 0; e < 16; ++e) matrices[16 * (size_t)layer + e] = (float)vp[e];
+            float* da = atlas + texels * layer;
+            float* sa = static_atlas + texels * layer;
+#pragma omp parallel for schedule(dynamic, 8)
+            for (int64_t yy = 0; yy < (int64_t)size; ++yy)
+                for (uint32_t x = 0; x < size; ++x) {
+                    da[(size_t)yy * size + x] = shadow_texel(vp, ivp, size, x, (uint32_t)yy, boxes, nb, 1);
+                    sa[(size_t)yy * size + x] = shadow_texel(vp, ivp, size, x, (uint32_t)yy, boxes, nb, 0);
+                }
         }
     }
     free(boxes);

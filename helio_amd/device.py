@@ -163,6 +163,19 @@ def pack_gbuffer(ctx: Context, surface: abi.hl_surface, gbuffer: abi.hl_gbuffer)
     ctx.lib.call("hl_gbuffer_pack", ctx.h, C.byref(surface), C.byref(gbuffer), ctx=ctx.h)
 
 
+class DeviceShadows:
+    """Shadow atlases and light matrices resident in HBM (hl_shadow_maps)."""
+
+    def __init__(self, ctx: Context, host):
+        self.atlas = ctx.upload(host.atlas)
+        self.static_atlas = ctx.upload(host.static_atlas)
+        self.matrices = ctx.upload(host.matrices)
+        d = abi.hl_shadow_maps()
+        d.atlas_d, d.static_atlas_d, d.matrices_d = self.atlas.ptr, self.static_atlas.ptr, self.matrices.ptr
+        d.num_layers, d.atlas_size, d.config = host.layers, host.size, host.config
+        self.desc = d
+
+
 class TileGrid:
     """tile_light_lists / tile_light_counts (crates/libhelio/src/frame.rs:230-234)."""
 
@@ -279,10 +292,16 @@ class DeferredLightPass:
         g.num_tiles_x = (self.width + 15) // 16
         self.globals = g
 
-    def execute(self, scene: DeviceScene, grid: abi.hl_light_grid, flags: int = 0) -> None:
-        self.ctx.lib.call("hl_deferred_forward", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
-                          C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
-                          self.out.ptr, self.width * 16, flags)
+    def execute(self, scene: DeviceScene, grid: abi.hl_light_grid, flags: int = 0,
+                shadows: "DeviceShadows | None" = None) -> None:
+        if shadows is None:
+            self.ctx.lib.call("hl_deferred_forward", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                              C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
+                              self.out.ptr, self.width * 16, flags)
+        else:
+            self.ctx.lib.call("hl_deferred_forward_shadowed", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                              C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
+                              C.byref(shadows.desc), self.out.ptr, self.width * 16, flags)
 
     def forward_backward_l2(self, scene: DeviceScene, grid: abi.hl_light_grid, target: DeviceArray,
                             grads: "AdjointBuffers") -> None:

@@ -148,6 +148,8 @@ def synth_lib() -> C.CDLL:
         lib.hl_synth_lights.argtypes = [C.POINTER(hl_synth_config), C.c_void_p]
         lib.hl_synth_gbuffer.argtypes = [C.POINTER(hl_synth_config)] + [C.c_void_p] * 5
         lib.hl_synth_subset.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+        lib.hl_synth_shadow_maps.argtypes = [C.POINTER(hl_synth_config), C.c_void_p, C.c_uint32,
+                                             C.POINTER(C.c_float), C.c_uint32, C.c_uint32] + [C.c_void_p] * 3
         lib.hl_synth_abi_size.restype = C.c_uint32
         assert lib.hl_synth_abi_size() == C.sizeof(hl_synth_config)
         _LIB = lib
@@ -419,3 +421,98 @@ def forward_globals(scene: HostScene) -> abi.ForwardLitGlobals:
     g.screen_width = float(scene.width)
     g.screen_height = float(scene.height)
     return g
+
+
+# ----------------------------------------------------------------- shadows
+# ShadowConfig::from_quality (crates/libhelio/src/shadow.rs:71-196): per cascade
+# (split_distance, depth_bias, filter_radius, pcss_light_size), then enable_pcss,
+# pcss_blocker_samples, pcss_filter_samples, pcf_sample_count.
+_SHADOW_PRESETS = {
+    "low": ([(16.0, 0.00010, 1.0, 0.0), (80.0, 0.00015, 1.5, 0.0), (300.0, 0.00020, 2.0, 0.0),
+             (1400.0, 0.00030, 3.0, 0.0)], 0, 4, 4, 8),
+    "medium": ([(16.0, 0.00010, 1.5, 0.0), (80.0, 0.00015, 2.0, 0.0), (300.0, 0.00020, 2.5, 0.0),
+                (1400.0, 0.00030, 3.5, 0.0)], 0, 8, 8, 12),
+    "high": ([(16.0, 0.00008, 2.0, 2.0), (80.0, 0.00012, 2.5, 4.0), (300.0, 0.00018, 3.0, 8.0),
+              (1400.0, 0.00025, 4.0, 16.0)], 1, 8, 16, 12),
+    "ultra": ([(16.0, 0.00008, 2.5, 2.0), (80.0, 0.00012, 3.0, 4.0), (300.0, 0.00018, 3.5, 8.0),
+               (1400.0, 0.00025, 4.5, 16.0)], 1, 16, 32, 16),
+}
+
+
+def shadow_config(quality: str = "medium") -> abi.ShadowConfig:
+    casc, pcss, nb, nf, npcf = _SHADOW_PRESETS[quality]
+    c = abi.ShadowConfig()
+    for i, (split, bias, radius, size) in enumerate(casc):
+        c.cascades[i].split_distance, c.cascades[i].depth_bias = split, bias
+        c.cascades[i].filter_radius, c.cascades[i].pcss_light_size = radius, size
+    c.enable_pcss, c.pcss_blocker_samples, c.pcss_filter_samples, c.pcf_sample_count = pcss, nb, nf, npcf
+    return c
+
+
+@dataclass
+class HostShadows:
+    """Shadow atlases (dynamic, static: [layers, size, size] float32), one light
+    matrix per layer ([layers, 16], column-major) and the ShadowConfig."""
+    atlas: np.ndarray
+    static_atlas: np.ndarray
+    matrices: np.ndarray
+    config: abi.ShadowConfig
+
+    @property
+    def size(self) -> int:
+        return self.atlas.shape[1]
+
+    @property
+    def layers(self) -> int:
+        return self.atlas.shape[0]
+
+    def host(self) -> abi.hl_shadow_maps:
+        """hl_shadow_maps whose pointers are the host arrays (for the CPU oracle)."""
+        h = abi.hl_shadow_maps()
+        h.atlas_d, h.static_atlas_d = self.atlas.ctypes.data, self.static_atlas.ctypes.data
+        h.matrices_d = self.matrices.ctypes.data
+        h.num_layers, h.atlas_size, h.config = self.layers, self.size, self.config
+        return h
+
+
+def add_shadow_casters(scene: HostScene, n_point: int = 8, n_spot: int = 8, directional: bool = True,
+                       size: int = 256, quality: str = "medium", seed: int = 11) -> HostShadows:
+    """Turns the first lights of the scene into shadow casters -- light 0 a
+    directional sun (4 cascades), then n_point point lights (6 faces each) and
+    n_spot spot lights (one layer each), all below MAX_SHADOW_LIGHTS = 42 --
+    and renders their shadow maps from the scene's boxes and ground
+    (libhelio_synth: one ray per texel)."""
+    lights = scene.lights
+    rng = np.random.default_rng(seed)
+    first = 1 if directional else 0
+    assert first + n_point + n_spot <= min(42, len(lights))
+    layer = 0
+    if directional:
+        d = np.array([0.35, -1.0, 0.25]); d /= np.linalg.norm(d)
+        lights[0]["light_type"] = abi.HL_LIGHT_DIRECTIONAL
+        lights[0]["direction_outer"] = (*d, 0.0)
+        lights[0]["color_intensity"] = (1.0, 0.95, 0.85, 1.5)
+        lights[0]["shadow_index"] = 0
+        layer = 4
+    for i in range(first, first + n_point):
+        lights[i]["light_type"] = abi.HL_LIGHT_POINT
+        lights[i]["direction_outer"] = (0.0, -1.0, 0.0, 0.0)
+        lights[i]["shadow_index"] = layer
+        layer += 6
+    for i in range(first + n_point, first + n_point + n_spot):
+        d = rng.normal(size=3); d[1] = -abs(d[1]) - 0.5; d /= np.linalg.norm(d)
+        outer = np.radians(rng.uniform(25.0, 45.0))
+        lights[i]["light_type"] = abi.HL_LIGHT_SPOT
+        lights[i]["direction_outer"] = (*d, np.cos(outer))
+        lights[i]["inner_angle"] = np.cos(outer * 0.8)
+        lights[i]["shadow_index"] = layer
+        layer += 1
+    atlas = np.zeros((layer, size, size), np.float32)
+    static = np.zeros_like(atlas)
+    mats = np.zeros((layer, 16), np.float32)
+    csm = (C.c_float * 4)(*scene.globals.csm_splits)
+    cfg = scene.spec.config()
+    if synth_lib().hl_synth_shadow_maps(C.byref(cfg), lights.ctypes.data, len(lights), csm, size, layer,
+                                        atlas.ctypes.data, static.ctypes.data, mats.ctypes.data) != 0:
+        raise ValueError("bad shadow map request")
+    return HostShadows(atlas, static, mats, shadow_config(quality))

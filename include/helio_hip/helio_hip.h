@@ -271,6 +271,41 @@ HL_API hl_status hl_deferred_forward(hl_context* ctx, const GpuCameraUniforms* c
                                      const hl_light_grid* grid, float* out_d,
                                      uint32_t out_pitch_bytes, uint32_t flags);
 
+/* Shadow maps shadow_factor samples (deferred_lighting.wgsl:234-565): the
+ * dynamic and the static shadow atlas (Depth32Float texture arrays bound at
+ * :160-161), num_layers square layers of atlas_size texels (ATLAS_SIZE = 1024
+ * at :236), each layer-major and tightly packed (texel (x, y) of layer l at
+ * [(l * atlas_size + y) * atlas_size + x], row 0 = v 0), and one light
+ * matrix per layer (column-major mat4, world -> light clip; shadow_matrices
+ * at :167).  config is the ShadowConfig uniform (:139). */
+typedef struct hl_shadow_maps {
+    const float* atlas_d;
+    const float* static_atlas_d;
+    const float* matrices_d;
+    uint32_t num_layers;
+    uint32_t atlas_size;
+    ShadowConfig config;
+} hl_shadow_maps;
+
+/* hl_deferred_forward with shadows: every term of a light i < 42
+ * (MAX_SHADOW_LIGHTS, :20) whose shadow_index != HL_NO_SHADOW is multiplied by
+ * shadow_factor (:452-565): normal-offset bias, the light's layer (point: base
+ * + cube face of :253-263; spot: base; directional: base + cascade, blending
+ * across the 10 % zones around globals->csm_splits with the camera distance),
+ * then PCF (Vogel disk of pcf_sample_count taps scaled per cascade, :273-330)
+ * or PCSS (blocker search + variable-radius PCF, :335-450) over both atlases,
+ * lit = min(dynamic, static) per tap.  The comparison sampler (LessEqual,
+ * bilinear, clamp to edge; helio-pass-shadow/src/lib.rs:365-375) is evaluated
+ * with FP32 bilinear weights; sin and cos of the Vogel angles use a fixed
+ * Cody-Waite reduction and Taylor polynomials, so the factor is reproducible
+ * bit for bit.  A layer index >= num_layers samples nothing (factor 1).
+ * shadows == NULL is hl_deferred_forward.  Same other arguments and results. */
+HL_API hl_status hl_deferred_forward_shadowed(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                              const DeferredGlobals* globals, const hl_gbuffer* gbuffer,
+                                              const GpuLight* lights_d, uint32_t num_lights,
+                                              const hl_light_grid* grid, const hl_shadow_maps* shadows,
+                                              float* out_d, uint32_t out_pitch_bytes, uint32_t flags);
+
 /* Counts Q, the (pixel, light) pairs the forward pass evaluates past the range
  * test (type != 0 implies dist <= range) -- the "pixel-light evals" of the
  * throughput metric.  Same inputs as hl_deferred_forward; writes *pairs (host). */

@@ -146,6 +146,22 @@ typedef struct hl_transform {
     float scale[3];                /* 24 */
 } hl_transform;
 
+/* Per-cascade and global shadow settings (crates/libhelio/src/shadow.rs:32-63). */
+typedef struct CascadeConfig {
+    float split_distance;          /*  0 */
+    float depth_bias;              /*  4 */
+    float filter_radius;           /*  8: PCF radius in atlas texels */
+    float pcss_light_size;         /* 12: 0 disables PCSS for the cascade */
+} CascadeConfig;
+
+typedef struct ShadowConfig {
+    CascadeConfig cascades[4];     /*  0 */
+    uint32_t enable_pcss;          /* 64 */
+    uint32_t pcss_blocker_samples; /* 68 */
+    uint32_t pcss_filter_samples;  /* 72 */
+    uint32_t pcf_sample_count;     /* 76 */
+} ShadowConfig;
+
 #ifdef __cplusplus
 #define HL_STATIC_ASSERT static_assert
 #else
@@ -192,6 +208,9 @@ HL_STATIC_ASSERT(offsetof(GpuMaterial, class_params) == 80, "material.class_para
 HL_STATIC_ASSERT(sizeof(ForwardLitGlobals) == 48, "ForwardLitGlobals must be 48 bytes");
 HL_STATIC_ASSERT(offsetof(ForwardLitGlobals, num_tiles_x) == 32, "forward globals.num_tiles_x");
 HL_STATIC_ASSERT(sizeof(hl_transform) == 36, "Transform must be 36 bytes (three packed [f32; 3])");
+HL_STATIC_ASSERT(sizeof(CascadeConfig) == 16, "CascadeConfig must be 16 bytes");
+HL_STATIC_ASSERT(sizeof(ShadowConfig) == 80, "ShadowConfig must be 80 bytes (shadow.rs:49-63)");
+HL_STATIC_ASSERT(offsetof(ShadowConfig, pcf_sample_count) == 76, "ShadowConfig.pcf_sample_count");
 HL_STATIC_ASSERT(offsetof(hl_transform, rotation) == 12, "transform.rotation");
 
 #ifdef __cplusplus

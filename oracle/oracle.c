@@ -407,9 +407,10 @@ uint32_t or_light_cull_clusters(const GpuCameraUniforms* cam, const hl_cluster_p
 
 /* deferred_lighting.wgsl:699-820 pbr_direct_light with sf = 1 (no shadow
  * caster bound), no IES/gobo (indices < 0), isotropic, no subsurface. */
-void or_pbr_direct_light(const GpuLight* light, const float wp[3], const float N[3],
-                         const float V[3], const float F0[3], const float albedo[3],
-                         float roughness, float metallic, float out[3]) {
+/* pbr_direct_light with its shadow factor sf (1 for the unshadowed call). */
+static void or_pbr_direct_light_sf(const GpuLight* light, const float wp[3], const float N[3],
+                                   const float V[3], const float F0[3], const float albedo[3],
+                                   float roughness, float metallic, float sf, float out[3]) {
     float L[3], radiance[3];
     out[0] = out[1] = out[2] = 0.0f;
     if (light->light_type == 0u) {
@@ -467,8 +468,14 @@ void or_pbr_direct_light(const GpuLight* light, const float wp[3], const float N
     for (int k = 0; k < 3; ++k) {
         float spec = D * G * F[k] / sden;
         float diff = kD[k] * albedo[k] / OR_PI;
-        out[k] = (diff + spec) * radiance[k] * NdL * 1.0f;
+        out[k] = (diff + spec) * radiance[k] * NdL * sf;
     }
+}
+
+void or_pbr_direct_light(const GpuLight* light, const float wp[3], const float N[3],
+                         const float V[3], const float F0[3], const float albedo[3],
+                         float roughness, float metallic, float out[3]) {
+    or_pbr_direct_light_sf(light, wp, N, V, F0, albedo, roughness, metallic, 1.0f, out);
 }
 
 /* Pixel -> cell list. */
@@ -545,9 +552,10 @@ static int or_debug_supported(uint32_t m) {
     return m == 0 || m == 1 || m == 2 || m == 4 || m == 5 || m == 20 || m == 21;
 }
 
-uint64_t or_shade(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
-                  const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
-                  float* out, uint32_t out_pitch, uint32_t flags, uint32_t row0, uint32_t row1) {
+uint64_t or_shade_shadowed(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
+                           const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
+                           const hl_shadow_maps* sh, float* out, uint32_t out_pitch, uint32_t flags,
+                           uint32_t row0, uint32_t row1) {
     (void)num_lights;
     if (!or_debug_supported(g->debug_mode)) return 0;
     if (row1 == 0 || row1 > gb->height) row1 = gb->height;
@@ -594,8 +602,13 @@ uint64_t or_shade(const GpuCameraUniforms* cam, const DeferredGlobals* g, const 
                     if (or_length(tl) > l->position_range[3]) continue;
                 }
                 ++pairs;
+                if (!o) continue;
+                /* :1136-1143 shadow_factor for every light in range */
+                const float sf = sh ? or_shadow_factor(sh, lights, list[i], cam->position_near, g->csm_splits, p.wp,
+                                                       p.N, (float)x + 0.5f, (float)y + 0.5f)
+                                    : 1.0f;
                 float c[3];
-                or_pbr_direct_light(l, p.wp, p.N, p.V, p.F0, p.albedo, p.rough, p.metal, c);
+                or_pbr_direct_light_sf(l, p.wp, p.N, p.V, p.F0, p.albedo, p.rough, p.metal, sf, c);
                 Lo[0] = Lo[0] + c[0];
                 Lo[1] = Lo[1] + c[1];
                 Lo[2] = Lo[2] + c[2];
@@ -616,6 +629,12 @@ uint64_t or_shade(const GpuCameraUniforms* cam, const DeferredGlobals* g, const 
         }
     }
     return pairs;
+}
+
+uint64_t or_shade(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
+                  const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
+                  float* out, uint32_t out_pitch, uint32_t flags, uint32_t row0, uint32_t row1) {
+    return or_shade_shadowed(cam, g, gb, lights, num_lights, grid, NULL, out, out_pitch, flags, row0, row1);
 }
 
 uint64_t or_count_pairs(const GpuCameraUniforms* cam, const DeferredGlobals* g,

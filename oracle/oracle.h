@@ -156,6 +156,25 @@ uint64_t or_forward_lit(const GpuCameraUniforms* cam, const ForwardLitGlobals* g
                         const hl_transform* transforms, uint32_t num_transforms, const hl_light_grid* grid,
                         float* out);
 void or_light_direction_from_rotation(const float rot_deg[3], float out[3]);
+
+/* Shadows (shadow.c, deferred_lighting.wgsl:234-565): or_shade with the terms
+ * of shadow-casting lights multiplied by shadow_factor (host pointers in sh),
+ * and the pieces of shadow_factor. */
+uint64_t or_shade_shadowed(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
+                           const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
+                           const hl_shadow_maps* sh, float* out, uint32_t out_pitch, uint32_t flags,
+                           uint32_t row0, uint32_t row1);
+float or_shadow_factor(const hl_shadow_maps* sh, const GpuLight* lights, uint32_t light_idx, const float cam_pos[3],
+                       const float csm_splits[4], const float wp[3], const float N[3], float fx, float fy);
+float or_sample_cascade_shadow(const hl_shadow_maps* sh, uint32_t layer, uint32_t cascade, float scale,
+                               const float wp[3], float fx, float fy);
+float or_sample_cascade_shadow_pcss(const hl_shadow_maps* sh, uint32_t layer, uint32_t cascade, const float wp[3],
+                                    float fx, float fy);
+float or_sample_compare(const float* atlas, uint32_t size, uint32_t layer, float u, float v, float ref);
+void or_sincos(float x, float* s, float* c);
+float or_hash22(float px, float py);
+void or_vogel(uint32_t i, uint32_t n, float theta, float out[2]);
+uint32_t or_point_light_face(const float d[3]);
 void or_forward_pbr_direct_light(const GpuLight* light, const float light_pos[3], const float light_dir[3],
                                  const float wp[3], const float N[3], const float V[3], const float F0[3],
                                  const float albedo[3], float roughness, float metallic, float out[3]);

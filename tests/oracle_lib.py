@@ -35,6 +35,19 @@ _SIGS = {
     "or_light_direction_from_rotation": (None, [P(C.c_float), P(C.c_float)]),
     "or_forward_pbr_direct_light": (None, [vp, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
                                            P(C.c_float), P(C.c_float), C.c_float, C.c_float, P(C.c_float)]),
+    "or_shade_shadowed": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
+                                       P(abi.hl_light_grid), P(abi.hl_shadow_maps), vp, u32, u32, u32, u32]),
+    "or_shadow_factor": (C.c_float, [P(abi.hl_shadow_maps), vp, u32, P(C.c_float), P(C.c_float), P(C.c_float),
+                                     P(C.c_float), C.c_float, C.c_float]),
+    "or_sample_cascade_shadow": (C.c_float, [P(abi.hl_shadow_maps), u32, u32, C.c_float, P(C.c_float), C.c_float,
+                                             C.c_float]),
+    "or_sample_cascade_shadow_pcss": (C.c_float, [P(abi.hl_shadow_maps), u32, u32, P(C.c_float), C.c_float,
+                                                  C.c_float]),
+    "or_sample_compare": (C.c_float, [vp, u32, u32, C.c_float, C.c_float, C.c_float]),
+    "or_sincos": (None, [C.c_float, P(C.c_float), P(C.c_float)]),
+    "or_hash22": (C.c_float, [C.c_float, C.c_float]),
+    "or_vogel": (None, [u32, u32, C.c_float, P(C.c_float)]),
+    "or_point_light_face": (u32, [P(C.c_float)]),
     "or_shade": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
                               P(abi.hl_light_grid), vp, u32, u32, u32, u32]),
     "or_count_pairs": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
@@ -117,17 +130,20 @@ def cluster_grid_host(params, off, cnt, pool):
     return g
 
 
-def shade(scene, grid, globals_=None, flags=0, rows=None, out=None):
-    """FP32 restatement of fs_main; returns (rgba [H,W,4], pairs)."""
+def shade(scene, grid, globals_=None, flags=0, rows=None, out=None, shadows=None):
+    """FP32 restatement of fs_main; returns (rgba [H,W,4], pairs).  shadows:
+    a synth.HostShadows to multiply the terms of shadow-casting lights by
+    shadow_factor."""
     W, H = scene.width, scene.height
     if out is None:
         out = np.zeros((H, W, 4), dtype=np.float32)
     g = globals_ if globals_ is not None else scene.globals
     gb = scene.gbuffer_host()
     r0, r1 = rows if rows else (0, 0)
-    pairs = oracle().or_shade(C.byref(scene.camera), C.byref(g), C.byref(gb),
-                              scene.lights.ctypes.data if len(scene.lights) else None, len(scene.lights),
-                              C.byref(grid), out.ctypes.data, W * 16, flags, r0, r1)
+    sh = C.byref(shadows.host()) if shadows is not None else None
+    pairs = oracle().or_shade_shadowed(C.byref(scene.camera), C.byref(g), C.byref(gb),
+                                       scene.lights.ctypes.data if len(scene.lights) else None, len(scene.lights),
+                                       C.byref(grid), sh, out.ctypes.data, W * 16, flags, r0, r1)
     return out, pairs
 
 

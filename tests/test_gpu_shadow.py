@@ -1,0 +1,90 @@
+"""Shadowed deferred shading on the GPU (hl_deferred_forward_shadowed) against
+the FP32 restatement with shadow_factor (oracle/shadow.c) on the same lists:
+radiance within 1e-4 relative (+1e-6 absolute).  The shadow factors are
+reproducible bit for bit (own sin/cos, FP32 sampler weights), so a tap that
+lands on another texel or a blocker counted differently would show as a
+large error, not a small one."""
+import numpy as np
+import pytest
+
+from helio_amd import abi, synth
+from helio_amd.device import DeferredLightPass, DeviceShadows, LightCullPass, upload_scene
+import oracle_lib as O
+from test_gpu_parity import check_radiance
+
+pytestmark = pytest.mark.gpu
+
+
+def run(ctx, hlib, name, mode, quality, size, flags=0, **over):
+    sc = synth.make_scene(name, **over)
+    sh = synth.add_shadow_casters(sc, n_point=8, n_spot=8, size=size, quality=quality)
+    ds = upload_scene(ctx, sc)
+    dsh = DeviceShadows(ctx, sh)
+    if mode == "tiles":
+        cull = LightCullPass(ctx, sc.width, sc.height)
+    else:
+        params = synth.cluster_params(hlib, sc)
+        cull = LightCullPass(ctx, sc.width, sc.height, clusters=params, pool_cap=1 << 22)
+    cull.prepare(ds)
+    cull.execute(ds)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    shade.out.fill_u32(0)
+    shade.execute(ds, cull.publish(), flags=flags, shadows=dsh)
+    out = shade.output()
+    if mode == "tiles":
+        lists = cull.tiles.lists.download(np.uint32)
+        counts = cull.tiles.counts.download(np.uint32)
+        grid = O.tile_grid_host(sc, lists, counts)
+        grid._arrays = (lists, counts)
+    else:
+        off, cnt, pool = cull.cgrid.download()
+        grid = O.cluster_grid_host(params, off, cnt, pool)
+    return sc, sh, out, grid, (ds, cull, shade, dsh)
+
+
+@pytest.mark.parametrize("name,mode,quality,size", [("c1_720p_64", "tiles", "medium", 256),
+                                                    ("c1_720p_64", "clusters", "high", 256),
+                                                    ("c2_1080p_1k", "tiles", "ultra", 512),
+                                                    ("c2_1080p_1k", "clusters", "low", 1024)])
+def test_shadowed_shading_matches_oracle(ctx, hlib, name, mode, quality, size):
+    print(f"[shadowed] {name} {mode} {quality} atlas {size}", flush=True)
+    sc, sh, out, grid, _ = run(ctx, hlib, name, mode, quality, size)
+    ref, _ = O.shade(sc, grid, shadows=sh)
+    unshadowed, _ = O.shade(sc, grid)
+    sky = sc.depth >= 1.0
+    r = check_radiance(out, ref, ~sky)
+    dark = np.mean(np.any(ref[~sky][:, :3] < unshadowed[~sky][:, :3] - 1e-6, axis=1))
+    print(f"  max rel err {r:.2e}, shadowed pixels {dark:.2f}")
+    assert dark > 0.05
+
+
+def test_shadowed_with_cull_records(ctx, hlib):
+    """HL_SHADE_LIGHTS_FROM_CULL: the cull's records carry the same shadow
+    words, so the image is the same bits."""
+    print("[shadowed] lights from cull", flush=True)
+    sc, sh, out, grid, (ds, cull, shade, dsh) = run(ctx, hlib, "c1_720p_64", "clusters", "medium", 256)
+    cull.execute(ds)
+    shade.out.fill_u32(0)
+    shade.execute(ds, cull.publish(), flags=abi.HL_SHADE_LIGHTS_FROM_CULL, shadows=dsh)
+    assert np.array_equal(shade.output(), out)
+
+
+def test_shadow_layer_past_the_atlas_is_lit(ctx, hlib):
+    print("[shadowed] shadow layers past the atlas", flush=True)
+    sc = synth.make_scene("c1_720p_64", width=320, height=180)
+    sh = synth.add_shadow_casters(sc, n_point=4, n_spot=4, size=64)
+    sc.lights["shadow_index"][1:3] = [sh.layers + 2, 0x7FFFFFFF]
+    ds = upload_scene(ctx, sc)
+    dsh = DeviceShadows(ctx, sh)
+    cull = LightCullPass(ctx, sc.width, sc.height)
+    cull.prepare(ds)
+    cull.execute(ds)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    shade.execute(ds, cull.publish(), shadows=dsh)
+    lists = cull.tiles.lists.download(np.uint32)
+    counts = cull.tiles.counts.download(np.uint32)
+    grid = O.tile_grid_host(sc, lists, counts)
+    ref, _ = O.shade(sc, grid, shadows=sh)
+    check_radiance(shade.output(), ref, sc.depth < 1.0)
