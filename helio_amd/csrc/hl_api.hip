@@ -330,7 +330,7 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->markq, &c->shadeq,
-                      &c->words, &c->accum})
+                      &c->words, &c->accum, &c->vogel})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return HL_OK;
@@ -616,8 +616,8 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
     if (flags & ~(uint32_t)(HL_SHADE_CLEAR_SKY | HL_SHADE_LIGHTS_FROM_CULL))
         return fail(c, HL_EINVAL, "unknown shading flags 0x%x", flags);
     if (sh) {
-        if (sh->atlas_size == 0 || sh->atlas_size > 16384)
-            return fail(c, HL_EINVAL, "shadow atlas_size %u out of range (1..16384)", sh->atlas_size);
+        if (sh->atlas_size < 2 || sh->atlas_size > 16384)
+            return fail(c, HL_EINVAL, "shadow atlas_size %u out of range (2..16384)", sh->atlas_size);
         if (sh->num_layers > 65536) return fail(c, HL_EINVAL, "%u shadow layers (at most 65536)", sh->num_layers);
         if (sh->num_layers && (!sh->atlas_d || !sh->static_atlas_d || !sh->matrices_d))
             return fail(c, HL_EINVAL, "shadow atlases or matrices are null");
@@ -653,6 +653,12 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
             v.light_size[k] = sh->config.cascades[k].pcss_light_size;
             v.csm[k] = g->csm_splits[k];
         }
+        if ((s = ensure(c, c->vogel, sizeof(float) * kVogelTableFloats)) != HL_OK) return s;
+        if (!c->vogel_ready) {
+            HL_HIP(c, launch_vogel_table(c->stream, (float*)c->vogel.p));
+            c->vogel_ready = true;
+        }
+        v.vogel = (const float*)c->vogel.p;
         mode = kModeShadowed;
     }
     return shade_launch(c, set, mode, a);

@@ -72,6 +72,7 @@ struct ShadowView {
     const float* atlas;
     const float* static_atlas;
     const float* matrices;  // 16 per layer
+    const float* vogel;     // Vogel disk tables (k_vogel_table)
     uint32_t layers, size;
     uint32_t pcf_count, enable_pcss, blocker_samples, filter_samples;
     float filter_radius[4], light_size[4];

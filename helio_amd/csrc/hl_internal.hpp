@@ -33,7 +33,8 @@ struct hl_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum;
+    DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum, vogel;
+    bool vogel_ready = false;  // vogel holds the Vogel disk tables
     uint32_t shade_parity = 0;  // which half of shadeq the next shading launch claims from
     // the lights the records in `hot` were last built from (stream order)
     const GpuLight* hot_src = nullptr;
@@ -228,6 +229,9 @@ inline size_t grad_accum_doubles(uint32_t num_lights) {
     return (size_t)kGradSlots * 4 * (num_lights ? num_lights : 1);
 }
 hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n);
+// The Vogel disk tables of the shadowed shading (kVogelFloats floats).
+constexpr size_t kVogelTableFloats = 2 * 256 + (size_t)257 * 256;
+hipError_t launch_vogel_table(hipStream_t s, float* t);
 hipError_t launch_light_grads_to_f32(hipStream_t s, const double* acc, uint32_t n,
                                      const uint32_t* subset, float* d_pos, float* d_int);
 

@@ -643,6 +643,24 @@ __global__ void k_light_grads_to_f32(const double* __restrict__ acc, uint32_t n,
     d_int[o] = (float)g[3];
 }
 
+// The Vogel tables of kVogelFloats floats: block n, thread i.
+__global__ void k_vogel_table(float* __restrict__ t) {
+#pragma clang fp contract(off)
+    const uint32_t n = blockIdx.x, i = threadIdx.x;
+    if (n == 0u) {
+        float sa, ca;
+        sh_sincos((float)i * 2.39996323f, sa, ca);
+        t[i] = ca;
+        t[kVogelMax + i] = sa;
+    }
+    if (i < n) t[kVogelRadii + (size_t)n * kVogelMax + i] = sqrtf((float)i + 0.5f) / sqrtf((float)n);
+}
+
+hipError_t launch_vogel_table(hipStream_t s, float* t) {
+    k_vogel_table<<<kVogelMax + 1, kVogelMax, 0, s>>>(t);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n) {
     if (n == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;

@@ -10,9 +10,11 @@
 //  - the comparison sampler (LessEqual, linear, clamp to edge) is evaluated
 //    as t = uv * S - 0.5, taps floor(t) and floor(t) + 1 clamped to the layer,
 //    FP32 bilinear weights of the four 0/1 comparisons;
-//  - sin and cos of the Vogel angles are a Cody-Waite reduction by pi/2 with
-//    a three-part constant and Taylor polynomials to degree 9 and 10 (Horner
-//    with fused multiply-adds), about 1 ulp for |x| < 6000;
+//  - sin and cos are a Cody-Waite reduction by pi/2 with a three-part
+//    constant and Taylor polynomials to degree 9 and 10 (Horner with fused
+//    multiply-adds), about 1 ulp for |x| < 6000; the Vogel angle
+//    i * GOLDEN_ANGLE + theta goes through the angle-addition identity from
+//    sin/cos of theta (once per pixel) and of f32(i) * GOLDEN_ANGLE (a table);
 //  - a projection with a NaN coordinate leaves the layer (factor 1), and so
 //    does a layer index past the atlas.
 #pragma once
@@ -61,25 +63,43 @@ __device__ __forceinline__ float sh_hash22(float px, float py) {
 __device__ __forceinline__ float sh_min(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float sh_max(float a, float b) { return a > b ? a : b; }
 
-// textureSampleCompareLevel with the LessEqual / linear / clamp-to-edge sampler
-// on layer texels t (size x size).
-__device__ __forceinline__ float sh_compare(const float* t, uint32_t size, float fs, float u, float v, float ref) {
+// Two adjacent texels of a row: one 8-byte load from a 4-byte aligned address
+// (dword alignment is enough for multi-dword global loads).
+struct __attribute__((aligned(4))) sh_f2 {
+    float x, y;
+};
+
+// One tap of the PCF: textureSampleCompareLevel with the LessEqual / linear /
+// clamp-to-edge sampler on the dynamic layer dl and the static layer sl
+// (size x size, size >= 2) at the same coordinate, min of the two (:314-326).
+// Each row's two taps come from one load of the pair starting at
+// clamp(x0, 0, size - 2): the left tap is its second texel when
+// x0 >= size - 1, the right tap its first when x0 < 0 (clamp to edge); a NaN
+// coordinate takes texel 0 for both.  c * w with c in {0, 1} and w >= 0 is
+// w or +0 exactly, so the weights are selected, not multiplied.
+__device__ __forceinline__ float sh_compare2(const float* dl, const float* sl, uint32_t size, float fs, float u,
+                                             float v, float ref) {
 #pragma clang fp contract(off)
     const float tx = u * fs - 0.5f, ty = v * fs - 0.5f;
     const float x0f = floorf(tx), y0f = floorf(ty);
     const float fx = tx - x0f, fy = ty - y0f;
     const float hi = (float)(size - 1u);
-    // fminf/fmaxf also map a NaN coordinate to the layer
-    const uint32_t x0 = (uint32_t)fminf(fmaxf(x0f, 0.0f), hi), x1 = (uint32_t)fminf(fmaxf(x0f + 1.0f, 0.0f), hi);
+    // fminf/fmaxf also map a NaN coordinate into the layer
+    const uint32_t xs = (uint32_t)fminf(fmaxf(x0f, 0.0f), hi - 1.0f);
     const uint32_t y0 = (uint32_t)fminf(fmaxf(y0f, 0.0f), hi), y1 = (uint32_t)fminf(fmaxf(y0f + 1.0f, 0.0f), hi);
-    const hl_gptr<const float> g = hl_g(t);
-    const float d00 = g[(size_t)y0 * size + x0], d10 = g[(size_t)y0 * size + x1];
-    const float d01 = g[(size_t)y1 * size + x0], d11 = g[(size_t)y1 * size + x1];
-    const float c00 = ref <= d00 ? 1.0f : 0.0f, c10 = ref <= d10 ? 1.0f : 0.0f;
-    const float c01 = ref <= d01 ? 1.0f : 0.0f, c11 = ref <= d11 ? 1.0f : 0.0f;
-    const float top = c00 * (1.0f - fx) + c10 * fx;
-    const float bot = c01 * (1.0f - fx) + c11 * fx;
-    return top * (1.0f - fy) + bot * fy;
+    const uint32_t o0 = y0 * size + xs, o1 = y1 * size + xs;  // < 2^28
+    const sh_f2 a0 = *reinterpret_cast<const sh_f2*>(dl + o0), a1 = *reinterpret_cast<const sh_f2*>(dl + o1);
+    const sh_f2 b0 = *reinterpret_cast<const sh_f2*>(sl + o0), b1 = *reinterpret_cast<const sh_f2*>(sl + o1);
+    const bool left_hi = x0f >= hi, right_lo = !(x0f >= 0.0f);
+    const float wl = 1.0f - fx, wt = 1.0f - fy;
+    const auto lerp = [&](sh_f2 r0, sh_f2 r1) {
+        const float d00 = left_hi ? r0.y : r0.x, d10 = right_lo ? r0.x : r0.y;
+        const float d01 = left_hi ? r1.y : r1.x, d11 = right_lo ? r1.x : r1.y;
+        const float top = (ref <= d00 ? wl : 0.0f) + (ref <= d10 ? fx : 0.0f);
+        const float bot = (ref <= d01 ? wl : 0.0f) + (ref <= d11 ? fx : 0.0f);
+        return top * wt + bot * fy;
+    };
+    return sh_min(lerp(a0, a1), lerp(b0, b1));
 }
 
 // light clip, NDC and the shadow uv (:281-290); false outside the layer
@@ -104,33 +124,58 @@ __device__ __forceinline__ bool sh_project(const ShadowView& s, uint32_t layer, 
     return u >= 0.0f && v >= 0.0f && u <= 1.0f && v <= 1.0f && nz >= 0.0f && nz <= 1.0f;
 }
 
+// Vogel disk tables of a context (k_vogel_table): cos and sin of
+// f32(i) * GOLDEN_ANGLE for i < kVogelMax, then for every count n <= kVogelMax
+// the radii sqrt(f32(i) + 0.5) / sqrt(f32(n)) at [kVogelRadii + n * kVogelMax + i].
+constexpr uint32_t kVogelMax = 256;
+constexpr uint32_t kVogelRadii = 2 * kVogelMax;
+constexpr size_t kVogelFloats = kVogelRadii + (size_t)(kVogelMax + 1) * kVogelMax;
+static_assert(kVogelFloats == kVogelTableFloats, "Vogel table size");
+
+// Tap i of count of vogel_disk_sample (:239-244) rotated by theta (sin sth,
+// cos cth): cos / sin(i * GOLDEN_ANGLE + theta) by the angle-addition identity.
+__device__ __forceinline__ void sh_vogel(const float* tab, uint32_t i, uint32_t count, float sth, float cth,
+                                         float& ox, float& oy) {
+#pragma clang fp contract(off)
+    const float ca = tab[i], sa = tab[kVogelMax + i], r = tab[kVogelRadii + (size_t)count * kVogelMax + i];
+    const float c = __builtin_fmaf(ca, cth, -(sa * sth));
+    const float sn = __builtin_fmaf(sa, cth, ca * sth);
+    ox = c * r;
+    oy = sn * r;
+}
+
 // PCF of count Vogel taps of the given radius (:306-329)
 __device__ __forceinline__ float sh_pcf(const ShadowView& s, uint32_t layer, float u, float v, float ref,
-                                        float radius, uint32_t count, float theta) {
+                                        float radius, uint32_t count, float sth, float cth) {
 #pragma clang fp contract(off)
     const size_t off = (size_t)layer * s.size * s.size;
     const float* dl = s.atlas + off;
     const float* sl = s.static_atlas + off;
     const float fs = (float)s.size;
-    const float rn = sqrtf((float)count);
     float sum = 0.0f;
-    for (uint32_t i = 0; i < count; ++i) {
-        // vogel_disk_sample, :239-244
-        const float r = sqrtf((float)i + 0.5f) / rn;
-        const float angle = (float)i * 2.39996323f + theta;
-        float sn, cs;
-        sh_sincos(angle, sn, cs);
-        const float su = u + (cs * r) * radius, sv = v + (sn * r) * radius;
-        const float dyn = sh_compare(dl, s.size, fs, su, sv, ref);
-        const float sta = sh_compare(sl, s.size, fs, su, sv, ref);
-        sum = sum + sh_min(dyn, sta);
+    uint32_t i = 0;
+    // two taps per step so that their loads are in flight together; the sum
+    // keeps the taps' order
+    for (; i + 1u < count; i += 2u) {
+        float ox0, oy0, ox1, oy1;
+        sh_vogel(s.vogel, i, count, sth, cth, ox0, oy0);
+        sh_vogel(s.vogel, i + 1u, count, sth, cth, ox1, oy1);
+        const float t0 = sh_compare2(dl, sl, s.size, fs, u + ox0 * radius, v + oy0 * radius, ref);
+        const float t1 = sh_compare2(dl, sl, s.size, fs, u + ox1 * radius, v + oy1 * radius, ref);
+        sum = sum + t0;
+        sum = sum + t1;
+    }
+    if (i < count) {
+        float ox, oy;
+        sh_vogel(s.vogel, i, count, sth, cth, ox, oy);
+        sum = sum + sh_compare2(dl, sl, s.size, fs, u + ox * radius, v + oy * radius, ref);
     }
     return sum / (float)count;
 }
 
 // sample_cascade_shadow, :273-330
 __device__ __forceinline__ float sh_cascade_pcf(const ShadowView& s, uint32_t layer, uint32_t cascade, float scale,
-                                                float wx, float wy, float wz, float theta) {
+                                                float wx, float wy, float wz, float sth, float cth) {
 #pragma clang fp contract(off)
     if (layer >= s.layers) return 1.0f;
     float u, v, z;
@@ -142,12 +187,12 @@ __device__ __forceinline__ float sh_cascade_pcf(const ShadowView& s, uint32_t la
     else if (cascade == 1u) n = base * 3u / 4u > 4u ? base * 3u / 4u : 4u;
     else if (cascade == 2u) n = base / 2u > 4u ? base / 2u : 4u;
     else n = base / 4u > 4u ? base / 4u : 4u;
-    return sh_pcf(s, layer, u, v, z, radius, n, theta);
+    return sh_pcf(s, layer, u, v, z, radius, n, sth, cth);
 }
 
 // sample_cascade_shadow_pcss with the blocker search and penumbra, :335-450
 __device__ __forceinline__ float sh_cascade_pcss(const ShadowView& s, uint32_t layer, uint32_t cascade, float wx,
-                                                 float wy, float wz, float theta) {
+                                                 float wy, float wz, float sth, float cth) {
 #pragma clang fp contract(off)
     if (layer >= s.layers) return 1.0f;
     float u, v, z;
@@ -159,14 +204,11 @@ __device__ __forceinline__ float sh_cascade_pcss(const ShadowView& s, uint32_t l
     const size_t off = (size_t)layer * size * size;
     const hl_gptr<const float> dl = hl_g(s.atlas + off), sl = hl_g(s.static_atlas + off);
     const uint32_t nb = s.blocker_samples;
-    const float rn = sqrtf((float)nb);
     float bsum = 0.0f, bcount = 0.0f;
     for (uint32_t i = 0; i < nb; ++i) {
-        const float r = sqrtf((float)i + 0.5f) / rn;
-        const float angle = (float)i * 2.39996323f + theta;
-        float sn, cs;
-        sh_sincos(angle, sn, cs);
-        const float su = u + (cs * r) * search, sv = v + (sn * r) * search;
+        float ox, oy;
+        sh_vogel(s.vogel, i, nb, sth, cth, ox, oy);
+        const float su = u + ox * search, sv = v + oy * search;
         // i32(): toward zero; the bounds test below rejects what lies outside
         const float fx = su * fs, fy = sv * fs;
         if (!(fx > -1.0f && fy > -1.0f && fx < fs && fy < fs)) continue;
@@ -182,14 +224,14 @@ __device__ __forceinline__ float sh_cascade_pcss(const ShadowView& s, uint32_t l
     const float avg = bsum / bcount;
     const float penumbra = (z - avg) / sh_max(avg, 0.001f) * light_size;
     const float radius = sh_min(sh_max(penumbra / fs, frad / fs), frad * 3.0f / fs);
-    return sh_pcf(s, layer, u, v, z, radius, s.filter_samples, theta);
+    return sh_pcf(s, layer, u, v, z, radius, s.filter_samples, sth, cth);
 }
 
 __device__ __forceinline__ float sh_cascade(const ShadowView& s, uint32_t base, uint32_t c, float wx, float wy,
-                                            float wz, float theta) {
+                                            float wz, float sth, float cth) {
 #pragma clang fp contract(off)
-    if (s.enable_pcss != 0u && s.light_size[c] > 0.0f) return sh_cascade_pcss(s, base + c, c, wx, wy, wz, theta);
-    return sh_cascade_pcf(s, base + c, c, 1.0f + (float)c * 1.5f, wx, wy, wz, theta);
+    if (s.enable_pcss != 0u && s.light_size[c] > 0.0f) return sh_cascade_pcss(s, base + c, c, wx, wy, wz, sth, cth);
+    return sh_cascade_pcf(s, base + c, c, 1.0f + (float)c * 1.5f, wx, wy, wz, sth, cth);
 }
 
 // shadow_factor (:452-565) of light record L (position p0.xyz, direction
@@ -218,6 +260,8 @@ __device__ __forceinline__ float shadow_factor(const ShadowView& s, const HotLig
     const float k = 1.0f - ndl;
     const float bx = wx + nx * 0.01f * k, by = wy + ny * 0.01f * k, bz = wz + nz * 0.01f * k;
     const float theta = sh_hash22(fx, fy) * 6.28318530718f;
+    float sth, cth;
+    sh_sincos(theta, sth, cth);
     if (type == HL_LIGHT_POINT) {
         const float tx = bx - L.p0.x, ty = by - L.p0.y, tz = bz - L.p0.z;
         const float ax = fabsf(tx), ay = fabsf(ty), az = fabsf(tz);
@@ -225,9 +269,9 @@ __device__ __forceinline__ float shadow_factor(const ShadowView& s, const HotLig
         if (ax >= ay && ax >= az) face = tx < 0.0f ? 1u : 0u;
         else if (ay >= ax && ay >= az) face = ty < 0.0f ? 3u : 2u;
         else face = tz < 0.0f ? 5u : 4u;
-        return sh_cascade_pcf(s, base + face, 0u, 1.0f, bx, by, bz, theta);
+        return sh_cascade_pcf(s, base + face, 0u, 1.0f, bx, by, bz, sth, cth);
     }
-    if (type != HL_LIGHT_DIRECTIONAL) return sh_cascade_pcf(s, base, 0u, 1.0f, bx, by, bz, theta);
+    if (type != HL_LIGHT_DIRECTIONAL) return sh_cascade_pcf(s, base, 0u, 1.0f, bx, by, bz, sth, cth);
     // directional: cascades by the camera distance of the unbiased position
     const float dx = wx - eye[0], dy = wy - eye[1], dz = wz - eye[2];
     const float dist = sqrtf(hl_x_dot3(dx, dy, dz, dx, dy, dz));
@@ -248,10 +292,10 @@ __device__ __forceinline__ float shadow_factor(const ShadowView& s, const HotLig
             break;
         }
     }
-    const float sa = sh_cascade(s, base, ca, bx, by, bz, theta);
+    const float sa = sh_cascade(s, base, ca, bx, by, bz, sth, cth);
     if (blend <= 0.001f) return sa;
     if (cb != ca) {
-        const float sb = sh_cascade(s, base, cb, bx, by, bz, theta);
+        const float sb = sh_cascade(s, base, cb, bx, by, bz, sth, cth);
         return sa * (1.0f - blend) + sb * blend;
     }
     return sa;

@@ -17,7 +17,9 @@
  *    ref <= texel;
  *  - sin and cos: Cody-Waite reduction by pi/2 (three-part constant) and the
  *    Taylor polynomials to degree 9 / 10, Horner with fused multiply-adds
- *    (or_sincos), accurate to about 1 ulp for |x| < 6000;
+ *    (or_sincos), accurate to about 1 ulp for |x| < 6000; the Vogel angle
+ *    i * GOLDEN_ANGLE + theta by the angle-addition identity from sin/cos of
+ *    theta and of f32(i) * GOLDEN_ANGLE (two fused multiply-adds);
  *  - a projection with a NaN coordinate leaves the layer (factor 1).
  */
 #include <math.h>
@@ -71,14 +73,24 @@ float or_hash22(float px, float py) {
     return sh_fract((x + y) * d);
 }
 
-/* vogel_disk_sample, :239-244 */
-void or_vogel(uint32_t i, uint32_t n, float theta, float out[2]) {
+/* vogel_disk_sample, :239-244, given sin and cos of the rotation theta:
+ * cos / sin(i * GOLDEN_ANGLE + theta) by the angle-addition identity from
+ * those of a = f32(i) * GOLDEN_ANGLE (a table per sample index in the
+ * library), r = sqrt(f32(i) + 0.5) / sqrt(f32(n)). */
+static void sh_vogel(uint32_t i, uint32_t n, float sth, float cth, float out[2]) {
     const float r = sqrtf((float)i + 0.5f) / sqrtf((float)n);
-    const float angle = (float)i * 2.39996323f + theta;
-    float s, c;
-    or_sincos(angle, &s, &c);
+    float sa, ca;
+    or_sincos((float)i * 2.39996323f, &sa, &ca);
+    const float c = fmaf(ca, cth, -(sa * sth));
+    const float s = fmaf(sa, cth, ca * sth);
     out[0] = c * r;
     out[1] = s * r;
+}
+
+void or_vogel(uint32_t i, uint32_t n, float theta, float out[2]) {
+    float sth, cth;
+    or_sincos(theta, &sth, &cth);
+    sh_vogel(i, n, sth, cth, out);
 }
 
 /* point_light_face, :253-263 */
@@ -93,7 +105,8 @@ static const float* sh_layer(const float* atlas, uint32_t size, uint32_t layer) 
     return atlas + (size_t)layer * size * size;
 }
 
-/* textureSampleCompareLevel with the LessEqual / linear / clamp-to-edge sampler. */
+/* textureSampleCompareLevel with the LessEqual / linear / clamp-to-edge sampler
+ * (size >= 2).  A NaN coordinate takes texel 0 for both columns / rows. */
 float or_sample_compare(const float* atlas, uint32_t size, uint32_t layer, float u, float v, float ref) {
     const float* t = sh_layer(atlas, size, layer);
     const float S = (float)size;
@@ -101,7 +114,8 @@ float or_sample_compare(const float* atlas, uint32_t size, uint32_t layer, float
     const float x0f = floorf(tx), y0f = floorf(ty);
     const float fx = tx - x0f, fy = ty - y0f;
     const float hi = (float)(size - 1u);
-    const uint32_t x0 = (uint32_t)sh_clamp(x0f, 0.0f, hi), x1 = (uint32_t)sh_clamp(x0f + 1.0f, 0.0f, hi);
+    const uint32_t x0 = (uint32_t)sh_clamp(x0f, 0.0f, hi);
+    const uint32_t x1 = x0f >= 0.0f ? (uint32_t)sh_clamp(x0f + 1.0f, 0.0f, hi) : 0u;
     const uint32_t y0 = (uint32_t)sh_clamp(y0f, 0.0f, hi), y1 = (uint32_t)sh_clamp(y0f + 1.0f, 0.0f, hi);
     const float c00 = ref <= t[(size_t)y0 * size + x0] ? 1.0f : 0.0f;
     const float c10 = ref <= t[(size_t)y0 * size + x1] ? 1.0f : 0.0f;
@@ -136,10 +150,11 @@ static int sh_project(const hl_shadow_maps* sh, uint32_t layer, const float wp[3
 
 static float sh_pcf(const hl_shadow_maps* sh, uint32_t layer, float u, float v, float ref, float radius,
                     uint32_t count, float theta) {
-    float sum = 0.0f;
+    float sum = 0.0f, sth, cth;
+    or_sincos(theta, &sth, &cth);
     for (uint32_t i = 0; i < count; ++i) {
         float o[2];
-        or_vogel(i, count, theta, o);
+        sh_vogel(i, count, sth, cth, o);
         const float su = u + o[0] * radius, sv = v + o[1] * radius;
         const float dyn = or_sample_compare(sh->atlas_d, sh->atlas_size, layer, su, sv, ref);
         const float sta = or_sample_compare(sh->static_atlas_d, sh->atlas_size, layer, su, sv, ref);
@@ -180,12 +195,13 @@ float or_sample_cascade_shadow_pcss(const hl_shadow_maps* sh, uint32_t layer, ui
     const float theta = or_hash22(fx, fy) * 6.28318530718f;
     const float search = cc->pcss_light_size / S;
     const uint32_t nb = sh->config.pcss_blocker_samples;
-    float bsum = 0.0f, bcount = 0.0f;
+    float bsum = 0.0f, bcount = 0.0f, sth, cth;
+    or_sincos(theta, &sth, &cth);
     const float* dl = sh_layer(sh->atlas_d, size, layer);
     const float* sl = sh_layer(sh->static_atlas_d, size, layer);
     for (uint32_t i = 0; i < nb; ++i) {
         float o[2];
-        or_vogel(i, nb, theta, o);
+        sh_vogel(i, nb, sth, cth, o);
         const float su = u + o[0] * search, sv = v + o[1] * search;
         const int px = (int)(su * S), py = (int)(sv * S); /* i32(): toward zero */
         if (px < 0 || py < 0 || px >= (int)size || py >= (int)size) continue;
