@@ -22,7 +22,8 @@ Further legs, each reported in its own object of the same line:
   shadowed_4k     the 4K/16k frame with shadow_factor (hl_deferred_forward_
                   shadowed): a directional sun (4 cascades) and 8 point + 8
                   spot shadow casters, 1024^2 atlases (60 layers, dynamic and
-                  static), PCF (medium preset) and PCSS (high preset)
+                  static), PCF (medium preset) and PCSS (high preset), and
+                  medium + radiance-cascade GI (hl_deferred_forward_ex)
   forward_lit_4k  forward-lit fragment shading (hl_forward_lit) of the 4K
                   frame's 4.9 M surfaces as fragments (8x8-pixel blocks in
                   raster order) under the same 16384 lights read through
@@ -243,6 +244,17 @@ def bench_shadowed(ctx, args):
                         "pcss": bool(dsh.desc.config.enable_pcss),
                         "pcf_samples": int(dsh.desc.config.pcf_sample_count)}
         log(f"shadowed 4K {quality}: {ms:.3f} ms/frame (shade {parts[1] * 1e3:.1f} us)")
+    # radiance-cascade indirect diffuse on top of the medium shadows (hl_deferred_forward_ex)
+    from helio_amd.device import DeviceTexture
+    atlas = synth.make_rc_probes(sc, box=((-100.0, -1.0, -100.0), (100.0, 30.0, 100.0)))
+    rc = DeviceTexture(ctx, atlas)
+    shade.prepare(ds)  # the globals now carry has_rc_gi and the probe box
+    dsh.desc.config = synth.shadow_config("medium")
+    ms, parts = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
+                           lambda: shade.execute(ds, grid, shadows=dsh, rc=rc))
+    res["medium_rc_gi"] = {"ms_per_frame": ms, "shade_ms": parts[1], "evals_per_s": q / (ms * 1e-3),
+                           "rc_atlas": "64x1024 RGBA16F, probe box (-100,-1,-100)..(100,30,100)"}
+    log(f"shadowed 4K medium + RC GI: {ms:.3f} ms/frame (shade {parts[1] * 1e3:.1f} us)")
     return res
 
 

@@ -218,6 +218,8 @@ SIGNATURES: dict[str, tuple] = {
                                    P(hl_light_grid), vp, u32, u32]),
     "hl_deferred_forward_shadowed": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                             P(hl_light_grid), P(hl_shadow_maps), vp, u32, u32]),
+    "hl_deferred_forward_ex": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32, P(hl_light_grid),
+                                      P(hl_shadow_maps), P(hl_texture), vp, u32, u32]),
     "hl_count_pixel_light_pairs": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                           P(hl_light_grid), P(C.c_uint64)]),
     "hl_forward_lit": (None, [vp, vp, P(ForwardLitGlobals), vp, u32, vp, u32, vp, vp, u32,

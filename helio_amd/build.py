@@ -93,7 +93,7 @@ def build_oracle() -> Path:
     out = ORACLE / "liboracle.so"
     _run(["gcc", *CC_FLAGS, f"-I{ROOT / 'include'}", str(ORACLE / "oracle.c"),
           str(ORACLE / "oracle64.c"), str(ORACLE / "gbuffer_pack.c"),
-          str(ORACLE / "forward_lit.c"), str(ORACLE / "shadow.c"), "-o", str(out), "-lm"])
+          str(ORACLE / "forward_lit.c"), str(ORACLE / "shadow.c"), str(ORACLE / "rc_gi.c"), "-o", str(out), "-lm"])
     return out
 
 

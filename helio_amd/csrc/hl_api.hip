@@ -605,10 +605,11 @@ HL_API hl_status hl_gbuffer_pack(hl_context* c, const hl_surface* sf, const hl_g
 
 namespace {
 
-// hl_deferred_forward and its shadowed variant (sh == nullptr: unshadowed).
+// hl_deferred_forward with optional shadows (sh) and radiance-cascade GI (rc).
 hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
                       const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights, const hl_light_grid* grid,
-                      const hl_shadow_maps* sh, float* out_d, uint32_t out_pitch_bytes, uint32_t flags) {
+                      const hl_shadow_maps* sh, const hl_texture* rc, float* out_d, uint32_t out_pitch_bytes,
+                      uint32_t flags) {
     if (!c) return HL_EINVAL;
     if (!out_d) return fail(c, HL_EINVAL, "output is null");
     ShadeArgs a;
@@ -627,6 +628,14 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
         if (k.pcf_sample_count > 256 || k.pcss_blocker_samples > 256 || k.pcss_filter_samples > 256)
             return fail(c, HL_EINVAL, "shadow sample counts above 256");
     }
+    const bool use_rc = rc && rc->data_d && g && g->has_rc_gi != 0u;
+    if (use_rc) {
+        if (rc->format != HL_FMT_RGBA16_FLOAT && rc->format != HL_FMT_RGBA32_FLOAT)
+            return fail(c, HL_EINVAL, "rc_cascade0 must be RGBA16_FLOAT or RGBA32_FLOAT");
+        const uint32_t tb = rc->format == HL_FMT_RGBA16_FLOAT ? 8u : 16u;
+        if (rc->pitch_bytes < 64u * tb || rc->pitch_bytes % tb || ((uintptr_t)rc->data_d) % tb)
+            return fail(c, HL_EINVAL, "rc_cascade0: pitch %u or alignment invalid for 64 texels", rc->pitch_bytes);
+    }
     hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, num_lights, grid, &a, &set, false,
                                 (flags & HL_SHADE_LIGHTS_FROM_CULL) != 0);
     if (s != HL_OK) return s;
@@ -637,6 +646,15 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
     a.out_pitch = out_pitch_bytes;
     a.flags = flags;
     int mode = kModeForward;
+    if (use_rc) {
+        a.rc = Tex{(const uint8_t*)rc->data_d, rc->pitch_bytes};
+        a.rc_fmt = rc->format;
+        for (int k = 0; k < 3; ++k) {
+            a.rc_min[k] = g->rc_world_min[k];
+            a.rc_size[k] = g->rc_world_max[k] - g->rc_world_min[k];
+        }
+        mode = kModeExtended;
+    }
     if (sh) {
         ShadowView& v = a.shadow;
         v.atlas = sh->atlas_d;
@@ -659,7 +677,7 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
             c->vogel_ready = true;
         }
         v.vogel = (const float*)c->vogel.p;
-        mode = kModeShadowed;
+        mode = kModeExtended;
     }
     return shade_launch(c, set, mode, a);
 }
@@ -673,7 +691,8 @@ HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* cam
                                      const GpuLight* lights_d, uint32_t num_lights,
                                      const hl_light_grid* grid, float* out_d,
                                      uint32_t out_pitch_bytes, uint32_t flags) {
-    return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, nullptr, out_d, out_pitch_bytes, flags);
+    return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, nullptr, nullptr, out_d, out_pitch_bytes,
+                       flags);
 }
 
 HL_API hl_status hl_deferred_forward_shadowed(hl_context* c, const GpuCameraUniforms* camera_d,
@@ -681,7 +700,17 @@ HL_API hl_status hl_deferred_forward_shadowed(hl_context* c, const GpuCameraUnif
                                               const GpuLight* lights_d, uint32_t num_lights,
                                               const hl_light_grid* grid, const hl_shadow_maps* shadows,
                                               float* out_d, uint32_t out_pitch_bytes, uint32_t flags) {
-    return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, shadows, out_d, out_pitch_bytes, flags);
+    return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, shadows, nullptr, out_d, out_pitch_bytes,
+                       flags);
+}
+
+HL_API hl_status hl_deferred_forward_ex(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
+                                        const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights,
+                                        const hl_light_grid* grid, const hl_shadow_maps* shadows,
+                                        const hl_texture* rc_cascade0, float* out_d, uint32_t out_pitch_bytes,
+                                        uint32_t flags) {
+    return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, shadows, rc_cascade0, out_d,
+                       out_pitch_bytes, flags);
 }
 
 HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUniforms* camera_d,

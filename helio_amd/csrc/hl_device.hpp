@@ -149,7 +149,12 @@ struct ShadeArgs {
     uint32_t num_blocks;     // block tiles of the view
     uint32_t tw_log2;        // wavefront tile: (1 << tw_log2) x (64 >> tw_log2) pixels
     uint32_t _pad0;
-    ShadowView shadow;       // kModeShadowed only
+    ShadowView shadow;       // kModeExtended only (layers 0: no shadows)
+    // kModeExtended: radiance-cascade probe atlas (rc_fmt 0: no RC GI) and
+    // its world box (min, max - min)
+    Tex rc;
+    uint32_t rc_fmt;
+    float rc_min[3], rc_size[3];
 };
 
 // One view of a batched persistent launch (device memory).

@@ -15,6 +15,7 @@
 // the range test, in list order, as deferred_lighting.wgsl:1125-1145 does.
 // The adjoint re-derives the same per-wave lists instead of storing them.
 #include "hl_lighting.hpp"
+#include "hl_rc.hpp"
 #include "hl_shadow.hpp"
 
 namespace hl {
@@ -151,8 +152,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
     const bool valid = inside && depth < 1.0f;
     constexpr bool kAdj = MODE == kModeBackward || MODE == kModeFusedL2;
-    constexpr bool kShadow = MODE == kModeShadowed;
-    constexpr bool kFwd = MODE == kModeForward || kShadow;
+    constexpr bool kExt = MODE == kModeExtended;
+    constexpr bool kFwd = MODE == kModeForward || kExt;
     constexpr bool kWrites = kFwd || MODE == kModeFusedL2;
     float4* out_px = (kWrites && inside)
                          ? (float4*)(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x
@@ -200,7 +201,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         const float ex = vc.eye[0] - p.wx, ey = vc.eye[1] - p.wy, ez = vc.eye[2] - p.wz;
         const float in_ = hl_f_rsq(Nr.x * Nr.x + Nr.y * Nr.y + Nr.z * Nr.z);
         const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
-        if (kShadow) {
+        if (kExt) {
             // shadow_factor's normal offset and N.L take the shader's exact N
 #pragma clang fp contract(off)
             const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
@@ -338,6 +339,24 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         mixv[c] = sky * 0.15f * (1.0f - ht) + sky * ht;
         amb[c] = mixv[c] * p.alb[c] * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
     }
+    if (kExt && a.rc_fmt != 0u && valid) {
+        // radiance-cascade indirect diffuse (deferred_lighting.wgsl:1159-1163,
+        // 1189-1205): mix(hemisphere, kD_ibl rc_irr albedo, clamp(4 |rc_irr|, 0, 1))
+        const float3 rc = rc_irradiance(a, p.wx, p.wy, p.wz, p.nx, p.ny, p.nz);
+        const float w = fminf(fmaxf(sqrtf(rc.x * rc.x + rc.y * rc.y + rc.z * rc.z) * 4.0f, 0.0f), 1.0f);
+        const float u = fminf(fmaxf(1.0f - p.ndv, 0.0f), 1.0f);
+        const float u2 = u * u;
+        const float p5 = u2 * u2 * u;
+        const float rcv[3] = {rc.x, rc.y, rc.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            // fresnel_schlick_roughness (pbr_eval.wgsl:63-66)
+            const float f = p.f0[c] + (fmaxf(1.0f - p.rough, p.f0[c]) - p.f0[c]) * p5;
+            const float diff = (1.0f - f) * (1.0f - p.metal) * rcv[c] * p.alb[c];
+            const float hemi = mixv[c] * p.alb[c];
+            amb[c] = (hemi * (1.0f - w) + diff * w) * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
+        }
+    }
     if (MODE != kModeBackward) {
         walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
             if (!mine) return;
@@ -345,11 +364,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             if (eval_term<false>(L, p, t)) {
                 // radiance x NdL = p1 (atten NdL)
                 float aw = t.atten * t.ndl;
-                if (kShadow) {
+                if (kExt) {
                     // pbr_direct_light's sf (deferred_lighting.wgsl:1136-1143); the
                     // word is the light's, so the branch is uniform
                     const uint32_t w = __float_as_uint(L.p3.w);
-                    if (w != kNoShadowWord)
+                    if (w != kNoShadowWord && a.shadow.layers != 0u)
                         aw *= shadow_factor(a.shadow, L, w >> 28, w & 0x0FFFFFFFu, p.wx, p.wy, p.wz, p.nx, p.ny,
                                             p.nz, (float)x + 0.5f, (float)y + 0.5f, vc.eye);
                 }
@@ -485,7 +504,7 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
 // Forward launches are held to 96 VGPRs (5 waves per SIMD): the tile's
 // memory round trips need the waves to hide them.
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeShadowed ? 4 : 1)) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeExtended ? 3 : 1)) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
                                                uint32_t* __restrict__ queue_next) {
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
@@ -611,7 +630,7 @@ hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& 
         case kModeCount: return launch_fmt<kModeCount>(s, gbuf_set, a, queue, queue_next);
         case kModeBackward: return launch_fmt<kModeBackward>(s, gbuf_set, a, queue, queue_next);
         case kModeFusedL2: return launch_fmt<kModeFusedL2>(s, gbuf_set, a, queue, queue_next);
-        case kModeShadowed: return launch_fmt<kModeShadowed>(s, gbuf_set, a, queue, queue_next);
+        case kModeExtended: return launch_fmt<kModeExtended>(s, gbuf_set, a, queue, queue_next);
         default: return hipErrorInvalidValue;
     }
 }

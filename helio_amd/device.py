@@ -176,6 +176,17 @@ class DeviceShadows:
         self.desc = d
 
 
+class DeviceTexture:
+    """An RGBA16F / RGBA32F image resident in HBM as an hl_texture (rows tightly packed)."""
+
+    def __init__(self, ctx: Context, img: np.ndarray):
+        self.buf = ctx.upload(img)
+        t = abi.hl_texture()
+        t.data_d, t.pitch_bytes = self.buf.ptr, img.shape[1] * img.itemsize * img.shape[2]
+        t.format = abi.HL_FMT_RGBA16_FLOAT if img.dtype == np.float16 else abi.HL_FMT_RGBA32_FLOAT
+        self.desc = t
+
+
 class TileGrid:
     """tile_light_lists / tile_light_counts (crates/libhelio/src/frame.rs:230-234)."""
 
@@ -293,8 +304,13 @@ class DeferredLightPass:
         self.globals = g
 
     def execute(self, scene: DeviceScene, grid: abi.hl_light_grid, flags: int = 0,
-                shadows: "DeviceShadows | None" = None) -> None:
-        if shadows is None:
+                shadows: "DeviceShadows | None" = None, rc: "DeviceTexture | None" = None) -> None:
+        if rc is not None:
+            self.ctx.lib.call("hl_deferred_forward_ex", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                              C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
+                              C.byref(shadows.desc) if shadows is not None else None, C.byref(rc.desc),
+                              self.out.ptr, self.width * 16, flags)
+        elif shadows is None:
             self.ctx.lib.call("hl_deferred_forward", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
                               C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
                               self.out.ptr, self.width * 16, flags)

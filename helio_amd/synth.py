@@ -516,3 +516,37 @@ def add_shadow_casters(scene: HostScene, n_point: int = 8, n_spot: int = 8, dire
                                         atlas.ctypes.data, static.ctypes.data, mats.ctypes.data) != 0:
         raise ValueError("bad shadow map request")
     return HostShadows(atlas, static, mats, shadow_config(quality))
+
+
+# --------------------------------------------------------- radiance cascades
+def make_rc_probes(scene: HostScene, fmt: int = abi.HL_FMT_RGBA16_FLOAT, seed: int = 13,
+                   box: tuple | None = None) -> np.ndarray:
+    """A synthetic rc_cascade0 probe atlas (64 x 1024 texels: 16^3 probes of
+    4 x 4 directions) with smooth positive radiance, and the scene's globals
+    switched to radiance-cascade GI over box ((min xyz), (max xyz); default:
+    the central part of the scene, so that the volume fade shows)."""
+    rng = np.random.default_rng(seed)
+    px, py, pz, dx, dy = np.meshgrid(np.arange(16), np.arange(16), np.arange(16), np.arange(4), np.arange(4),
+                                     indexing="ij")
+    base = rng.uniform(0.0, 0.6, size=(3, 4, 4))
+    val = np.zeros((16, 16, 16, 4, 4, 3))
+    for c in range(3):
+        val[..., c] = base[c][dx, dy] * (0.5 + 0.5 * np.sin(0.4 * px + 0.3 * py + 0.5 * pz + c))
+    atlas = np.zeros((1024, 64, 4))
+    # texel (x, y) = (px * 4 + dx, (py * 16 + pz) * 4 + dy)
+    atlas[((py * 16 + pz) * 4 + dy).ravel(), (px * 4 + dx).ravel(), :3] = val.reshape(-1, 3)
+    atlas[..., 3] = 1.0
+    g = scene.globals
+    lo, hi = box if box is not None else ((-15.0, -1.0, -15.0), (15.0, 12.0, 15.0))
+    g.rc_world_min[:] = (*lo, 0.0)
+    g.rc_world_max[:] = (*hi, 0.0)
+    g.has_rc_gi = 1
+    dt = np.float16 if fmt == abi.HL_FMT_RGBA16_FLOAT else np.float32
+    return np.ascontiguousarray(atlas.astype(dt))
+
+
+def rc_texture_host(atlas: np.ndarray) -> abi.hl_texture:
+    t = abi.hl_texture()
+    t.data_d, t.pitch_bytes = atlas.ctypes.data, atlas.shape[1] * atlas.itemsize * 4
+    t.format = abi.HL_FMT_RGBA16_FLOAT if atlas.dtype == np.float16 else abi.HL_FMT_RGBA32_FLOAT
+    return t

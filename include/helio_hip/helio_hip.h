@@ -306,6 +306,23 @@ HL_API hl_status hl_deferred_forward_shadowed(hl_context* ctx, const GpuCameraUn
                                               const hl_light_grid* grid, const hl_shadow_maps* shadows,
                                               float* out_d, uint32_t out_pitch_bytes, uint32_t flags);
 
+/* hl_deferred_forward_shadowed plus the radiance-cascade indirect diffuse
+ * (sample_rc_irradiance, deferred_lighting.wgsl:842-904, combined at
+ * :1159-1163 and :1189-1205): when rc_cascade0 is given and
+ * globals->has_rc_gi != 0, the ambient term becomes
+ * mix(hemisphere, kD_ibl * rc_irr * albedo, clamp(4 |rc_irr|, 0, 1)) * ao with
+ * kD_ibl from fresnel_schlick_roughness (pbr_eval.wgsl:63-66).  rc_cascade0 is
+ * the RC probe atlas (bound at :168): RC_DIR_DIM * RC_PROBE_DIM = 64 texels
+ * wide, 1024 rows, RGBA16_FLOAT or RGBA32_FLOAT, read texel by texel
+ * (textureLoad) over the box globals->rc_world_min .. rc_world_max.  shadows
+ * and rc_cascade0 may each be NULL. */
+HL_API hl_status hl_deferred_forward_ex(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                        const DeferredGlobals* globals, const hl_gbuffer* gbuffer,
+                                        const GpuLight* lights_d, uint32_t num_lights,
+                                        const hl_light_grid* grid, const hl_shadow_maps* shadows,
+                                        const hl_texture* rc_cascade0, float* out_d, uint32_t out_pitch_bytes,
+                                        uint32_t flags);
+
 /* Counts Q, the (pixel, light) pairs the forward pass evaluates past the range
  * test (type != 0 implies dist <= range) -- the "pixel-light evals" of the
  * throughput metric.  Same inputs as hl_deferred_forward; writes *pairs (host). */

@@ -552,11 +552,12 @@ static int or_debug_supported(uint32_t m) {
     return m == 0 || m == 1 || m == 2 || m == 4 || m == 5 || m == 20 || m == 21;
 }
 
-uint64_t or_shade_shadowed(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
-                           const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
-                           const hl_shadow_maps* sh, float* out, uint32_t out_pitch, uint32_t flags,
-                           uint32_t row0, uint32_t row1) {
+uint64_t or_shade_ex(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
+                     const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
+                     const hl_shadow_maps* sh, const hl_texture* rc, float* out, uint32_t out_pitch,
+                     uint32_t flags, uint32_t row0, uint32_t row1) {
     (void)num_lights;
+    const int use_rc = rc && rc->data_d && g->has_rc_gi != 0u;
     if (!or_debug_supported(g->debug_mode)) return 0;
     if (row1 == 0 || row1 > gb->height) row1 = gb->height;
     uint64_t pairs = 0;
@@ -614,12 +615,28 @@ uint64_t or_shade_shadowed(const GpuCameraUniforms* cam, const DeferredGlobals* 
                 Lo[2] = Lo[2] + c[2];
             }
             if (!o) continue;
-            /* :1196-1231 ambient + combine (no RC, no lightmap, no SSS) */
+            /* :1159-1163 RC indirect diffuse */
+            float rc_irr[3] = {0.0f, 0.0f, 0.0f}, diff_ind[3] = {0.0f, 0.0f, 0.0f}, rc_w = 0.0f;
+            if (use_rc) {
+                or_rc_irradiance(rc, g->rc_world_min, g->rc_world_max, p.wp, p.N, rc_irr);
+                const float ndv = or_maxf(or_dot(p.N, p.V), 0.0f);
+                const float p5 = or_pow5(or_clampf(1.0f - ndv, 0.0f, 1.0f));
+                for (int k = 0; k < 3; ++k) {
+                    /* fresnel_schlick_roughness, pbr_eval.wgsl:63-66 */
+                    const float f = p.F0[k] + (or_maxf(1.0f - p.rough, p.F0[k]) - p.F0[k]) * p5;
+                    const float kd = (1.0f - f) * (1.0f - p.metal);
+                    diff_ind[k] = kd * rc_irr[k] * p.albedo[k];
+                }
+                /* :1198 clamp(length(rc_irr) * 4, 0, 1) */
+                rc_w = or_clampf(or_length(rc_irr) * 4.0f, 0.0f, 1.0f);
+            }
+            /* :1189-1231 ambient + combine (no lightmap, no SSS) */
             float hemi_t = p.N[1] * 0.5f + 0.5f;
             for (int k = 0; k < 3; ++k) {
                 float sky = g->ambient_color[k] * g->ambient_intensity;
                 float ground = sky * 0.15f;
                 float hemi = (ground * (1.0f - hemi_t) + sky * hemi_t) * p.albedo[k];
+                if (use_rc) hemi = hemi * (1.0f - rc_w) + diff_ind[k] * rc_w; /* mix(hemi, diff_ind, rc_weight) */
                 float indirect = hemi * p.ao;
                 float color = Lo[k] + indirect;
                 color = color + p.emissive[k];
@@ -634,7 +651,14 @@ uint64_t or_shade_shadowed(const GpuCameraUniforms* cam, const DeferredGlobals* 
 uint64_t or_shade(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
                   const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
                   float* out, uint32_t out_pitch, uint32_t flags, uint32_t row0, uint32_t row1) {
-    return or_shade_shadowed(cam, g, gb, lights, num_lights, grid, NULL, out, out_pitch, flags, row0, row1);
+    return or_shade_ex(cam, g, gb, lights, num_lights, grid, NULL, NULL, out, out_pitch, flags, row0, row1);
+}
+
+uint64_t or_shade_shadowed(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
+                           const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
+                           const hl_shadow_maps* sh, float* out, uint32_t out_pitch, uint32_t flags,
+                           uint32_t row0, uint32_t row1) {
+    return or_shade_ex(cam, g, gb, lights, num_lights, grid, sh, NULL, out, out_pitch, flags, row0, row1);
 }
 
 uint64_t or_count_pairs(const GpuCameraUniforms* cam, const DeferredGlobals* g,

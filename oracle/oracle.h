@@ -164,6 +164,17 @@ uint64_t or_shade_shadowed(const GpuCameraUniforms* cam, const DeferredGlobals* 
                            const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
                            const hl_shadow_maps* sh, float* out, uint32_t out_pitch, uint32_t flags,
                            uint32_t row0, uint32_t row1);
+/* or_shade with shadows (sh, may be NULL) and the radiance-cascade indirect
+ * diffuse from the rc_cascade0 probe atlas (rc, host texture; used when
+ * g->has_rc_gi != 0). */
+uint64_t or_shade_ex(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
+                     const GpuLight* lights, uint32_t num_lights, const hl_light_grid* grid,
+                     const hl_shadow_maps* sh, const hl_texture* rc, float* out, uint32_t out_pitch,
+                     uint32_t flags, uint32_t row0, uint32_t row1);
+/* RC GI pieces (rc_gi.c, deferred_lighting.wgsl:827-904). */
+void or_rc_oct_decode(float u, float v, float out[3]);
+void or_rc_irradiance(const hl_texture* t, const float wmin[3], const float wmax[3], const float wp[3],
+                      const float N[3], float out[3]);
 float or_shadow_factor(const hl_shadow_maps* sh, const GpuLight* lights, uint32_t light_idx, const float cam_pos[3],
                        const float csm_splits[4], const float wp[3], const float N[3], float fx, float fy);
 float or_sample_cascade_shadow(const hl_shadow_maps* sh, uint32_t layer, uint32_t cascade, float scale,

@@ -35,6 +35,13 @@ _SIGS = {
     "or_light_direction_from_rotation": (None, [P(C.c_float), P(C.c_float)]),
     "or_forward_pbr_direct_light": (None, [vp, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
                                            P(C.c_float), P(C.c_float), C.c_float, C.c_float, P(C.c_float)]),
+    "or_world_pos": (None, [P(abi.GpuCameraUniforms), u32, u32, u32, u32, C.c_float, P(C.c_float)]),
+    "or_shade_ex": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
+                                 P(abi.hl_light_grid), P(abi.hl_shadow_maps), P(abi.hl_texture), vp, u32, u32, u32,
+                                 u32]),
+    "or_rc_oct_decode": (None, [C.c_float, C.c_float, P(C.c_float)]),
+    "or_rc_irradiance": (None, [P(abi.hl_texture), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
+                                P(C.c_float)]),
     "or_shade_shadowed": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer), vp, u32,
                                        P(abi.hl_light_grid), P(abi.hl_shadow_maps), vp, u32, u32, u32, u32]),
     "or_shadow_factor": (C.c_float, [P(abi.hl_shadow_maps), vp, u32, P(C.c_float), P(C.c_float), P(C.c_float),
@@ -130,10 +137,11 @@ def cluster_grid_host(params, off, cnt, pool):
     return g
 
 
-def shade(scene, grid, globals_=None, flags=0, rows=None, out=None, shadows=None):
+def shade(scene, grid, globals_=None, flags=0, rows=None, out=None, shadows=None, rc=None):
     """FP32 restatement of fs_main; returns (rgba [H,W,4], pairs).  shadows:
     a synth.HostShadows to multiply the terms of shadow-casting lights by
-    shadow_factor."""
+    shadow_factor; rc: an rc_cascade0 probe atlas (synth.make_rc_probes) for the
+    radiance-cascade indirect diffuse (used when the globals have has_rc_gi)."""
     W, H = scene.width, scene.height
     if out is None:
         out = np.zeros((H, W, 4), dtype=np.float32)
@@ -141,10 +149,16 @@ def shade(scene, grid, globals_=None, flags=0, rows=None, out=None, shadows=None
     gb = scene.gbuffer_host()
     r0, r1 = rows if rows else (0, 0)
     sh = C.byref(shadows.host()) if shadows is not None else None
-    pairs = oracle().or_shade_shadowed(C.byref(scene.camera), C.byref(g), C.byref(gb),
-                                       scene.lights.ctypes.data if len(scene.lights) else None, len(scene.lights),
-                                       C.byref(grid), sh, out.ctypes.data, W * 16, flags, r0, r1)
+    rt = C.byref(rc_texture_of(rc)) if rc is not None else None
+    pairs = oracle().or_shade_ex(C.byref(scene.camera), C.byref(g), C.byref(gb),
+                                 scene.lights.ctypes.data if len(scene.lights) else None, len(scene.lights),
+                                 C.byref(grid), sh, rt, out.ctypes.data, W * 16, flags, r0, r1)
     return out, pairs
+
+
+def rc_texture_of(atlas):
+    from helio_amd.synth import rc_texture_host
+    return rc_texture_host(atlas)
 
 
 def forward_lit(camera, globals_, frags, lights, grid, entity=None, transforms=None, count_only=False):

@@ -1,0 +1,176 @@
+"""Radiance-cascade indirect diffuse (deferred_lighting.wgsl:824-904 and its
+combine at :1159-1163, 1189-1205), CPU side.
+
+The oracle (oracle/rc_gi.c) is pinned by
+* rc_oct_decode: unit vectors, the closed forms of an inner and a folded
+  cell, the mirror symmetry of the 4 x 4 grid;
+* sample_rc_irradiance closed forms: a constant atlas gives the constant times
+  the volume weight; outside the box 0; the smoothstep fade across the margin;
+  a single lit direction bin gives its cosine-weighted share; linear radiance
+  along x is reproduced between probe centres (trilinear);
+* the shaded frame: has_rc_gi = 0 is bit-identical to the plain shading; with
+  RC, pixels outside the box keep the hemisphere ambient bit for bit, and
+  inside it the ambient follows mix(hemi, kD_ibl rc albedo, clamp(4|rc|,0,1)).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from helio_amd import abi, synth
+import oracle_lib as O
+
+cf = C.c_float
+
+
+def oct_decode(u, v):
+    o = (cf * 3)()
+    O.oracle().or_rc_oct_decode(cf(u), cf(v), o)
+    return np.array(o[:], np.float64)
+
+
+def irradiance(atlas, wp, N, lo=(0.0, 0.0, 0.0), hi=(16.0, 16.0, 16.0)):
+    t = synth.rc_texture_host(atlas)
+    o = (cf * 3)()
+    O.oracle().or_rc_irradiance(C.byref(t), (cf * 3)(*lo), (cf * 3)(*hi), (cf * 3)(*wp), (cf * 3)(*N), o)
+    return np.array(o[:], np.float64)
+
+
+def atlas_from(fn, dtype=np.float32):
+    """texel (px * 4 + dx, (py * 16 + pz) * 4 + dy) = fn(px, py, pz, dx, dy) (rgb)."""
+    a = np.zeros((1024, 64, 4), dtype)
+    for px in range(16):
+        for py in range(16):
+            for pz in range(16):
+                for dx in range(4):
+                    for dy in range(4):
+                        a[(py * 16 + pz) * 4 + dy, px * 4 + dx, :3] = fn(px, py, pz, dx, dy)
+    return a
+
+
+def test_oct_decode():
+    dirs = np.array([oct_decode((dx + 0.5) / 4, (dy + 0.5) / 4) for dx in range(4) for dy in range(4)])
+    np.testing.assert_allclose(np.linalg.norm(dirs, axis=1), 1.0, atol=2e-7)
+    # (dx, dy) = (1, 1): f = (-0.25, -0.25), l = 0.5 <= 1: (f.x, 1 - l, f.y) normalised
+    np.testing.assert_allclose(oct_decode(0.375, 0.375), np.array([-0.25, 0.5, -0.25]) / math.sqrt(0.375), atol=1e-7)
+    # (0, 0): f = (-0.75, -0.75), l = 1.5 > 1: ((1 - 0.75) * -1, -0.5, (1 - 0.75) * -1)
+    np.testing.assert_allclose(oct_decode(0.125, 0.125), np.array([-0.25, -0.5, -0.25]) / math.sqrt(0.375), atol=1e-7)
+    # mirror symmetry in both axes
+    for dx in range(4):
+        for dy in range(4):
+            a, b = dirs[dx * 4 + dy], dirs[(3 - dx) * 4 + dy]
+            assert a[0] == -b[0] and a[1] == b[1] and a[2] == b[2]
+
+
+def test_constant_atlas_and_volume_fade():
+    c = np.array([0.3, 0.5, 0.7])
+    a = atlas_from(lambda *k: c)
+    for N in ((0, 1, 0), (0.6, 0.8, 0), (0, -1, 0)):
+        got = irradiance(a, (8.0, 8.0, 8.0), N)
+        np.testing.assert_allclose(got, c, rtol=2e-6)
+    assert np.all(irradiance(a, (17.0, 8.0, 8.0), (0, 1, 0)) == 0)     # outside the box
+    assert np.all(irradiance(a, (-0.5, 8.0, 8.0), (0, 1, 0)) == 0)
+    for x in (0.2, 0.4, 0.6, 15.7):                                     # the 5 % margin (0.8 units)
+        t = x / 16.0
+        s = lambda e0, e1, v: (lambda q: q * q * (3 - 2 * q))(min(max((v - e0) / (e1 - e0), 0), 1))  # noqa
+        want = s(0, 0.05, t) * s(1, 0.95, t)
+        np.testing.assert_allclose(irradiance(a, (x, 8.0, 8.0), (0, 1, 0)), c * want, rtol=1e-5)
+
+
+def test_single_direction_bin():
+    dirs = [oct_decode((dx + 0.5) / 4, (dy + 0.5) / 4) for dx in range(4) for dy in range(4)]
+    k = 2 * 4 + 1  # (dx, dy) = (2, 1), an upper-hemisphere bin
+    a = atlas_from(lambda px, py, pz, dx, dy: (1.0, 2.0, 3.0) if dx * 4 + dy == k else (0.0, 0.0, 0.0))
+    N = dirs[k]
+    w = np.maximum(0.0, np.array([N @ d for d in dirs], np.float32))
+    share = w[k] / w[w > 0.001].sum()
+    got = irradiance(a, (8.0, 8.0, 8.0), tuple(N.astype(np.float32)))
+    np.testing.assert_allclose(got, np.array([1.0, 2.0, 3.0]) * share, rtol=1e-5)
+
+
+def test_trilinear_between_probe_centres():
+    a = atlas_from(lambda px, py, pz, dx, dy: (px, 0.0, 0.0))
+    for x in (6.5, 6.75, 7.0, 7.5, 8.25):  # probe centres at i + 0.5 (16 units, 16 probes)
+        got = irradiance(a, (x, 8.0, 8.0), (0, 1, 0))
+        assert abs(got[0] - (x - 0.5)) < 2e-5, (x, got)
+
+
+def small(name="c1_720p_64"):
+    return synth.make_scene(name, width=320, height=180)
+
+
+def test_rc_off_is_plain_shading():
+    sc = small()
+    lists, counts = O.cull_tiles(sc)
+    g = O.tile_grid_host(sc, lists.ravel(), counts)
+    atlas = synth.make_rc_probes(sc)
+    sc.globals.has_rc_gi = 0
+    assert np.array_equal(O.shade(sc, g, rc=atlas)[0], O.shade(sc, g)[0])
+
+
+def test_rc_ambient_inside_the_box_only():
+    sc = small()
+    lists, counts = O.cull_tiles(sc)
+    g = O.tile_grid_host(sc, lists.ravel(), counts)
+    plain, _ = O.shade(sc, g)
+    atlas = synth.make_rc_probes(sc, box=((-12.0, -1.0, -12.0), (12.0, 10.0, 12.0)))
+    rc, _ = O.shade(sc, g, rc=atlas)
+    m = sc.depth < 1.0
+    H, W = sc.height, sc.width
+    wp = np.zeros((H, W, 3), np.float32)
+    for y in range(0, H):
+        for x in range(0, W):
+            if m[y, x]:
+                o = (cf * 3)()
+                O.oracle().or_world_pos(C.byref(sc.camera), W, H, x, y, cf(sc.depth[y, x]), o)
+                wp[y, x] = o[:]
+    inside = m & np.all((wp > (-12.0, -1.0, -12.0)) & (wp < (12.0, 10.0, 12.0)), axis=2)
+    outside = m & ~np.all((wp >= (-12.0, -1.0, -12.0)) & (wp <= (12.0, 10.0, 12.0)), axis=2)
+    assert outside.any() and inside.mean() > 0.05
+    assert np.array_equal(rc[outside], plain[outside])
+    assert np.mean(np.any(rc[inside] != plain[inside], axis=1)) > 0.9
+
+
+def test_rc_combine_closed_form():
+    """No lights, constant probes c, fp32 G-buffer: every pixel well inside the
+    box shades to mix(hemi, kD_ibl c albedo, clamp(4 |c|, 0, 1)) ao + emissive,
+    kD_ibl = (1 - F0 - (max(1 - r, F0) - F0) (1 - NdV)^5) (1 - metallic)."""
+    sc = synth.make_scene("c4_512_adj", width=128, height=128, num_lights=0)
+    lists, counts = O.cull_tiles(sc)
+    g = O.tile_grid_host(sc, lists.ravel(), counts)
+    c = np.array([0.05, 0.1, 0.15])
+    atlas = atlas_from(lambda *k: c)
+    synth.make_rc_probes(sc, box=((-60.0, -30.0, -60.0), (60.0, 30.0, 60.0)))  # sets the globals; vw = 1 inside
+    out, _ = O.shade(sc, g, rc=atlas)
+    H, W = sc.height, sc.width
+    alb = sc.albedo.view(np.float32).reshape(H, W, 4).astype(np.float64)
+    nrm = sc.normal.view(np.float32).reshape(H, W, 4).astype(np.float64)
+    orm = sc.orm.view(np.float32).reshape(H, W, 4).astype(np.float64)
+    emi = sc.emissive.view(np.float32).reshape(H, W, 4).astype(np.float64)
+    sky = np.array(sc.globals.ambient_color[:3], np.float64) * sc.globals.ambient_intensity
+    eye = np.array(sc.camera.position_near[:3], np.float64)
+    checked = 0
+    for y in range(0, H, 7):
+        for x in range(0, W, 7):
+            if sc.depth[y, x] >= 1.0:
+                continue
+            o = (cf * 3)()
+            O.oracle().or_world_pos(C.byref(sc.camera), W, H, x, y, cf(sc.depth[y, x]), o)
+            wp = np.array(o[:], np.float64)
+            if np.any(np.abs(wp) > (50.0, 25.0, 50.0)):
+                continue
+            N = nrm[y, x, :3] / np.linalg.norm(nrm[y, x, :3])
+            V = (eye - wp) / np.linalg.norm(eye - wp)
+            ndv = max(N @ V, 0.0)
+            F0 = np.clip([nrm[y, x, 3], orm[y, x, 3], emi[y, x, 3]], 0.0, 0.999)
+            r, m, ao = orm[y, x, 1], orm[y, x, 2], orm[y, x, 0]
+            F = F0 + (np.maximum(1 - r, F0) - F0) * (1 - ndv) ** 5
+            diff = (1 - F) * (1 - m) * c * alb[y, x, :3]
+            t = N[1] * 0.5 + 0.5
+            hemi = (sky * 0.15 * (1 - t) + sky * t) * alb[y, x, :3]
+            w = min(max(np.linalg.norm(c) * 4, 0.0), 1.0)
+            want = (hemi * (1 - w) + diff * w) * ao + emi[y, x, :3]
+            np.testing.assert_allclose(out[y, x, :3], want, rtol=2e-5, atol=1e-7)
+            checked += 1
+    assert checked > 100
