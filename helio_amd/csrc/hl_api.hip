@@ -632,9 +632,11 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
     if (use_rc) {
         if (rc->format != HL_FMT_RGBA16_FLOAT && rc->format != HL_FMT_RGBA32_FLOAT)
             return fail(c, HL_EINVAL, "rc_cascade0 must be RGBA16_FLOAT or RGBA32_FLOAT");
+        // 16-byte rows and start: the kernel reads two RGBA16F texels per load
         const uint32_t tb = rc->format == HL_FMT_RGBA16_FLOAT ? 8u : 16u;
-        if (rc->pitch_bytes < 64u * tb || rc->pitch_bytes % tb || ((uintptr_t)rc->data_d) % tb)
-            return fail(c, HL_EINVAL, "rc_cascade0: pitch %u or alignment invalid for 64 texels", rc->pitch_bytes);
+        if (rc->pitch_bytes < 64u * tb || rc->pitch_bytes % 16u || ((uintptr_t)rc->data_d) % 16u)
+            return fail(c, HL_EINVAL, "rc_cascade0: pitch %u invalid for 64 texels or not 16-byte aligned",
+                        rc->pitch_bytes);
     }
     hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, num_lights, grid, &a, &set, false,
                                 (flags & HL_SHADE_LIGHTS_FROM_CULL) != 0);
@@ -653,7 +655,7 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
             a.rc_min[k] = g->rc_world_min[k];
             a.rc_size[k] = g->rc_world_max[k] - g->rc_world_min[k];
         }
-        mode = kModeExtended;
+        mode = kModeExtended;  // shadows (if any) and RC
     }
     if (sh) {
         ShadowView& v = a.shadow;
@@ -677,7 +679,7 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
             c->vogel_ready = true;
         }
         v.vogel = (const float*)c->vogel.p;
-        mode = kModeExtended;
+        if (mode != kModeExtended) mode = kModeShadowed;
     }
     return shade_launch(c, set, mode, a);
 }

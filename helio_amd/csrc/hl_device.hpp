@@ -329,6 +329,7 @@ __device__ __forceinline__ float hl_unorm8(uint32_t v) {
 
 typedef unsigned int hl_u32x2 __attribute__((ext_vector_type(2)));
 typedef float hl_f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int hl_u32x4 __attribute__((ext_vector_type(4)));
 
 // Global-memory view of a device pointer.  Accesses through it compile to
 // global instructions even when the pointer itself was read from memory (a

@@ -169,10 +169,18 @@ hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCam
                                uint32_t* flags, uint32_t* flags2);
 
 // launchers (hl_shade.hip)
-// kModeExtended: kModeForward with shadow_factor on the shadow-casting lights
-// (ShadeArgs.shadow.layers != 0) and the radiance-cascade indirect diffuse
-// (ShadeArgs.rc_fmt != 0)
-enum ShadeMode : int { kModeForward = 0, kModeCount = 1, kModeBackward = 2, kModeFusedL2 = 3, kModeExtended = 4 };
+// kModeShadowed: kModeForward with shadow_factor on the shadow-casting lights
+// (ShadeArgs.shadow.layers != 0); kModeExtended: kModeShadowed plus the
+// radiance-cascade indirect diffuse (ShadeArgs.rc_fmt != 0), kept apart so
+// that the shadow path keeps its occupancy
+enum ShadeMode : int {
+    kModeForward = 0,
+    kModeCount = 1,
+    kModeBackward = 2,
+    kModeFusedL2 = 3,
+    kModeShadowed = 4,
+    kModeExtended = 5
+};
 // Claim counters of one persistent shading launch: kShadeQueues counters,
 // kShadeQueueStride words apart, zeroed beforehand (launch_fill_u32).
 constexpr uint32_t kShadeQueues = 64;

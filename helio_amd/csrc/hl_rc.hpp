@@ -78,22 +78,49 @@ __device__ __forceinline__ float3 rc_irradiance(const ShadeArgs& a, float wx, fl
 #pragma clang fp contract(off)
         return p * (1.0f - t) + q * t;
     };
-    // rc_corner_irradiance_precomp (:842-864) of corner (x, y, z) offsets
+    // rc_corner_irradiance_precomp (:842-864) of corner (x, y, z) offsets; the
+    // sum runs over direction k = dx * 4 + dy in the shader's order
     const auto corner = [&](uint32_t ox, uint32_t oy, uint32_t oz) {
 #pragma clang fp contract(off)
         const uint32_t dim = kRcProbeDim - 1u;
         const uint32_t cx = min(px + ox, dim), cy = min(py + oy, dim), cz = min(pz + oz, dim);
+        const uint32_t row0 = (cy * kRcProbeDim + cz) * kRcDirDim;
         float ir = 0.0f, ig = 0.0f, ib = 0.0f, ws = 0.0f;
-#pragma unroll
-        for (uint32_t k = 0; k < 16u; ++k) {
+        const auto add = [&](uint32_t k, float r, float g, float b) {
             const float w = cw[k];
             if (w > 0.001f) {
-                const float3 t = rc_texel(a.rc, a.rc_fmt, cx * kRcDirDim + (k >> 2),
-                                          (cy * kRcProbeDim + cz) * kRcDirDim + (k & 3u));
-                ir = ir + t.x * w;
-                ig = ig + t.y * w;
-                ib = ib + t.z * w;
+                ir = ir + r * w;
+                ig = ig + g * w;
+                ib = ib + b * w;
                 ws = ws + w;
+            }
+        };
+        if (a.rc_fmt == HL_FMT_RGBA16_FLOAT) {
+            // the block's rows are 4 texels (32 B): two 16-B loads per row,
+            // columns dx = 2h, 2h + 1 in the load of half h
+#pragma unroll
+            for (uint32_t h = 0; h < 2u; ++h) {
+                hl_u32x4 q[4];
+#pragma unroll
+                for (uint32_t dy = 0; dy < 4u; ++dy)
+                    q[dy] = *hl_g(reinterpret_cast<const hl_u32x4*>(a.rc.data + (size_t)(row0 + dy) * a.rc.pitch) +
+                                  (cx * 2u + h));
+#pragma unroll
+                for (uint32_t dy = 0; dy < 4u; ++dy)
+                    add((2u * h) * 4u + dy, hl_half((uint16_t)(q[dy].x & 0xffffu)), hl_half((uint16_t)(q[dy].x >> 16)),
+                        hl_half((uint16_t)(q[dy].y & 0xffffu)));
+#pragma unroll
+                for (uint32_t dy = 0; dy < 4u; ++dy)
+                    add((2u * h + 1u) * 4u + dy, hl_half((uint16_t)(q[dy].z & 0xffffu)),
+                        hl_half((uint16_t)(q[dy].z >> 16)), hl_half((uint16_t)(q[dy].w & 0xffffu)));
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < 16u; ++k) {
+                if (cw[k] > 0.001f) {
+                    const float3 t = rc_texel(a.rc, a.rc_fmt, cx * kRcDirDim + (k >> 2), row0 + (k & 3u));
+                    add(k, t.x, t.y, t.z);
+                }
             }
         }
         const float d = fmaxf(ws, 0.001f);

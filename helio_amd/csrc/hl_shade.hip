@@ -152,7 +152,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
     const bool valid = inside && depth < 1.0f;
     constexpr bool kAdj = MODE == kModeBackward || MODE == kModeFusedL2;
-    constexpr bool kExt = MODE == kModeExtended;
+    constexpr bool kRC = MODE == kModeExtended;
+    constexpr bool kExt = MODE == kModeShadowed || kRC;  // exact N, shadow_factor
     constexpr bool kFwd = MODE == kModeForward || kExt;
     constexpr bool kWrites = kFwd || MODE == kModeFusedL2;
     float4* out_px = (kWrites && inside)
@@ -339,7 +340,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         mixv[c] = sky * 0.15f * (1.0f - ht) + sky * ht;
         amb[c] = mixv[c] * p.alb[c] * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
     }
-    if (kExt && a.rc_fmt != 0u && valid) {
+    if (kRC && a.rc_fmt != 0u && valid) {
         // radiance-cascade indirect diffuse (deferred_lighting.wgsl:1159-1163,
         // 1189-1205): mix(hemisphere, kD_ibl rc_irr albedo, clamp(4 |rc_irr|, 0, 1))
         const float3 rc = rc_irradiance(a, p.wx, p.wy, p.wz, p.nx, p.ny, p.nz);
@@ -504,7 +505,7 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
 // Forward launches are held to 96 VGPRs (5 waves per SIMD): the tile's
 // memory round trips need the waves to hide them.
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeExtended ? 3 : 1)) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeShadowed ? 4 : (MODE == kModeExtended ? 3 : 1))) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
                                                uint32_t* __restrict__ queue_next) {
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
@@ -630,6 +631,7 @@ hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& 
         case kModeCount: return launch_fmt<kModeCount>(s, gbuf_set, a, queue, queue_next);
         case kModeBackward: return launch_fmt<kModeBackward>(s, gbuf_set, a, queue, queue_next);
         case kModeFusedL2: return launch_fmt<kModeFusedL2>(s, gbuf_set, a, queue, queue_next);
+        case kModeShadowed: return launch_fmt<kModeShadowed>(s, gbuf_set, a, queue, queue_next);
         case kModeExtended: return launch_fmt<kModeExtended>(s, gbuf_set, a, queue, queue_next);
         default: return hipErrorInvalidValue;
     }

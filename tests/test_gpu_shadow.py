@@ -129,4 +129,4 @@ def test_rc_gi_matches_oracle(ctx, hlib, name, mode, fmt, shadows):
     r = check_radiance(out, ref, ~sky)
     changed = np.mean(np.any(ref[~sky][:, :3] != plain[~sky][:, :3], axis=1))
     print(f"  max rel err {r:.2e}, pixels with RC ambient {changed:.2f}")
-    assert changed > 0.2
+    assert changed > 0.1
