@@ -29,6 +29,9 @@ Further legs, each reported in its own object of the same line:
                   raster order) under the same 16384 lights read through
                   entity indices and transforms, on the reference's 16-px tile
                   lists (cap 64) and on the 32x18x48 cluster lists
+  reflection_4k   the pass's second draw (hl_deferred_reflection) over the 4K
+                  G-buffer: 16 reflection captures over 17 RGBA16F cubes of
+                  128^2 x 8 mips, SSR and planar images, RC rough specular
   multiview       (opt-in, --multiview) 48 probe faces (16 each at 128^2,
                   256^2, 512^2) + 2 XR eyes at 2064x2208 over an 8192-light
                   pool, adjoint on the eyes: one batched frame (autotuned, graph
@@ -258,6 +261,35 @@ def bench_shadowed(ctx, args):
     return res
 
 
+def bench_reflection(ctx, args):
+    """The pass's second draw (hl_deferred_reflection) over the 4K/16k frame:
+    16 captures (8 oriented boxes, 8 spheres, one not resident) over a cube
+    array of 17 RGBA16F cubes of 128^2 with 8 mips, full-resolution SSR and
+    planar images, and the radiance-cascade rough-specular fallback."""
+    from helio_amd import abi, synth
+    from helio_amd.device import DeviceReflections, ReflectionPass, upload_scene
+    sc = synth.make_scene(args.workload)
+    refl = synth.make_reflections(sc, n_box=8, n_sphere=8, face=128, mips=8, rc=True)
+    ds = upload_scene(ctx, sc)
+    dr = DeviceReflections(ctx, refl)
+    out = ctx.alloc(sc.width * sc.height * 16)
+    out.fill_u32(0)
+    rp = ReflectionPass(ctx)
+    ms, _ = time_steps(ctx, args.steps, args.warmup, lambda: rp.execute(ds, dr, out))
+    shaded = int((sc.depth < 1.0).sum())
+    texel = sc.texel[0] + 2 * sc.texel[1]  # orm, normal, emissive
+    # per shaded pixel: depth, G-buffer texels, SSR and planar (RGBA16F), output read and written;
+    # per sky pixel: depth.  The cube array (17.8 MB) stays in the caches.
+    nbytes = shaded * (4 + texel + 8 + 8 + 32) + (sc.width * sc.height - shaded) * 4
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"reflection 4K: {ms * 1e3:.1f} us, {gbs:.0f} GB/s")
+    return {"ms_per_pass": ms, "bytes_per_pass": nbytes, "achieved_GBs": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
+            "cube_array_bytes": int(refl.cubes.nbytes),
+            "workload": f"{args.workload} G-buffer; 16 reflection captures (8 box, 8 sphere, 1 without a cube), "
+                        "17 RGBA16F cubes of 128^2 x 8 mips, RGBA16F SSR + planar, RC rough specular; "
+                        "hl_deferred_reflection added over the frame"}
+
+
 def bench_forward_lit(ctx, args):
     """hl_forward_lit over every surface of the 4K/16k scene as one fragment
     (80-byte records), lights positioned through entity indices and
@@ -397,6 +429,7 @@ def main() -> int:
     ap.add_argument("--no-pack", action="store_true", help="skip the 4K G-buffer pack leg")
     ap.add_argument("--no-forward", action="store_true", help="skip the 4K forward-lit leg")
     ap.add_argument("--no-shadows", action="store_true", help="skip the 4K shadowed leg")
+    ap.add_argument("--no-reflection", action="store_true", help="skip the 4K reflection-composite leg")
     ap.add_argument("--multiview", action="store_true",
                     help="also run the full-size 50-view batched leg (opt-in: its autotuned plan hit a GPU "
                          "memory fault once, cause not yet found)")
@@ -431,6 +464,7 @@ def main() -> int:
         pack = None if args.no_pack else bench_gbuffer_pack(ctx, args)
         fwd = None if args.no_forward else bench_forward_lit(ctx, args)
         shd = None if args.no_shadows else bench_shadowed(ctx, args)
+        rfl = None if args.no_reflection else bench_reflection(ctx, args)
         mv = bench_multiview(ctx, args) if args.multiview else None
 
     sc, q = r4k["scene"], r4k["q"]
@@ -491,6 +525,8 @@ def main() -> int:
         res["forward_lit_4k"] = fwd
     if shd is not None:
         res["shadowed_4k"] = shd
+    if rfl is not None:
+        res["reflection_4k"] = rfl
     res["multiview"] = mv if mv is not None else {
         "skipped": "full-size 50-view leg is opt-in (--multiview): its autotuned plan faulted the GPU once "
                    "and stays off the GPU until the cause is found; the 1/8-scale 50-view plan is "

@@ -114,6 +114,31 @@ class hl_texture(C.Structure):
     _fields_ = [("data_d", C.c_void_p), ("pitch_bytes", C.c_uint32), ("format", C.c_uint32)]
 
 
+HL_CAPTURE_SHAPE_SPHERE, HL_CAPTURE_SHAPE_BOX = 0, 1
+HL_REFLECTION_STORE = 1
+
+
+class GpuReflectionCapture(C.Structure):
+    _fields_ = [("position_radius", C.c_float * 4), ("extents_transition", C.c_float * 4),
+                ("world_to_local", C.c_float * 16), ("cubemap_index", C.c_int32), ("shape", C.c_uint32),
+                ("mobility", C.c_uint32), ("brightness", C.c_float)]
+
+
+CAPTURE_DTYPE = np.dtype([("position_radius", "<f4", 4), ("extents_transition", "<f4", 4),
+                          ("world_to_local", "<f4", 16), ("cubemap_index", "<i4"), ("shape", "<u4"),
+                          ("mobility", "<u4"), ("brightness", "<f4")])
+
+
+class hl_cube_array(C.Structure):
+    _fields_ = [("data_d", C.c_void_p), ("face_size", C.c_uint32), ("mip_levels", C.c_uint32),
+                ("num_cubes", C.c_uint32), ("format", C.c_uint32)]
+
+
+class hl_reflection_inputs(C.Structure):
+    _fields_ = [("env", hl_cube_array), ("captures_d", C.c_void_p), ("ssr", hl_texture), ("planar", hl_texture),
+                ("rc_cascade0", hl_texture)]
+
+
 class hl_gbuffer(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("albedo", hl_texture),
                 ("normal", hl_texture), ("orm", hl_texture), ("emissive", hl_texture),
@@ -171,6 +196,8 @@ def check_layouts() -> None:
     assert C.sizeof(hl_transform) == 36 == TRANSFORM_DTYPE.itemsize
     assert C.sizeof(hl_fragment) == 80 == FRAGMENT_DTYPE.itemsize
     assert C.sizeof(ShadowConfig) == 80
+    assert C.sizeof(GpuReflectionCapture) == 112 == CAPTURE_DTYPE.itemsize
+    assert C.sizeof(hl_cube_array) == 24
 
 
 check_layouts()
@@ -220,6 +247,8 @@ SIGNATURES: dict[str, tuple] = {
                                             P(hl_light_grid), P(hl_shadow_maps), vp, u32, u32]),
     "hl_deferred_forward_ex": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32, P(hl_light_grid),
                                       P(hl_shadow_maps), P(hl_texture), vp, u32, u32]),
+    "hl_deferred_reflection": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), P(hl_reflection_inputs), vp,
+                                       u32, u32]),
     "hl_count_pixel_light_pairs": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                           P(hl_light_grid), P(C.c_uint64)]),
     "hl_forward_lit": (None, [vp, vp, P(ForwardLitGlobals), vp, u32, vp, u32, vp, vp, u32,

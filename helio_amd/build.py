@@ -22,7 +22,7 @@ CSRC = PKG / "csrc"
 ORACLE = ROOT / "oracle"
 
 HIP_SOURCES = ["hl_api.hip", "hl_cull.hip", "hl_shade.hip", "hl_batch.hip", "hl_gbuffer.hip",
-               "hl_forward.hip"]
+               "hl_forward.hip", "hl_reflect.hip"]
 ARCH = os.environ.get("HELIO_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -93,7 +93,8 @@ def build_oracle() -> Path:
     out = ORACLE / "liboracle.so"
     _run(["gcc", *CC_FLAGS, f"-I{ROOT / 'include'}", str(ORACLE / "oracle.c"),
           str(ORACLE / "oracle64.c"), str(ORACLE / "gbuffer_pack.c"),
-          str(ORACLE / "forward_lit.c"), str(ORACLE / "shadow.c"), str(ORACLE / "rc_gi.c"), "-o", str(out), "-lm"])
+          str(ORACLE / "forward_lit.c"), str(ORACLE / "shadow.c"), str(ORACLE / "rc_gi.c"),
+          str(ORACLE / "reflection.c"), "-o", str(out), "-lm"])
     return out
 
 

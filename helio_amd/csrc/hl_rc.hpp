@@ -53,8 +53,11 @@ __device__ __forceinline__ float3 rc_texel(const Tex& t, uint32_t fmt, uint32_t 
 }
 
 // sample_rc_irradiance (:866-904) at world position (wx, wy, wz) with unit
-// normal (nx, ny, nz); the caller checks has_rc_gi.
-__device__ __forceinline__ float3 rc_irradiance(const ShadeArgs& a, float wx, float wy, float wz, float nx, float ny,
+// normal (nx, ny, nz); the caller checks has_rc_gi.  sample_rc_specular
+// (:929-983) is the same body along its lookup direction.  A: any launch
+// record with the atlas (rc, rc_fmt) and its box (rc_min, rc_size).
+template <class A>
+__device__ __forceinline__ float3 rc_irradiance(const A& a, float wx, float wy, float wz, float nx, float ny,
                                                 float nz) {
 #pragma clang fp contract(off)
     const float sx = a.rc_size[0], sy = a.rc_size[1], sz = a.rc_size[2];

@@ -187,6 +187,40 @@ class DeviceTexture:
         self.desc = t
 
 
+class DeviceReflections:
+    """hl_deferred_reflection's inputs resident in HBM (from a synth.HostReflections)."""
+
+    def __init__(self, ctx: Context, host):
+        self.host = host
+        self.cubes = ctx.upload(host.cubes)
+        self.caps = ctx.upload(host.captures) if len(host.captures) else None
+        self.tex = {}
+        r = abi.hl_reflection_inputs()
+        r.env = host.env_host()
+        r.env.data_d = self.cubes.ptr
+        r.captures_d = self.caps.ptr if self.caps is not None else None
+        for name, img in (("ssr", host.ssr), ("planar", host.planar), ("rc_cascade0", host.rc)):
+            if img is not None:
+                self.tex[name] = DeviceTexture(ctx, img)
+                setattr(r, name, self.tex[name].desc)
+        self.inputs = r
+
+
+class ReflectionPass:
+    """The deferred pass's second draw (fs_reflection, deferred_lighting.wgsl:1266-1341)
+    over a frame that DeferredLightPass lit: adds the indirect specular composite."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def execute(self, scene: DeviceScene, refl: DeviceReflections, out: DeviceArray, flags: int = 0,
+                globals_: abi.DeferredGlobals | None = None) -> None:
+        g = globals_ if globals_ is not None else scene.host.globals
+        self.ctx.lib.call("hl_deferred_reflection", self.ctx.h, scene.camera.ptr, C.byref(g),
+                          C.byref(scene.gbuffer), C.byref(refl.inputs), out.ptr, scene.host.width * 16, flags,
+                          ctx=self.ctx.h)
+
+
 class TileGrid:
     """tile_light_lists / tile_light_counts (crates/libhelio/src/frame.rs:230-234)."""
 

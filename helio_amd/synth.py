@@ -550,3 +550,147 @@ def rc_texture_host(atlas: np.ndarray) -> abi.hl_texture:
     t.data_d, t.pitch_bytes = atlas.ctypes.data, atlas.shape[1] * atlas.itemsize * 4
     t.format = abi.HL_FMT_RGBA16_FLOAT if atlas.dtype == np.float16 else abi.HL_FMT_RGBA32_FLOAT
     return t
+
+
+# ------------------------------------------------------------- reflections
+def cube_dirs(res: int) -> np.ndarray:
+    """Unit directions through the texel centres of the six faces (+X -X +Y
+    -Y +Z -Z) of a res x res cube, [6, res, res, 3]: the inverse of the
+    sampler's face table (sc, tc on the major axis)."""
+    c = (np.arange(res) + 0.5) / res * 2.0 - 1.0
+    tc, sc = np.meshgrid(c, c, indexing="ij")  # row: tc, column: sc
+    one = np.ones_like(sc)
+    faces = [np.stack([one, -tc, -sc], -1), np.stack([-one, -tc, sc], -1),
+             np.stack([sc, one, tc], -1), np.stack([sc, -one, -tc], -1),
+             np.stack([sc, -tc, one], -1), np.stack([-sc, -tc, -one], -1)]
+    d = np.stack(faces)
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+@dataclass
+class HostReflections:
+    """Inputs of hl_deferred_reflection on the host: the cube array (cube 0
+    the skylight, cube i + 1 capture i's), the capture records and the
+    optional screen textures."""
+    cubes: np.ndarray
+    face_size: int
+    mip_levels: int
+    num_cubes: int
+    format: int
+    captures: np.ndarray
+    ssr: np.ndarray | None = None
+    planar: np.ndarray | None = None
+    rc: np.ndarray | None = None
+
+    def env_host(self) -> abi.hl_cube_array:
+        e = abi.hl_cube_array()
+        e.data_d, e.face_size, e.mip_levels = self.cubes.ctypes.data, self.face_size, self.mip_levels
+        e.num_cubes, e.format = self.num_cubes, self.format
+        return e
+
+    def inputs_host(self) -> abi.hl_reflection_inputs:
+        r = abi.hl_reflection_inputs()
+        r.env = self.env_host()
+        r.captures_d = self.captures.ctypes.data if len(self.captures) else None
+        for name, img in (("ssr", self.ssr), ("planar", self.planar), ("rc_cascade0", self.rc)):
+            if img is not None:
+                t = getattr(r, name)
+                t.data_d, t.pitch_bytes = img.ctypes.data, img.shape[1] * img.itemsize * 4
+                t.format = abi.HL_FMT_RGBA16_FLOAT if img.dtype == np.float16 else abi.HL_FMT_RGBA32_FLOAT
+        return r
+
+
+def make_cube_array(num_cubes: int, face: int, mips: int, fmt: int = abi.HL_FMT_RGBA16_FLOAT,
+                    seed: int = 21) -> np.ndarray:
+    """A pre-filtered-looking cube array: cube c mip m holds a smooth positive
+    function of the texel direction, its own tint and a per-mip fade, in the
+    layout hl_cube_array documents (cubes, then mips, then faces, then rows)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for c in range(num_cubes):
+        tint = rng.uniform(0.3, 1.5, size=3)
+        k = rng.uniform(1.0, 3.0, size=3)
+        ph = rng.uniform(0.0, 6.28, size=3)
+        for m in range(mips):
+            r = max(1, face >> m)
+            d = cube_dirs(r)
+            val = np.empty((6, r, r, 4))
+            for ch in range(3):
+                val[..., ch] = tint[ch] * (0.55 + 0.45 * np.sin(k[ch] * d[..., ch] + d[..., (ch + 1) % 3] + ph[ch])) \
+                    * (1.0 - 0.06 * m)
+            val[..., 3] = 1.0
+            parts.append(val.reshape(-1, 4))
+    arr = np.concatenate(parts)
+    if fmt == abi.HL_FMT_RGBA8_UNORM:
+        return np.ascontiguousarray(np.clip(np.rint(np.clip(arr, 0, 1) * 255.0), 0, 255).astype(np.uint8))
+    return np.ascontiguousarray(arr.astype(np.float16 if fmt == abi.HL_FMT_RGBA16_FLOAT else np.float32))
+
+
+def make_reflections(scene: HostScene, n_box: int = 4, n_sphere: int = 4, face: int = 64, mips: int = 7,
+                     fmt: int = abi.HL_FMT_RGBA16_FLOAT, ssr: bool = True, planar: bool = True,
+                     rc: bool = False, seed: int = 23) -> HostReflections:
+    """Reflection captures around visible points of the scene (boxes with a
+    random yaw, spheres; one capture without a cubemap), sorted by influence
+    size smallest first (crates/libhelio/src/reflection.rs:46-49), their cube
+    array, an SSR image whose confidence is zero over part of the frame, a
+    planar image on up-facing pixels, and (rc) the radiance-cascade atlas.
+    Switches the scene's globals to reflections on."""
+    rng = np.random.default_rng(seed)
+    H, W = scene.height, scene.width
+    cam = scene.camera
+    ivp = np.array(cam.inv_view_proj, np.float64).reshape(4, 4).T  # column-major storage
+    ys, xs = np.nonzero(scene.depth < 1.0)
+    pick = rng.choice(len(xs), size=n_box + n_sphere, replace=False)
+    caps = np.zeros(n_box + n_sphere, abi.CAPTURE_DTYPE)
+    for i, j in enumerate(pick):
+        x, y = xs[j], ys[j]
+        ndc = np.array([(x + 0.5) / W * 2 - 1, 1 - (y + 0.5) / H * 2, float(scene.depth[y, x]), 1.0])
+        h = ivp @ ndc
+        p = h[:3] / h[3]
+        c = caps[i]
+        c["cubemap_index"] = i + 1
+        c["brightness"] = rng.uniform(0.6, 1.4)
+        if i < n_box:
+            yaw = rng.uniform(0, 2 * np.pi)
+            rot = np.array([[np.cos(yaw), 0, np.sin(yaw)], [0, 1, 0], [-np.sin(yaw), 0, np.cos(yaw)]])
+            ext = rng.uniform((3.0, 2.0, 3.0), (10.0, 6.0, 10.0))
+            m = np.eye(4)
+            m[:3, :3] = rot.T
+            m[:3, 3] = -rot.T @ p
+            c["position_radius"] = (*p, float(np.linalg.norm(ext)))
+            c["extents_transition"] = (*ext, rng.uniform(0.5, 2.5))
+            c["world_to_local"] = m.T.ravel()  # column-major
+            c["shape"] = abi.HL_CAPTURE_SHAPE_BOX
+        else:
+            c["position_radius"] = (*p, rng.uniform(4.0, 14.0))
+            c["world_to_local"] = np.eye(4).ravel()
+            c["shape"] = abi.HL_CAPTURE_SHAPE_SPHERE
+    size = np.where(caps["shape"] == abi.HL_CAPTURE_SHAPE_BOX,
+                    np.linalg.norm(caps["extents_transition"][:, :3], axis=1), caps["position_radius"][:, 3])
+    caps = caps[np.argsort(size, kind="stable")]
+    if len(caps) > 2:
+        caps[len(caps) // 2]["cubemap_index"] = -1  # not resident yet
+    num_cubes = n_box + n_sphere + 1
+    cubes = make_cube_array(num_cubes, face, mips, fmt, seed=seed + 1)
+    g = scene.globals
+    g.reflection_capture_count = len(caps)
+    g.enable_reflections = 1
+    g.enable_env_reflections = 1
+    s_img = p_img = None
+    if ssr:
+        yy, xx = np.mgrid[0:H, 0:W]
+        a = np.clip(np.sin(xx * 0.013) * np.cos(yy * 0.017) * 1.5, 0.0, 1.0)
+        s_img = np.empty((H, W, 4), np.float32)
+        s_img[..., :3] = rng.uniform(0.0, 2.0, size=(H, W, 3))
+        s_img[..., 3] = a
+        s_img = s_img.astype(np.float16)
+    if planar:
+        n = scene.normal.view(np.float16 if scene.formats[2] == abi.HL_FMT_RGBA16_FLOAT else np.float32)
+        ny = n.reshape(H, W, 4)[..., 1].astype(np.float32)
+        p_img = np.zeros((H, W, 4), np.float32)
+        up = ny > 0.9
+        p_img[up, :3] = rng.uniform(0.0, 1.5, size=(int(up.sum()), 3))
+        p_img[up, 3] = 0.6
+        p_img = p_img.astype(np.float16)
+    rc_atlas = make_rc_probes(scene) if rc else None
+    return HostReflections(cubes, face, mips, num_cubes, fmt, caps, s_img, p_img, rc_atlas)

@@ -323,6 +323,66 @@ HL_API hl_status hl_deferred_forward_ex(hl_context* ctx, const GpuCameraUniforms
                                         const hl_texture* rc_cascade0, float* out_d, uint32_t out_pitch_bytes,
                                         uint32_t flags);
 
+/* ------------------------------------------------------------ reflections */
+/* Pre-filtered reflection cube array (the baked cube array of
+ * crates/helio-bake/src/data.rs:300-365, bound as env_cube at
+ * deferred_lighting.wgsl:167).  Cube c holds mip_levels mips, mip m of
+ * res_m = max(1, face_size >> m) texels per side; within a cube the mips follow
+ * one another and within a mip the faces +X -X +Y -Y +Z -Z, each res_m rows of
+ * res_m tightly packed texels.  Cube c starts cube_bytes(face_size, mip_levels,
+ * format) * c bytes after data_d.  data_d NULL is the 1x1 black fallback cube
+ * array (every sample 0). */
+typedef struct hl_cube_array {
+    const void* data_d;
+    uint32_t face_size;   /* 1 .. 16384 */
+    uint32_t mip_levels;  /* 1 .. 1 + log2(face_size) */
+    uint32_t num_cubes;
+    uint32_t format;      /* HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA16_FLOAT or HL_FMT_RGBA8_UNORM */
+} hl_cube_array;
+
+/* Inputs of the reflection composite besides the G-buffer.  Every texture is
+ * optional (data_d NULL binds the reference's black fallback); ssr and planar
+ * are full-resolution RGBA16_FLOAT or RGBA32_FLOAT (a = hit confidence);
+ * rc_cascade0 is the probe atlas of hl_deferred_forward_ex. */
+typedef struct hl_reflection_inputs {
+    hl_cube_array env;
+    const GpuReflectionCapture* captures_d; /* min(globals->reflection_capture_count, 64) records */
+    hl_texture ssr;
+    hl_texture planar;
+    hl_texture rc_cascade0;
+} hl_reflection_inputs;
+
+enum {
+    HL_REFLECTION_STORE = 1u  /* write (contribution, 0) instead of adding it (the default, the
+                                 pass's One + One blend over fs_main's output) */
+};
+
+/* The deferred pass's second draw, fs_reflection (deferred_lighting.wgsl:
+ * 1266-1341; drawn at crates/passes/3d/helio-pass-deferred-light/src/lib.rs:
+ * 1188-1197 with the One + One colour blend of :464-478, alpha kept): per
+ * non-sky pixel,
+ *   env = sample_reflection_environment(P, R, roughness * ENV_MAX_LOD) (:646-691:
+ *         the captures front to back with capture_weight (:619-639) and box or
+ *         sphere parallax (:577-615), layer 0 as the skylight for what is left)
+ *         times F0 * env_brdf.x + env_brdf.y (pbr_eval.wgsl:70-76),
+ *   then mixed toward ssr.rgb * F_ibl by ssr.a and planar.rgb * F_ibl by planar.a
+ *   (enable_reflections), and toward sample_rc_specular (:929-983) by
+ *   smoothstep(0.6, 0.9, roughness) * 0.4 where has_rc_gi and roughness > 0.6,
+ *   times ao (the screen-space AO input is its fallback, 1);
+ * out.rgb += that, or out = (that, 0) with HL_REFLECTION_STORE.  The cube
+ * sampler (linear, linear mips, clamp to edge; data.rs:372-380) is evaluated
+ * with FP32 bilinear and mip weights on the face that the direction's major
+ * axis selects (no filtering across faces); the cube index is clamped to
+ * num_cubes - 1.  debug_mode 30 and 31 write the SSR diagnostics (a, a, a, 1)
+ * and (rgb, 1) (:1272-1277) in place (the debug pipeline has no blend, lib.rs:489-510); 1, 2, 4, 5, 10, 11, 20 and 21 add nothing
+ * (:1280-1285).  Sky pixels (depth >= 1) are left untouched.  Same G-buffer
+ * rules as hl_deferred_forward (lightmap_uv NULL, so the contribution is
+ * multiplied by ao). */
+HL_API hl_status hl_deferred_reflection(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                        const DeferredGlobals* globals, const hl_gbuffer* gbuffer,
+                                        const hl_reflection_inputs* inputs, float* out_d,
+                                        uint32_t out_pitch_bytes, uint32_t flags);
+
 /* Counts Q, the (pixel, light) pairs the forward pass evaluates past the range
  * test (type != 0 implies dist <= range) -- the "pixel-light evals" of the
  * throughput metric.  Same inputs as hl_deferred_forward; writes *pairs (host). */

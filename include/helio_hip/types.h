@@ -162,6 +162,20 @@ typedef struct ShadowConfig {
     uint32_t pcf_sample_count;     /* 76 */
 } ShadowConfig;
 
+/* One reflection capture (crates/libhelio/src/reflection.rs:64-80; WGSL
+ * deferred_lighting.wgsl:189-198), 112 bytes.  world_to_local is column-major
+ * (WGSL mat4x4: column c at [4c .. 4c+3]). */
+enum { HL_CAPTURE_SHAPE_SPHERE = 0, HL_CAPTURE_SHAPE_BOX = 1 };
+typedef struct GpuReflectionCapture {
+    float position_radius[4];    /*  0: xyz world position, w influence radius   */
+    float extents_transition[4]; /* 16: xyz local half-extents (box), w transition */
+    float world_to_local[16];    /* 32: box parallax; identity for spheres         */
+    int32_t cubemap_index;       /* 96: cube of the array, -1 = none resident      */
+    uint32_t shape;              /* 100: HL_CAPTURE_SHAPE_*                         */
+    uint32_t mobility;           /* 104: 0 static, 1 dynamic (not read)            */
+    float brightness;            /* 108                                             */
+} GpuReflectionCapture;
+
 #ifdef __cplusplus
 #define HL_STATIC_ASSERT static_assert
 #else
@@ -212,6 +226,8 @@ HL_STATIC_ASSERT(sizeof(CascadeConfig) == 16, "CascadeConfig must be 16 bytes");
 HL_STATIC_ASSERT(sizeof(ShadowConfig) == 80, "ShadowConfig must be 80 bytes (shadow.rs:49-63)");
 HL_STATIC_ASSERT(offsetof(ShadowConfig, pcf_sample_count) == 76, "ShadowConfig.pcf_sample_count");
 HL_STATIC_ASSERT(offsetof(hl_transform, rotation) == 12, "transform.rotation");
+HL_STATIC_ASSERT(sizeof(GpuReflectionCapture) == 112, "GpuReflectionCapture must be 112 bytes");
+HL_STATIC_ASSERT(offsetof(GpuReflectionCapture, cubemap_index) == 96, "capture.cubemap_index");
 
 #ifdef __cplusplus
 }

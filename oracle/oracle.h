@@ -175,6 +175,19 @@ uint64_t or_shade_ex(const GpuCameraUniforms* cam, const DeferredGlobals* g, con
 void or_rc_oct_decode(float u, float v, float out[3]);
 void or_rc_irradiance(const hl_texture* t, const float wmin[3], const float wmax[3], const float wp[3],
                       const float N[3], float out[3]);
+
+/* reflection.c: fs_reflection (deferred_lighting.wgsl:1266-1341) and its parts */
+size_t or_cube_bytes(uint32_t face_size, uint32_t mip_levels, uint32_t format);
+void or_cube_face(const float d[3], uint32_t* face, float* u, float* v);
+void or_cube_sample(const hl_cube_array* env, uint32_t cube, const float dir[3], float lod, float out[3]);
+float or_capture_weight(const float P[3], const GpuReflectionCapture* cap);
+void or_capture_parallax(const float P[3], const float R[3], const GpuReflectionCapture* cap, float out[3]);
+void or_reflection_env(const DeferredGlobals* g, const hl_reflection_inputs* in, const float P[3], const float R[3],
+                       float lod, float out[3]);
+void or_env_brdf_approx(float ndv, float roughness, float out[2]);
+uint64_t or_reflection(const GpuCameraUniforms* cam, const DeferredGlobals* g, const hl_gbuffer* gb,
+                       const hl_reflection_inputs* in, float* out, uint32_t out_pitch, uint32_t flags,
+                       uint32_t row0, uint32_t row1);
 float or_shadow_factor(const hl_shadow_maps* sh, const GpuLight* lights, uint32_t light_idx, const float cam_pos[3],
                        const float csm_splits[4], const float wp[3], const float N[3], float fx, float fy);
 float or_sample_cascade_shadow(const hl_shadow_maps* sh, uint32_t layer, uint32_t cascade, float scale,

@@ -63,6 +63,16 @@ _SIGS = {
                                            vp, u32, P(abi.hl_light_grid), u32, u32, P(C.c_uint64)]),
     "or_pbr_direct_light": (None, [vp, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
                                    C.c_float, C.c_float, P(C.c_float)]),
+    "or_cube_bytes": (C.c_size_t, [u32, u32, u32]),
+    "or_cube_face": (None, [P(C.c_float), P(u32), P(C.c_float), P(C.c_float)]),
+    "or_cube_sample": (None, [P(abi.hl_cube_array), u32, P(C.c_float), C.c_float, P(C.c_float)]),
+    "or_capture_weight": (C.c_float, [P(C.c_float), P(abi.GpuReflectionCapture)]),
+    "or_capture_parallax": (None, [P(C.c_float), P(C.c_float), P(abi.GpuReflectionCapture), P(C.c_float)]),
+    "or_reflection_env": (None, [P(abi.DeferredGlobals), P(abi.hl_reflection_inputs), P(C.c_float), P(C.c_float),
+                                 C.c_float, P(C.c_float)]),
+    "or_env_brdf_approx": (None, [C.c_float, C.c_float, P(C.c_float)]),
+    "or_reflection": (C.c_uint64, [P(abi.GpuCameraUniforms), P(abi.DeferredGlobals), P(abi.hl_gbuffer),
+                                   P(abi.hl_reflection_inputs), vp, u32, u32, u32, u32]),
     "or64_decode": (None, [P(abi.GpuCameraUniforms), P(abi.hl_gbuffer), vp]),
     "or64_shade_pixel": (C.c_uint64, [P(or64_pixel), P(abi.DeferredGlobals), vp, u32, P(C.c_double)]),
     "or64_forward": (C.c_double, [vp, u32, u32, P(abi.DeferredGlobals), vp, u32, vp, vp, u32]),
@@ -154,6 +164,22 @@ def shade(scene, grid, globals_=None, flags=0, rows=None, out=None, shadows=None
                                  scene.lights.ctypes.data if len(scene.lights) else None, len(scene.lights),
                                  C.byref(grid), sh, rt, out.ctypes.data, W * 16, flags, r0, r1)
     return out, pairs
+
+
+def reflection(scene, refl, out=None, flags=0, rows=None, globals_=None):
+    """FP32 restatement of fs_reflection over the frame out (added to it, or
+    stored with abi.HL_REFLECTION_STORE); refl: a synth.HostReflections.
+    Returns (out, pixels written)."""
+    W, H = scene.width, scene.height
+    if out is None:
+        out = np.zeros((H, W, 4), dtype=np.float32)
+    g = globals_ if globals_ is not None else scene.globals
+    gb = scene.gbuffer_host()
+    r0, r1 = rows if rows else (0, H)
+    inp = refl.inputs_host()
+    n = oracle().or_reflection(C.byref(scene.camera), C.byref(g), C.byref(gb), C.byref(inp), out.ctypes.data,
+                               W * 16, flags, r0, r1)
+    return out, n
 
 
 def rc_texture_of(atlas):
