@@ -5,9 +5,11 @@
 // reflections and the radiance-cascade rough-specular fallback, added to the
 // lit frame.
 //
-// One thread per pixel, a 16 x 16-pixel block of four 16 x 4 wavefronts.  The
-// capture records are read with scalar loads (the capture index is uniform);
-// a lane leaves the capture loop once its coverage saturates.  The cube
+// One thread per pixel, a 16 x 16-pixel block of four 16 x 4 wavefronts.  A
+// wave first finds, one capture per lane, the captures whose influence volume
+// can reach the bounding box of its pixels' world positions (a conservative
+// test), then walks only those, in order, with the records read by scalar
+// loads; a lane leaves the walk once its coverage saturates.  The cube
 // sampler is evaluated in FP32 exactly as the restatement defines it (face of
 // the major axis, bilinear weights on uv * size - 0.5 clamped to the face's
 // edge, linear mips), so that, with the IEEE division and square root, the
@@ -30,7 +32,7 @@ struct ReflectArgs {
     const GpuReflectionCapture* caps;
     const uint8_t* env;  // null: the black fallback cube array
     uint32_t env_fmt, env_face, env_mips, env_cubes;
-    size_t cube_bytes;
+    uint32_t cube_bytes;  // below 4 GiB
     Tex ssr, planar;  // data null: none
     uint32_t ssr_fmt, planar_fmt;
     Tex rc;  // data null: black
@@ -52,9 +54,9 @@ __device__ __forceinline__ float4 load_texel(const uint8_t* p, uint32_t fmt) {
 }
 
 // Where one sample's two mips sit within a cube, computed once per pixel
-// (every cube of the array has the same layout).
+// (every cube of the array has the same layout; a cube is below 4 GiB).
 struct MipPair {
-    size_t off0, off1;
+    uint32_t off0, off1;
     uint32_t res0, res1;
     float t;
 };
@@ -67,16 +69,15 @@ __device__ __forceinline__ MipPair mip_pair(const ReflectArgs& a, float lod) {
     const uint32_t m0 = (uint32_t)lf;
     mp.t = l - lf;
     const uint32_t tb = texel_size(a.env_fmt);
-    size_t off = 0;
-    uint32_t r = a.env_face;
+    uint32_t off = 0, r = a.env_face;
     for (uint32_t m = 0; m < m0; ++m) {
-        off += 6u * (size_t)r * r * tb;
+        off += 6u * r * r * tb;
         r = r > 1u ? r >> 1 : 1u;
     }
     mp.off0 = off;
     mp.res0 = r;
     if (m0 + 1u < a.env_mips) {
-        mp.off1 = off + 6u * (size_t)r * r * tb;
+        mp.off1 = off + 6u * r * r * tb;
         mp.res1 = r > 1u ? r >> 1 : 1u;
     } else {
         mp.off1 = off;
@@ -96,11 +97,11 @@ __device__ __forceinline__ float3 cube_bilinear(const ReflectArgs& a, const uint
     const float hi = fs - 1.0f;
     const uint32_t ix0 = (uint32_t)fminf(fmaxf(x0, 0.0f), hi), ix1 = (uint32_t)fminf(fmaxf(x0 + 1.0f, 0.0f), hi);
     const uint32_t iy0 = (uint32_t)fminf(fmaxf(y0, 0.0f), hi), iy1 = (uint32_t)fminf(fmaxf(y0 + 1.0f, 0.0f), hi);
-    const uint8_t* fb = mip + (size_t)face * res * res * tb;
-    const float4 t00 = load_texel(fb + ((size_t)iy0 * res + ix0) * tb, a.env_fmt);
-    const float4 t10 = load_texel(fb + ((size_t)iy0 * res + ix1) * tb, a.env_fmt);
-    const float4 t01 = load_texel(fb + ((size_t)iy1 * res + ix0) * tb, a.env_fmt);
-    const float4 t11 = load_texel(fb + ((size_t)iy1 * res + ix1) * tb, a.env_fmt);
+    const uint8_t* fb = mip + face * res * res * tb;
+    const float4 t00 = load_texel(fb + (iy0 * res + ix0) * tb, a.env_fmt);
+    const float4 t10 = load_texel(fb + (iy0 * res + ix1) * tb, a.env_fmt);
+    const float4 t01 = load_texel(fb + (iy1 * res + ix0) * tb, a.env_fmt);
+    const float4 t11 = load_texel(fb + (iy1 * res + ix1) * tb, a.env_fmt);
     const float gx = 1.0f - fx, gy = 1.0f - fy;
     const float r = (t00.x * gx + t10.x * fx) * gy + (t01.x * gx + t11.x * fx) * fy;
     const float g = (t00.y * gx + t10.y * fx) * gy + (t01.y * gx + t11.y * fx) * fy;
@@ -133,7 +134,7 @@ __device__ __forceinline__ float3 cube_sample(const ReflectArgs& a, const MipPai
         tc = -dy;
     }
     const float u = (sc / ma + 1.0f) * 0.5f, v = (tc / ma + 1.0f) * 0.5f;
-    const uint8_t* cb = a.env + a.cube_bytes * cube;
+    const uint8_t* cb = a.env + (size_t)a.cube_bytes * cube;
     const float3 c0 = cube_bilinear(a, cb + mp.off0, mp.res0, face, u, v);
     const float3 c1 = cube_bilinear(a, cb + mp.off1, mp.res1, face, u, v);
     const float t = mp.t, s = 1.0f - t;
@@ -192,16 +193,80 @@ __device__ __forceinline__ float3 capture_dir(const GpuReflectionCapture& cap, f
     return make_float3((px + rx * t) - cx, (py + ry * t) - cy, (pz + rz * t) - cz);
 }
 
-// sample_reflection_environment (:646-691); the caller checks a.env and
-// enable_reflections
-__device__ __forceinline__ float3 reflection_env(const ReflectArgs& a, const MipPair& mp, float px, float py, float pz,
-                                                 float rx, float ry, float rz) {
+// Whether capture cap can give any position of the box [lo, hi] a positive
+// capture_weight, with a margin far above the rounding of both this test and
+// the per-pixel weight (non-finite fields keep the capture).
+__device__ __forceinline__ bool capture_may_touch(const GpuReflectionCapture& cap, const float lo[3],
+                                                  const float hi[3]) {
+    if (cap.shape == HL_CAPTURE_SHAPE_BOX) {
+        // the box's positions map into the local box centre +- |A| half-extent
+        const float* m = cap.world_to_local;
+        float c[3], h[3], mc[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c[k] = (lo[k] + hi[k]) * 0.5f;
+            h[k] = (hi[k] - lo[k]) * 0.5f;
+            mc[k] = fabsf(c[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float cl = hl_x_mat_row(m, k, c[0], c[1], c[2], 1.0f);
+            const float a0 = fabsf(m[0 * 4 + k]), a1 = fabsf(m[1 * 4 + k]), a2 = fabsf(m[2 * 4 + k]);
+            const float hl = a0 * h[0] + a1 * h[1] + a2 * h[2];
+            const float mag = a0 * mc[0] + a1 * mc[1] + a2 * mc[2] + fabsf(m[3 * 4 + k]) + hl;
+            const float e = cap.extents_transition[k];
+            if (fabsf(cl) - hl > e + 1e-5f * (mag + fabsf(e)) + 1e-6f) return false;
+        }
+        return true;
+    }
+    float d2 = 0.0f, mag = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = cap.position_radius[k];
+        const float d = fmaxf(fmaxf(lo[k] - c, c - hi[k]), 0.0f);
+        d2 += d * d;
+        mag = fmaxf(mag, fmaxf(fabsf(c), fmaxf(fabsf(lo[k]), fabsf(hi[k]))));
+    }
+    const float r = cap.position_radius[3];
+    const float rr = r + 1e-5f * (mag + fabsf(r)) + 1e-6f;
+    return !(d2 > rr * rr);
+}
+
+// Captures (bit i: capture i) that may weigh on a live pixel of the wave:
+// every capture that gives a live pixel a positive weight is set (all of them
+// when a live position is not finite).  Called by every lane of the wave.
+__device__ __forceinline__ uint64_t wave_captures(const ReflectArgs& a, bool live, float px, float py, float pz) {
+    const uint64_t all = a.ncap >= 64u ? ~0ull : ((1ull << a.ncap) - 1ull);
+    const bool finite = __builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(pz);
+    if (hl_ballot(live && !finite) != 0) return all;
+    const float inf = __builtin_inff();
+    float lo[3] = {live ? px : inf, live ? py : inf, live ? pz : inf};
+    float hi[3] = {live ? px : -inf, live ? py : -inf, live ? pz : -inf};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = hl_uniform(hl_wave_min(lo[k]));
+        hi[k] = hl_uniform(hl_wave_max(hi[k]));
+    }
+    const uint32_t j = hl_lane();
+    bool hit = false;
+    if (j < a.ncap) {
+        const GpuReflectionCapture& cap = a.caps[j];
+        hit = cap.cubemap_index >= 0 && capture_may_touch(cap, lo, hi);
+    }
+    return hl_ballot(hit);
+}
+
+// sample_reflection_environment (:646-691) over the captures of mask (a
+// capture outside it weighs 0 on every pixel of the wave, which the shader's
+// loop skips); the caller checks a.env and enable_reflections
+__device__ __forceinline__ float3 reflection_env(const ReflectArgs& a, const MipPair& mp, uint64_t mask, float px,
+                                                 float py, float pz, float rx, float ry, float rz) {
 #pragma clang fp contract(off)
     float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
-    for (uint32_t i = 0; i < a.ncap; ++i) {
+    for (uint64_t bits = mask; bits; bits &= bits - 1ull) {
+        const uint32_t i = (uint32_t)__builtin_ctzll(bits);
         if (aa >= 0.999f) break;
         const GpuReflectionCapture& cap = a.caps[i];
-        if (cap.cubemap_index < 0) continue;
         const float w = capture_weight(cap, px, py, pz);
         if (w <= 0.0f) continue;
         const float3 d = capture_dir(cap, px, py, pz, rx, ry, rz);
@@ -233,32 +298,38 @@ __global__ __launch_bounds__(256) void k_reflection(ReflectArgs a) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t x = blockIdx.x * 16u + (lane & 15u);
     const uint32_t y = blockIdx.y * 16u + wave * 4u + (lane >> 4);
-    if (x >= a.width || y >= a.height) return;
-    const float depth = hl_load_depth(a.depth, x, y);
-    if (depth >= 1.0f) return;  // discard
+    // lanes stay active until the wave's capture set is known; sky pixels
+    // (depth >= 1) are discarded, those outside the frame have nothing to do
+    const bool inside = x < a.width && y < a.height;
+    const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
+    const bool live = inside && !(depth >= 1.0f);
+    if (hl_ballot(live) == 0) return;
     float4* o = reinterpret_cast<float4*>(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x;
-    const float4 ssr = a.ssr.data ? load_texel(a.ssr.data + (size_t)y * a.ssr.pitch + (size_t)x * texel_size(a.ssr_fmt),
-                                               a.ssr_fmt)
-                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 ssr = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (live && a.ssr.data)
+        ssr = load_texel(a.ssr.data + (size_t)y * a.ssr.pitch + (size_t)x * texel_size(a.ssr_fmt), a.ssr_fmt);
     if (a.debug == 30u) {
-        hl_store_nt(o, make_float4(ssr.w, ssr.w, ssr.w, 1.0f));
+        if (live) hl_store_nt(o, make_float4(ssr.w, ssr.w, ssr.w, 1.0f));
         return;
     }
     if (a.debug == 31u) {
-        hl_store_nt(o, make_float4(ssr.x, ssr.y, ssr.z, 1.0f));
+        if (live) hl_store_nt(o, make_float4(ssr.x, ssr.y, ssr.z, 1.0f));
         return;
     }
     if (debug_adds_nothing(a.debug)) {
-        if (a.flags & HL_REFLECTION_STORE) hl_store_nt(o, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        if (live && (a.flags & HL_REFLECTION_STORE)) hl_store_nt(o, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         return;
     }
-    const typename TexRaw<FN>::type rN = hl_fetch<FN>(a.normal, x, y);
-    const typename TexRaw<FA>::type rO = hl_fetch<FA>(a.orm, x, y);
-    const typename TexRaw<FN>::type rE = hl_fetch<FN>(a.emissive, x, y);
-    const float4 pl = a.planar.data ? load_texel(a.planar.data + (size_t)y * a.planar.pitch +
-                                                     (size_t)x * texel_size(a.planar_fmt),
-                                                 a.planar_fmt)
-                                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    typename TexRaw<FN>::type rN{}, rE{};
+    typename TexRaw<FA>::type rO{};
+    if (live) {
+        rN = hl_fetch<FN>(a.normal, x, y);
+        rO = hl_fetch<FA>(a.orm, x, y);
+        rE = hl_fetch<FN>(a.emissive, x, y);
+        if (a.planar.data)
+            pl = load_texel(a.planar.data + (size_t)y * a.planar.pitch + (size_t)x * texel_size(a.planar_fmt),
+                            a.planar_fmt);
+    }
     const float4 Nr = hl_decode<FN>(rN), O = hl_decode<FA>(rO), E = hl_decode<FN>(rE);
     const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
     const float rough = O.y;
@@ -288,11 +359,14 @@ __global__ __launch_bounds__(256) void k_reflection(ReflectArgs a) {
     const float ix = -V.x, iy = -V.y, iz = -V.z;
     const float dni = hl_x_dot3(n.x, n.y, n.z, ix, iy, iz);
     const float rx = ix - 2.0f * dni * n.x, ry = iy - 2.0f * dni * n.y, rz = iz - 2.0f * dni * n.z;
+    const bool use_env = a.enable_env && a.env && a.enable_refl;
+    const uint64_t capmask = use_env && a.ncap ? wave_captures(a, live, px, py, pz) : 0ull;
+    if (!live) return;
     float spec[3] = {0.0f, 0.0f, 0.0f};
     if (a.enable_env) {
         // a black cube array samples 0 everywhere: the environment term is 0
         float3 env = make_float3(0.0f, 0.0f, 0.0f);
-        if (a.env && a.enable_refl) env = reflection_env(a, mip_pair(a, rough * kEnvMaxLod), px, py, pz, rx, ry, rz);
+        if (use_env) env = reflection_env(a, mip_pair(a, rough * kEnvMaxLod), capmask, px, py, pz, rx, ry, rz);
         // env_brdf_approx (pbr_eval.wgsl:70-76)
         const float r0 = rough * -1.0f + 1.0f, r1 = rough * -0.0275f + 0.0425f;
         const float r2 = rough * -0.572f + 1.04f, r3 = rough * 0.022f + -0.04f;
@@ -407,12 +481,14 @@ HL_API hl_status hl_deferred_reflection(hl_context* c, const GpuCameraUniforms* 
             const size_t r = (e.face_size >> m) ? (e.face_size >> m) : 1u;
             cb += 6u * r * r * tb;
         }
+        if (cb > 0xFFFFFFFFull)
+            return fail(c, HL_EUNSUPPORTED, "one cube of the env array takes %zu bytes (at most 4 GiB)", cb);
         a.env = (const uint8_t*)e.data_d;
         a.env_fmt = e.format;
         a.env_face = e.face_size;
         a.env_mips = e.mip_levels;
         a.env_cubes = e.num_cubes;
-        a.cube_bytes = cb;
+        a.cube_bytes = (uint32_t)cb;
         a.ncap = g->reflection_capture_count < kMaxCaptures ? g->reflection_capture_count : kMaxCaptures;
         if (a.enable_env && a.enable_refl && a.ncap && !in->captures_d)
             return fail(c, HL_EINVAL, "%u reflection captures but captures_d is null", a.ncap);
