@@ -330,7 +330,7 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->markq, &c->shadeq,
-                      &c->words, &c->accum, &c->vogel})
+                      &c->words, &c->accum, &c->vogel, &c->rcirr})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return HL_OK;
@@ -656,6 +656,8 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
             a.rc_size[k] = g->rc_world_max[k] - g->rc_world_min[k];
         }
         mode = kModeExtended;  // shadows (if any) and RC
+        if ((s = ensure(c, c->rcirr, 16 * (size_t)gb->width * gb->height)) != HL_OK) return s;
+        a.rc_irr = (const float4*)c->rcirr.p;
     }
     if (sh) {
         ShadowView& v = a.shadow;
@@ -681,6 +683,7 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
         v.vogel = (const float*)c->vogel.p;
         if (mode != kModeExtended) mode = kModeShadowed;
     }
+    if (mode == kModeExtended) HL_HIP(c, launch_rc_irradiance(c->stream, set, a));
     return shade_launch(c, set, mode, a);
 }
 

@@ -155,6 +155,9 @@ struct ShadeArgs {
     Tex rc;
     uint32_t rc_fmt;
     float rc_min[3], rc_size[3];
+    // the probe atlas sampled per pixel ahead of the launch (k_rc_irradiance:
+    // width * height RGBA32F, rgb = sample_rc_irradiance at the pixel)
+    const float4* rc_irr;
 };
 
 // One view of a batched persistent launch (device memory).

@@ -33,7 +33,7 @@ struct hl_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum, vogel;
+    DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum, vogel, rcirr;
     bool vogel_ready = false;  // vogel holds the Vogel disk tables
     uint32_t shade_parity = 0;  // which half of shadeq the next shading launch claims from
     // the lights the records in `hot` were last built from (stream order)
@@ -242,6 +242,8 @@ hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n);
 // The Vogel disk tables of the shadowed shading (kVogelFloats floats).
 constexpr size_t kVogelTableFloats = 2 * 256 + (size_t)257 * 256;
 hipError_t launch_vogel_table(hipStream_t s, float* t);
+// a.rc_irr[pixel] = sample_rc_irradiance at every non-sky pixel (G-buffer format set gbuf_set)
+hipError_t launch_rc_irradiance(hipStream_t s, int gbuf_set, const ShadeArgs& a);
 hipError_t launch_light_grads_to_f32(hipStream_t s, const double* acc, uint32_t n,
                                      const uint32_t* subset, float* d_pos, float* d_int);
 

@@ -343,7 +343,8 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     if (kRC && a.rc_fmt != 0u && valid) {
         // radiance-cascade indirect diffuse (deferred_lighting.wgsl:1159-1163,
         // 1189-1205): mix(hemisphere, kD_ibl rc_irr albedo, clamp(4 |rc_irr|, 0, 1))
-        const float3 rc = rc_irradiance(a, p.wx, p.wy, p.wz, p.nx, p.ny, p.nz);
+        const float4 r4 = hl_ld4(a.rc_irr + pix);  // k_rc_irradiance, same N and position
+        const float3 rc = make_float3(r4.x, r4.y, r4.z);
         const float w = fminf(fmaxf(sqrtf(rc.x * rc.x + rc.y * rc.y + rc.z * rc.z) * 4.0f, 0.0f), 1.0f);
         const float u = fminf(fmaxf(1.0f - p.ndv, 0.0f), 1.0f);
         const float u2 = u * u;
@@ -505,7 +506,7 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
 // Forward launches are held to 96 VGPRs (5 waves per SIMD): the tile's
 // memory round trips need the waves to hide them.
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeShadowed ? 4 : (MODE == kModeExtended ? 3 : 1))) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeShadowed || MODE == kModeExtended ? 4 : 1)) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
                                                uint32_t* __restrict__ queue_next) {
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
@@ -675,6 +676,38 @@ __global__ void k_vogel_table(float* __restrict__ t) {
         t[kVogelMax + i] = sa;
     }
     if (i < n) t[kVogelRadii + (size_t)n * kVogelMax + i] = sqrtf((float)i + 0.5f) / sqrtf((float)n);
+}
+
+// sample_rc_irradiance per non-sky pixel ahead of the extended shading launch,
+// from the N and world position that kernel computes (exact normalize, the
+// same row sums over inv_view_proj): the atlas walk stays out of the light
+// loop's register budget.  One thread per pixel, 64 x 4 pixels per block.
+template <int FN>
+__global__ __launch_bounds__(256) void k_rc_irradiance(ShadeArgs a) {
+#pragma clang fp contract(off)
+    const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u), y = blockIdx.y * 4u + (threadIdx.x >> 6);
+    if (x >= a.width || y >= a.height) return;
+    const float depth = hl_load_depth(a.depth, x, y);
+    if (!(depth < 1.0f)) return;
+    const float4 Nr = hl_load_rgba<FN>(a.normal, x, y);
+    const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
+    const float u = ((float)x + 0.5f) / (float)a.width;
+    const float v = ((float)y + 0.5f) / (float)a.height;
+    const float ndx = u * 2.0f - 1.0f, ndy = 1.0f - v * 2.0f;
+    const float* m = a.cam->inv_view_proj;
+    const float hw = hl_x_mat_row(m, 3, ndx, ndy, depth, 1.0f);
+    const float wx = hl_x_mat_row(m, 0, ndx, ndy, depth, 1.0f) / hw;
+    const float wy = hl_x_mat_row(m, 1, ndx, ndy, depth, 1.0f) / hw;
+    const float wz = hl_x_mat_row(m, 2, ndx, ndy, depth, 1.0f) / hw;
+    const float3 rc = rc_irradiance(a, wx, wy, wz, n.x, n.y, n.z);
+    hl_store_nt(const_cast<float4*>(a.rc_irr) + (size_t)y * a.width + x, make_float4(rc.x, rc.y, rc.z, 0.0f));
+}
+
+hipError_t launch_rc_irradiance(hipStream_t s, int gbuf_set, const ShadeArgs& a) {
+    const dim3 grid((a.width + 63u) / 64u, (a.height + 3u) / 4u);
+    if (gbuf_set == kSet32) k_rc_irradiance<HL_FMT_RGBA32_FLOAT><<<grid, 256, 0, s>>>(a);
+    else k_rc_irradiance<HL_FMT_RGBA16_FLOAT><<<grid, 256, 0, s>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_vogel_table(hipStream_t s, float* t) {
