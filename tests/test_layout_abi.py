@@ -65,18 +65,22 @@ def test_c_header_static_asserts_compile(tmp_path):
 
 def test_ctypes_structs_match_c_sizes(tmp_path):
     src = tmp_path / "s.c"
-    src.write_text('#include <stdio.h>\n#include "helio_hip/helio_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(hl_gbuffer),'
-                   'sizeof(hl_cluster_params), sizeof(hl_light_grid), sizeof(hl_view_desc),'
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "helio_hip/helio_hip.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+                   'sizeof(hl_gbuffer), sizeof(hl_cluster_params), sizeof(hl_light_grid), sizeof(hl_view_desc),'
                    'sizeof(hl_texture), sizeof(hl_surface), sizeof(hl_fragment), sizeof(hl_transform),'
-                   'sizeof(ForwardLitGlobals));return 0;}\n')
+                   'sizeof(ForwardLitGlobals), sizeof(hl_shadow_maps), sizeof(GpuReflectionCapture),'
+                   'sizeof(hl_cube_array), sizeof(hl_reflection_inputs),'
+                   'offsetof(hl_reflection_inputs, rc_cascade0));return 0;}\n')
     exe = tmp_path / "s"
     subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     assert sizes == [C.sizeof(abi.hl_gbuffer), C.sizeof(abi.hl_cluster_params),
                      C.sizeof(abi.hl_light_grid), C.sizeof(abi.hl_view_desc), C.sizeof(abi.hl_texture),
                      C.sizeof(abi.hl_surface), C.sizeof(abi.hl_fragment), C.sizeof(abi.hl_transform),
-                     C.sizeof(abi.ForwardLitGlobals)]
+                     C.sizeof(abi.ForwardLitGlobals), C.sizeof(abi.hl_shadow_maps),
+                     C.sizeof(abi.GpuReflectionCapture), C.sizeof(abi.hl_cube_array),
+                     C.sizeof(abi.hl_reflection_inputs), abi.hl_reflection_inputs.rc_cascade0.offset]
 
 
 def declared_symbols():
