@@ -13,7 +13,8 @@ The oracle (oracle/reflection.c) is pinned by
   captures with the skylight taking the rest, the -1 cube, the 64-capture
   limit and enable_reflections;
 * the frame: toggles off add exactly 0, the SSR debug modes, and the SSR-only,
-  environment-only and RC-only composites against float64 closed forms.
+  SSR-then-planar, environment-only and RC-only composites against float64
+  closed forms.
 """
 import ctypes as C
 import math
@@ -312,6 +313,11 @@ def test_ssr_and_env_closed_forms():
     sc = small("c4_512_adj")  # fp32 G-buffer
     H, W = sc.height, sc.width
     refl = synth.make_reflections(sc, n_box=0, n_sphere=0, face=8, mips=4, planar=False)
+    # a planar image over half of the frame (make_reflections' planar covers up-facing pixels only)
+    pimg = np.zeros((H, W, 4), np.float32)
+    pimg[:, : W // 2, :3] = np.random.default_rng(9).uniform(0.0, 1.5, size=(H, W // 2, 3))
+    pimg[:, : W // 2, 3] = 0.35
+    pimg = pimg.astype(np.float16)
     refl.cubes[:, :3] = np.array([0.2, 0.4, 0.6], refl.cubes.dtype)  # every texel of every mip: constant
     c = refl.cubes[0, :3].astype(np.float64)
     orm = synth._decode_plane(sc.orm, sc.formats[0], H, W).astype(np.float64)
@@ -322,6 +328,10 @@ def test_ssr_and_env_closed_forms():
     # SSR only
     g.enable_env_reflections = 0
     out_s, _ = O.reflection(sc, refl, flags=abi.HL_REFLECTION_STORE, globals_=g)
+    # SSR, then planar over it (the shader's order)
+    refl_sp = synth.HostReflections(refl.cubes, refl.face_size, refl.mip_levels, refl.num_cubes, refl.format,
+                                    refl.captures, ssr=refl.ssr, planar=pimg)
+    out_sp, _ = O.reflection(sc, refl_sp, flags=abi.HL_REFLECTION_STORE, globals_=g)
     # environment only (no SSR: enable_reflections gates SSR and the capture/sky blend alike, so SSR a = 0)
     g.enable_env_reflections = 1
     refl_e = synth.HostReflections(refl.cubes, refl.face_size, refl.mip_levels, refl.num_cubes, refl.format,
@@ -340,6 +350,10 @@ def test_ssr_and_env_closed_forms():
             a = s[y, x, 3]
             want = (s[y, x, :3] * Fi * a if a > 0 else np.zeros(3)) * ao
             np.testing.assert_allclose(out_s[y, x, :3], want, rtol=2e-5, atol=1e-7)
+            pa = float(pimg[y, x, 3])
+            spec_s = s[y, x, :3] * Fi * a if a > 0 else np.zeros(3)
+            want_sp = (spec_s * (1 - pa) + pimg[y, x, :3].astype(np.float64) * Fi * pa if pa > 0 else spec_s) * ao
+            np.testing.assert_allclose(out_sp[y, x, :3], want_sp, rtol=2e-5, atol=1e-7)
             q = r * np.array([-1.0, -0.0275, -0.572, 0.022]) + np.array([1.0, 0.0425, 1.04, -0.04])
             a004 = min(q[0] ** 2, 2.0 ** (-9.28 * ndv)) * q[0] + q[1]
             want_e = c * (F0 * (-1.04 * a004 + q[2]) + (1.04 * a004 + q[3])) * ao
