@@ -1,8 +1,10 @@
 // hl_rc.hpp -- the deferred shader's radiance-cascade indirect diffuse
 // (crates/passes/3d/helio-pass-deferred-light/shaders/deferred_lighting.wgsl:
 // 842-904, sample_rc_irradiance) for the shading kernel.  rc_cascade0 is read
-// with textureLoad (no filtering): every operation is one IEEE rounding and
-// the irradiance agrees with the restatement bit for bit.
+// with textureLoad (no filtering); the corner sums are fused multiply-adds
+// (WGSL leaves contraction to the backend; the restatement fuses the same
+// ones) and every other operation is one IEEE rounding, so the irradiance
+// agrees with the restatement bit for bit.
 #pragma once
 
 #include "hl_internal.hpp"
@@ -92,9 +94,9 @@ __device__ __forceinline__ float3 rc_irradiance(const A& a, float wx, float wy, 
         const auto add = [&](uint32_t k, float r, float g, float b) {
             const float w = cw[k];
             if (w > 0.001f) {
-                ir = ir + r * w;
-                ig = ig + g * w;
-                ib = ib + b * w;
+                ir = __builtin_fmaf(r, w, ir);
+                ig = __builtin_fmaf(g, w, ig);
+                ib = __builtin_fmaf(b, w, ib);
                 ws = ws + w;
             }
         };

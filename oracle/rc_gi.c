@@ -6,7 +6,9 @@
  *
  * TEST INFRASTRUCTURE ONLY: used by the tests as the checker of
  * hl_deferred_forward_ex, never by the library.  Built with
- * -ffp-contract=off: one IEEE operation per source operation.
+ * -ffp-contract=off: one IEEE operation per source operation, except the
+ * corner sums, written as fused multiply-adds (fmaf): WGSL leaves contraction
+ * to the backend, and the kernel fuses exactly these.
  *
  * rc_cascade0 is read with textureLoad (no filtering), so the restatement is
  * exact: the probe atlas is RC_DIR_DIM * RC_PROBE_DIM = 64 texels wide and
@@ -102,7 +104,7 @@ static void rc_corner(const hl_texture* t, uint32_t px, uint32_t py, uint32_t pz
             if (w > 0.001f) {
                 float c[3];
                 rc_texel(t, cpx * RC_DIR_DIM + dx, (cpy * RC_PROBE_DIM + cpz) * RC_DIR_DIM + dy, c);
-                for (int k = 0; k < 3; ++k) irr[k] = irr[k] + c[k] * w;
+                for (int k = 0; k < 3; ++k) irr[k] = fmaf(c[k], w, irr[k]); /* contracted, as the kernel */
                 wsum = wsum + w;
             }
         }
