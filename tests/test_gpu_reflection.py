@@ -103,3 +103,27 @@ def test_rejects_bad_inputs(ctx):
     with pytest.raises(abi.HelioError):
         ReflectionPass(ctx).execute(ds, dr, out, globals_=g)
     ReflectionPass(ctx).execute(ds, dr, out)  # and the restored inputs run
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_random_configurations(ctx, case):
+    """Ragged frame sizes, 0-19 boxes + 1-19 spheres, cube faces of 8-64
+    texels with any mip count, the three cube formats, RC on and off, added
+    over a random frame: within 1e-4 of the restatement (in practice a few
+    1e-7, from exp2 in env_brdf_approx)."""
+    rng = np.random.default_rng(1000 + case)
+    name = ("c1_720p_64", "c2_1080p_1k", "c4_512_adj")[case % 3]
+    W, H = int(rng.integers(97, 700)), int(rng.integers(61, 400))
+    sc = synth.make_scene(name, width=W, height=H)
+    face = int(rng.choice([8, 16, 32, 64]))
+    mips = int(rng.integers(1, int(np.log2(face)) + 2))
+    fmt = (abi.HL_FMT_RGBA16_FLOAT, abi.HL_FMT_RGBA32_FLOAT, abi.HL_FMT_RGBA8_UNORM)[case % 3]
+    nb, ns = int(rng.integers(0, 20)), int(rng.integers(1, 20))
+    print(f"[reflection] random case {case}: {name} {W}x{H}, {nb}+{ns} captures, face {face} x {mips} mips, "
+          f"format {fmt}", flush=True)
+    refl = synth.make_reflections(sc, n_box=nb, n_sphere=ns, face=face, mips=mips, fmt=fmt, rc=bool(case % 2),
+                                  seed=int(rng.integers(1, 1000)))
+    base = rng.uniform(0.0, 1.0, size=(H, W, 4)).astype(np.float32)
+    got = run(ctx, sc, refl, base)
+    ref, _ = O.reflection(sc, refl, out=base.copy())
+    check_radiance(got, ref)
