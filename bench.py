@@ -32,11 +32,9 @@ Further legs, each reported in its own object of the same line:
   reflection_4k   the pass's second draw (hl_deferred_reflection) over the 4K
                   G-buffer: 16 reflection captures over 17 RGBA16F cubes of
                   128^2 x 8 mips, SSR and planar images, RC rough specular
-  multiview       (opt-in, --multiview) 48 probe faces (16 each at 128^2,
-                  256^2, 512^2) + 2 XR eyes at 2064x2208 over an 8192-light
-                  pool, adjoint on the eyes: one batched frame (autotuned, graph
-                  replay) against the same views run one by one on the plain
-                  single-view path
+  multiview       the full-size 50-view set (48 probe faces + 2 XR eyes at
+                  2064x2208) is refused: multi-view plans are limited to
+                  HL_BATCH_MAX_MULTIVIEW_PIXELS pixels (--multiview exits 2)
 """
 from __future__ import annotations
 
@@ -46,7 +44,6 @@ import json
 import os
 import statistics
 import sys
-import tempfile
 import time
 from pathlib import Path
 
@@ -54,7 +51,6 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-
 
 
 def roofline_constants() -> dict[str, float]:
@@ -364,60 +360,6 @@ def bench_adjoint(ctx, args):
                         "position/intensity per light"}
 
 
-def bench_multiview(ctx, args):
-    from helio_amd import abi
-    from helio_amd.batch import BatchPlan, make_multiview
-    from helio_amd.device import AdjointBuffers, DeferredLightPass, LightCullPass
-    from helio_amd import synth
-
-    t0 = time.perf_counter()
-    pool, pool_size, views, hosts, _ = make_multiview(ctx, small=False)
-    for v in views:
-        if v.adjoint:
-            t = np.zeros((v.scene.host.height, v.scene.host.width, 4), np.float32)
-            t[..., :3] = 0.25
-            v.target.upload(t)
-    npx = sum(v.scene.host.width * v.scene.host.height for v in views)
-    log(f"multiview: {len(views)} views, {npx} px, set-up {time.perf_counter() - t0:.1f} s")
-    cache = os.path.join(tempfile.gettempdir(), "helio_hip_autotune.txt")
-    t0 = time.perf_counter()
-    plan = BatchPlan(ctx, pool, pool_size, views, autotune=True, max_trials=args.tune_trials, cache_path=cache)
-    tune_s = time.perf_counter() - t0
-    try:
-        cfgs = [plan.config(i) for i in range(len(views))]
-        eager, _ = time_steps(ctx, max(5, args.steps // 5), 2, plan.launch)
-        plan.capture()
-        graph, _ = time_steps(ctx, max(5, args.steps // 5), 2, plan.launch)
-    finally:
-        plan.destroy()
-    log(f"multiview batched: graph {graph:.3f} ms, eager {eager:.3f} ms, plan+tune {tune_s:.1f} s")
-    # the same views one at a time on the plain path, with the plan's grids
-    steps = []
-    for v, c in zip(views, cfgs):
-        sc = v.scene.host
-        p = synth.cluster_params(ctx.lib, sc, grid=(c.grid_x, c.grid_y, c.grid_z), cap=0)
-        cull = LightCullPass(ctx, sc.width, sc.height, clusters=p, pool_cap=1 << 23)
-        shade = DeferredLightPass(ctx, sc.width, sc.height)
-        cull.prepare(v.scene)
-        shade.prepare(v.scene)
-        grid = cull.publish()
-        steps.append(lambda cull=cull, ds=v.scene: cull.execute(ds))
-        if v.adjoint:
-            g = AdjointBuffers(ctx, sc.width, sc.height, v.num_lights)
-            steps.append(lambda shade=shade, ds=v.scene, grid=grid, t=v.target, g=g:
-                         shade.forward_backward_l2(ds, grid, t, g))
-        else:
-            steps.append(lambda shade=shade, ds=v.scene, grid=grid: shade.execute(ds, grid))
-    plain, _ = time_steps(ctx, max(5, args.steps // 5), 2, lambda: [s() for s in steps])
-    log(f"multiview plain single-view sum: {plain:.3f} ms")
-    tiles = sorted({(c.wave_tile_w, c.wave_tile_h) for c in cfgs})
-    return {"ms_per_frame_graph": graph, "ms_per_frame_eager": eager, "ms_per_frame_single_view_sum": plain,
-            "speedup_vs_single_view": plain / graph, "views": len(views), "pixels": npx,
-            "plan_and_autotune_s": tune_s, "autotune_trials": args.tune_trials,
-            "wave_tiles_chosen": [f"{w}x{h}" for w, h in tiles],
-            "depth_slices_chosen": sorted({c.grid_z for c in cfgs})}
-
-
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -431,14 +373,21 @@ def main() -> int:
     ap.add_argument("--no-shadows", action="store_true", help="skip the 4K shadowed leg")
     ap.add_argument("--no-reflection", action="store_true", help="skip the 4K reflection-composite leg")
     ap.add_argument("--multiview", action="store_true",
-                    help="also run the full-size 50-view batched leg (opt-in: its autotuned plan hit a GPU "
-                         "memory fault once, cause not yet found)")
+                    help="the full-size 50-view batched leg: refused (its plan hit a device memory fault "
+                         "once, cause not found; multi-view plans are limited to "
+                         "HL_BATCH_MAX_MULTIVIEW_PIXELS pixels)")
     ap.add_argument("--tune-trials", type=int, default=4)
     ap.add_argument("--workload", default="c3_4k_16k")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.multiview:
+        print("bench.py --multiview is refused: the full-size 50-view plan (48 probe faces + 2 eyes at "
+              "2064x2208, 14.6 M pixels) hit a device memory fault once and its cause is not found, so "
+              "hl_batch_plan_create limits multi-view plans to HL_BATCH_MAX_MULTIVIEW_PIXELS pixels; the "
+              "1/8-scale 50-view plan is checked by tests/test_gpu_batch.py", file=sys.stderr)
+        return 2
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported; the path would shard by screen tiles and views)",
@@ -465,7 +414,6 @@ def main() -> int:
         fwd = None if args.no_forward else bench_forward_lit(ctx, args)
         shd = None if args.no_shadows else bench_shadowed(ctx, args)
         rfl = None if args.no_reflection else bench_reflection(ctx, args)
-        mv = bench_multiview(ctx, args) if args.multiview else None
 
     sc, q = r4k["scene"], r4k["q"]
     ms_frame, shade_ms = r4k["ms_frame"], r4k["shade_ms"]
@@ -527,10 +475,10 @@ def main() -> int:
         res["shadowed_4k"] = shd
     if rfl is not None:
         res["reflection_4k"] = rfl
-    res["multiview"] = mv if mv is not None else {
-        "skipped": "full-size 50-view leg is opt-in (--multiview): its autotuned plan faulted the GPU once "
-                   "and stays off the GPU until the cause is found; the 1/8-scale 50-view plan is "
-                   "checked by the GPU tests"}
+    res["multiview"] = {
+        "skipped": "full-size 50-view plan refused (HL_EUNSUPPORTED above HL_BATCH_MAX_MULTIVIEW_PIXELS): it "
+                   "hit a device memory fault once and its cause is not found; the 1/8-scale 50-view plan "
+                   "is checked by the GPU tests"}
     if not args.no_cpu:
         log("CPU baseline")
         res["cpu_baseline"] = cpu_baseline(sc, params, r4k["pool_cap"])

@@ -25,6 +25,7 @@ HL_SHADE_CLEAR_SKY = 1
 HL_SHADE_LIGHTS_FROM_CULL = 2
 HL_OVERFLOW_CAP, HL_OVERFLOW_POOL = 1, 2
 HL_PIXEL_GRAD_PLANES = 5
+HL_BATCH_MAX_VIEWS, HL_BATCH_MAX_MULTIVIEW_PIXELS = 255, 262144
 
 # GpuLight (crates/libhelio/src/light.rs:60-106), 128 bytes.
 GPU_LIGHT_DTYPE = np.dtype([

@@ -177,6 +177,16 @@ hl_status build_plan(hl_plan* p) {
             hl_status st = hl_cluster_params_init(&d.camera, W, H, gx, gy, cf.gz, 0, &cp);
             if (st != HL_OK) return fail(c, st, "view %u: camera is not a standard-Z perspective", v);
         }
+        // The grid bounds of check_cluster_params (at most 65535 columns and
+        // 2^26 cells) keep the packed fields of the mark queue in range: box
+        // extents and slice fields in 16 bits, chunk indices (ncell / 64 <
+        // 2^20) in the 24 bits beside the 8-bit view index.
+        {
+            const hl_status st = check_cluster_params(c, &cp);
+            if (st != HL_OK) return st;
+        }
+        static_assert(HL_BATCH_MAX_VIEWS < 256u, "view index of a mark queue entry: 8 bits");
+        static_assert((1u << 26) / 64u < (1u << 24), "chunk index of a mark queue entry: 24 bits");
         const uint32_t n = p->nlights[v];
         const uint32_t ncol = cp.grid_x * cp.grid_y, ncell = ncol * cp.grid_z;
         ViewBufs& b = p->bufs[v];
@@ -212,6 +222,10 @@ hl_status build_plan(hl_plan* p) {
         cv.W = (n + 31u) / 32u;
         cv.SW = (cv.W + 31u) / 32u;
         cv.ncell = ncell;
+        if ((uint64_t)light_base[v] + n >= (1ull << 28) ||
+            (uint64_t)cell_base[v] + ncell + kBatchCellAlign >= (1ull << 31))
+            return fail(c, HL_ECAPACITY, "plan too large: %u views up to view %u exceed 2^28 lights or 2^31 cells",
+                        nv, v);
         light_base[v + 1] = light_base[v] + n;
         cell_base[v + 1] = cell_base[v] + (ncell + kBatchCellAlign - 1u) / kBatchCellAlign * kBatchCellAlign;
         tile_base[v + 1] = tile_base[v] + cluster_scan_tiles(ncell);
@@ -448,7 +462,16 @@ HL_API hl_status hl_batch_plan_create(hl_context* c, const GpuLight* pool_d, uin
     if (!out) return fail(c, HL_EINVAL, "plan output is null");
     *out = nullptr;
     if (!views || num_views == 0) return fail(c, HL_EINVAL, "no views");
-    if (num_views > 255) return fail(c, HL_EINVAL, "at most 255 views per plan");
+    if (num_views > HL_BATCH_MAX_VIEWS) return fail(c, HL_EINVAL, "at most %u views per plan", HL_BATCH_MAX_VIEWS);
+    if (num_views > 1) {
+        uint64_t px = 0;
+        for (uint32_t v = 0; v < num_views; ++v) px += (uint64_t)views[v].gbuffer.width * views[v].gbuffer.height;
+        if (px > HL_BATCH_MAX_MULTIVIEW_PIXELS)
+            return fail(c, HL_EUNSUPPORTED,
+                        "multi-view plan of %llu pixels: plans of more than one view are limited to %u pixels "
+                        "in total (larger ones are not validated on the device)",
+                        (unsigned long long)px, HL_BATCH_MAX_MULTIVIEW_PIXELS);
+    }
     if (pool_size && !pool_d) return fail(c, HL_EINVAL, "light pool is null");
     HL_HIP(c, hipSetDevice(c->device));
     hl_plan* p = new hl_plan();

