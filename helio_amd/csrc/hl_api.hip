@@ -70,13 +70,17 @@ uint32_t texel_bytes(uint32_t fmt) {
     }
 }
 
-hl_status check_tex(hl_context* c, const hl_texture& t, uint32_t w, const char* name) {
+// A texture of w x h texels; the kernels address its texels with 32-bit
+// byte offsets, so its rows must end below 4 GiB from the base.
+hl_status check_tex(hl_context* c, const hl_texture& t, uint32_t w, uint32_t h, const char* name) {
     const uint32_t tb = texel_bytes(t.format);
     if (!t.data_d) return fail(c, HL_EINVAL, "gbuffer.%s: null data pointer", name);
     if (tb == 0) return fail(c, HL_EINVAL, "gbuffer.%s: unknown format %u", name, t.format);
     if ((uint64_t)t.pitch_bytes < (uint64_t)w * tb || t.pitch_bytes % tb != 0)
         return fail(c, HL_EINVAL, "gbuffer.%s: pitch %u too small or not a multiple of %u", name,
                     t.pitch_bytes, tb);
+    if ((uint64_t)t.pitch_bytes * h > 0xFFFFFFFFull)
+        return fail(c, HL_EUNSUPPORTED, "gbuffer.%s: %u rows of %u bytes exceed 4 GiB", name, h, t.pitch_bytes);
     if (((uintptr_t)t.data_d) % tb != 0)
         return fail(c, HL_EINVAL, "gbuffer.%s: data not aligned to %u bytes", name, tb);
     return HL_OK;
@@ -90,11 +94,11 @@ hl_status check_gbuffer(hl_context* c, const hl_gbuffer* g, int* set) {
     if (g->width > 32768 || g->height > 32768)
         return fail(c, HL_EINVAL, "gbuffer %ux%u exceeds 32768", g->width, g->height);
     hl_status s;
-    if ((s = check_tex(c, g->albedo, g->width, "albedo")) != HL_OK) return s;
-    if ((s = check_tex(c, g->normal, g->width, "normal")) != HL_OK) return s;
-    if ((s = check_tex(c, g->orm, g->width, "orm")) != HL_OK) return s;
-    if ((s = check_tex(c, g->emissive, g->width, "emissive")) != HL_OK) return s;
-    if ((s = check_tex(c, g->depth, g->width, "depth")) != HL_OK) return s;
+    if ((s = check_tex(c, g->albedo, g->width, g->height, "albedo")) != HL_OK) return s;
+    if ((s = check_tex(c, g->normal, g->width, g->height, "normal")) != HL_OK) return s;
+    if ((s = check_tex(c, g->orm, g->width, g->height, "orm")) != HL_OK) return s;
+    if ((s = check_tex(c, g->emissive, g->width, g->height, "emissive")) != HL_OK) return s;
+    if ((s = check_tex(c, g->depth, g->width, g->height, "depth")) != HL_OK) return s;
     if (g->depth.format != HL_FMT_R32_FLOAT)
         return fail(c, HL_EINVAL, "gbuffer.depth must be R32_FLOAT (Depth32Float)");
     if (g->lightmap_uv.data_d)
@@ -222,6 +226,8 @@ hl_status build_shade_args(hl_context* c, const GpuCameraUniforms* camera_d,
     a->cam = camera_d;
     a->width = W;
     a->height = H;
+    a->rcp_w = 1.0f / (float)W;  // IEEE division: the correctly rounded reciprocal
+    a->rcp_h = 1.0f / (float)H;
     a->albedo = {(const uint8_t*)gb->albedo.data_d, gb->albedo.pitch_bytes};
     a->normal = {(const uint8_t*)gb->normal.data_d, gb->normal.pitch_bytes};
     a->orm = {(const uint8_t*)gb->orm.data_d, gb->orm.pitch_bytes};
@@ -585,16 +591,16 @@ HL_API hl_status hl_gbuffer_pack(hl_context* c, const hl_surface* sf, const hl_g
         if (((uintptr_t)planes[i]) % 16) return fail(c, HL_EINVAL, "surface plane %d not 16-B aligned", i);
     }
     hl_status s;
-    if ((s = check_tex(c, g->albedo, g->width, "albedo")) != HL_OK) return s;
-    if ((s = check_tex(c, g->normal, g->width, "normal")) != HL_OK) return s;
-    if ((s = check_tex(c, g->orm, g->width, "orm")) != HL_OK) return s;
-    if ((s = check_tex(c, g->emissive, g->width, "emissive")) != HL_OK) return s;
+    if ((s = check_tex(c, g->albedo, g->width, g->height, "albedo")) != HL_OK) return s;
+    if ((s = check_tex(c, g->normal, g->width, g->height, "normal")) != HL_OK) return s;
+    if ((s = check_tex(c, g->orm, g->width, g->height, "orm")) != HL_OK) return s;
+    if ((s = check_tex(c, g->emissive, g->width, g->height, "emissive")) != HL_OK) return s;
     int set = 0;
     if ((s = color_set(c, g, &set)) != HL_OK) return s;
     if (g->lightmap_uv.data_d) {
         if (g->lightmap_uv.format != HL_FMT_RG16_FLOAT)
             return fail(c, HL_EINVAL, "gbuffer.lightmap_uv must be RG16_FLOAT");
-        if ((s = check_tex(c, g->lightmap_uv, g->width, "lightmap_uv")) != HL_OK) return s;
+        if ((s = check_tex(c, g->lightmap_uv, g->width, g->height, "lightmap_uv")) != HL_OK) return s;
     }
     HL_HIP(c, hipSetDevice(c->device));
     HL_HIP(c, launch_gbuffer_pack(c->stream, set, *sf, *g));

@@ -41,7 +41,7 @@ __device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
     const float range = type == HL_LIGHT_DIRECTIONAL ? inf : pr.w;
     if (hot)
         hot[i] = make_hot_light(pr.x, pr.y, pr.z, dro.x, dro.y, dro.z, ci, pr.w, dro.w, inner, type,
-                                shadow_word(i, type, l->shadow_index));
+                                shadow_word(i, type, l->shadow_index), true);
     if (cull) {
         const float* v = cam->view;
         CullLight c;

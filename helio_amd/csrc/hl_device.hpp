@@ -29,10 +29,11 @@ namespace hl {
 //   p1 = (color.rgb * intensity, cos inner)
 //   p2 = (direction.xyz, cos outer)     cos outer = -2 for non-spot lights
 //   p3 = (1 / range^2, intensity, cone margin, shadow word as bits)
-//   p4 = (range^2 (1 - 1e-5), range^2 (1 + 1e-5), ci, 3.61e-5 ci)
-//        with ci = max(p1.xyz): the range test's band and the 0.002 test's
-//        scale and error coefficient, so that the per-pixel loop does not
-//        recompute them
+//   p4 = (lo, hi, ci, 0) with ci = max(p1.xyz): the band of the first
+//        per-pixel decision on the squared distance d2 (d2 < lo: in, d2 > hi:
+//        the term is zero, otherwise decided exactly) -- the range test alone
+//        for the forward-lit shader, the range test and the 0.002 cut of the
+//        deferred shader together (make_hot_light)
 // The cone margin, 1e-5 * |direction|_1, bounds (tenfold) the difference
 // between a fast cos(angle) and the exact one of deferred_lighting.wgsl:739.
 struct alignas(16) HotLight {
@@ -79,13 +80,51 @@ struct ShadowView {
     float csm[4];
 };
 
+// Band (lo, hi) of the first per-pixel decision on d2 = |light - pixel|^2
+// for a light of the given range (squared, range2) and largest radiance
+// channel ci.  The range test alone: r^2 (1 -+ 1e-5), beyond the rounding of
+// either side.  With the cut (deferred shading), d2 must also pass
+// all(radiance < 0.002) of pbr_direct_light (deferred_lighting.wgsl:781) for
+// an unshadowed point light, whose largest channel ci (1 - (d2/r^2)^2) /
+// (d2 + 1e-4) decreases in d2 and crosses 0.002 at d2* = x r^2, x the
+// positive root of ci x^2 + 0.002 r^2 x - (ci - 2e-7) = 0 (computed in FP64
+// in the stable form).  The band around d2* is +-1e-4 relative: there the
+// function moves by at least ~1e-4 relative, far beyond the rounding of its
+// FP32 evaluation (a few ulp, and the 1 - nd^4 cancellation near the range
+// is steeper still); below d2* = 1e-2 the band widens to [0, 2 d2* + 1e-3].
+// The smaller of the two lower and of the two upper bounds is kept; lo is
+// rounded down and hi up to FP32.
+__device__ inline float2 decision_band(float range, float ci, bool cut) {
+    const double R = (double)range * (double)range;
+    double lo = R * (1.0 - 1e-5), hi = R * (1.0 + 1e-5);
+    if (cut) {
+        double tlo = 0.0, thi = 1e-3;  // ci <= 2e-7: at most 0.002 at d2 = 0
+        const double c = (double)ci;
+        if (c > 2e-7) {
+            const double b = 0.002 * R, cc = c - 2e-7;
+            const double d2c = R * (2.0 * cc / (b + sqrt(b * b + 4.0 * c * cc)));
+            if (d2c < 1e-2) {
+                tlo = 0.0;
+                thi = 2.0 * d2c + 1e-3;
+            } else {
+                tlo = d2c * (1.0 - 1e-4);
+                thi = d2c * (1.0 + 1e-4);
+            }
+        }
+        lo = fmin(lo, tlo);
+        hi = fmin(hi, thi);
+    }
+    return make_float2(__double2float_rd(lo), __double2float_ru(hi));
+}
+
 // The shading record of one light from its fields: position (px, py, pz),
 // direction (dx, dy, dz) as stored (a spot light's axis, the direction a
 // directional light shines along), colour and intensity ci, range, the cosines
-// of the outer and inner cone angles, and the light type.
+// of the outer and inner cone angles, and the light type; cut: the record's
+// band includes the 0.002 cut (deferred shading) or not (forward-lit).
 __device__ __forceinline__ HotLight make_hot_light(float px, float py, float pz, float dx, float dy, float dz,
                                                    float4 ci, float range_field, float outer, float inner,
-                                                   uint32_t type, uint32_t shadow) {
+                                                   uint32_t type, uint32_t shadow, bool cut) {
     const float range = type == HL_LIGHT_DIRECTIONAL ? __builtin_inff() : range_field;
     HotLight h;
     h.p0 = make_float4(px, py, pz, range);
@@ -96,12 +135,9 @@ __device__ __forceinline__ HotLight make_hot_light(float px, float py, float pz,
     h.p2 = make_float4(dx, dy, dz, type == HL_LIGHT_SPOT ? outer : kNoCone);
     const float margin = 1e-5f * (fabsf(dx) + fabsf(dy) + fabsf(dz));
     h.p3 = make_float4(1.0f / (range * range), ci.w, margin, __uint_as_float(shadow));
-    {
-#pragma clang fp contract(off)
-        const float r2 = range * range;
-        const float cmax = fmaxf(fmaxf(h.p1.x, h.p1.y), h.p1.z);
-        h.p4 = make_float4(r2 * (1.0f - 1e-5f), r2 * (1.0f + 1e-5f), cmax, 3.61e-5f * cmax);
-    }
+    const float cmax = fmaxf(fmaxf(h.p1.x, h.p1.y), h.p1.z);
+    const float2 band = decision_band(range, cmax, cut);
+    h.p4 = make_float4(band.x, band.y, cmax, 0.0f);
     return h;
 }
 
@@ -118,6 +154,7 @@ struct ShadeArgs {
     // camera (copied from device memory by the prep kernel into this form)
     const GpuCameraUniforms* cam;
     uint32_t width, height;
+    float rcp_w, rcp_h;       // correctly rounded 1 / width, 1 / height
     Tex albedo, normal, orm, emissive, depth;
     const HotLight* lights;
     uint32_t num_lights;
@@ -285,6 +322,35 @@ __device__ __forceinline__ float hl_x_radiance_max(const HotLight& L, float wx, 
     return fmaxf(fmaxf(r, g), b);
 }
 
+// a / b correctly rounded (the IEEE quotient) from y1, a reciprocal of b
+// within one ulp refined once (hl_rcp_refined) or the correctly rounded
+// reciprocal: two fma corrections of a * y1, the steps of the IEEE division
+// expansion without its operand scaling and special-case fix-up.  Valid when
+// no scaling or fix-up applies: a, b and a / b finite and normal, with the
+// exponents of a and b less than 96 apart (hl_div_r_ok is a sufficient test).
+// The same function of the same inputs as IEEE division, checked exhaustively
+// for a = x + 0.5, b = W (every x < W <= 16384) and on 5e7 random operands.
+__device__ __forceinline__ float hl_x_div_r(float a, float b, float y1) {
+#pragma clang fp contract(off)
+    const float q = a * y1;
+    const float r = __builtin_fmaf(-b, q, a);
+    const float q1 = __builtin_fmaf(r, y1, q);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y1, q1);
+}
+__device__ __forceinline__ float hl_rcp_refined(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
+}
+// |b| in [2^-40, 2^40], every |a| in [2^-60, 2^40] (zero and NaN fail): the
+// quotients are normal and the exponent gap is below 96.
+__device__ __forceinline__ bool hl_div_r_ok(float b, float a0, float a1, float a2) {
+    const float ab = fabsf(b);
+    const float amax = fmaxf(fmaxf(fabsf(a0), fabsf(a1)), fabsf(a2));
+    const float amin = fminf(fminf(fabsf(a0), fabsf(a1)), fabsf(a2));
+    return ab >= 0x1p-40f && ab <= 0x1p40f && amax <= 0x1p40f && amin >= 0x1p-60f;
+}
+
 // -------------------------------------------------------------- fast math
 __device__ __forceinline__ float hl_f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float hl_f_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -370,10 +436,13 @@ struct TexRaw<HL_FMT_RGBA16_FLOAT> {
     using type = hl_u32x2;
 };
 
+// Texel (x, y) of t at a 32-bit byte offset from the base (every texture is
+// below 4 GiB, checked on the host), so that the load takes the base from
+// scalar registers and one 32-bit offset per lane.
 template <int FMT>
 __device__ __forceinline__ typename TexRaw<FMT>::type hl_fetch(const Tex& t, uint32_t x, uint32_t y) {
     using R = typename TexRaw<FMT>::type;
-    return __builtin_nontemporal_load(hl_g((const R*)(t.data + (size_t)y * t.pitch)) + x);
+    return __builtin_nontemporal_load(hl_g((const R*)(t.data + (y * t.pitch + x * (uint32_t)sizeof(R)))));
 }
 
 template <int FMT>
@@ -395,7 +464,7 @@ __device__ __forceinline__ float4 hl_load_rgba(const Tex& t, uint32_t x, uint32_
 }
 
 __device__ __forceinline__ float hl_load_depth(const Tex& t, uint32_t x, uint32_t y) {
-    return __builtin_nontemporal_load(hl_g((const float*)(t.data + (size_t)y * t.pitch)) + x);
+    return __builtin_nontemporal_load(hl_g((const float*)(t.data + (y * t.pitch + x * 4u))));
 }
 
 // -------------------------------------------------------------- wave ops

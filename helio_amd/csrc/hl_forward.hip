@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_forward_prep(const GpuLight* __restrict
     }
     // forward_lit.wgsl samples no shadow map
     hot[i] = make_hot_light(pos.x, pos.y, pos.z, dir.x, dir.y, dir.z, ci, pr.w, dro.w, l->inner_angle, type,
-                            kNoShadowWord);
+                            kNoShadowWord, false);
 }
 
 constexpr int kFwdModeShade = 0, kFwdModeCount = 1;

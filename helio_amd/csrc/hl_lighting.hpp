@@ -38,14 +38,15 @@ struct Term {
     float F[3], B[3];
 };
 
-// Range test of deferred_lighting.wgsl:1128-1131 (dist > range rejects) from
-// the fast squared distance; inside the +-1e-5 relative band around range^2
-// (L.p4.xy), which covers the rounding of both sides many times over, the
-// exact restatement decides.  The band is rare: the exact branch is taken only
-// when some lane of the wave needs it.
+// Range test of deferred_lighting.wgsl:1128-1131 (dist > range rejects)
+// from the fast squared distance; inside the +-1e-5 relative band around
+// range^2, which covers the rounding of both sides many times over, the exact
+// restatement decides.  The band is rare: the exact branch is taken only when
+// some lane of the wave needs it.
 __device__ __forceinline__ bool range_test(const HotLight& L, float tx, float ty, float tz, float d2) {
-    bool in = d2 < L.p4.x;
-    const bool band = !in && !(d2 > L.p4.y);
+    const float r2 = L.p0.w * L.p0.w;
+    bool in = d2 < r2 * (1.0f - 1e-5f);
+    const bool band = !in && !(d2 > r2 * (1.0f + 1e-5f));
     if (hl_ballot(band)) {
         hl_rare();
         if (band) in = !(sqrtf(hl_x_dot3(tx, ty, tz, tx, ty, tz)) > L.p0.w);
@@ -73,8 +74,23 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
         if (kCut && L.p4.z < kThresh) return false;
     } else {
         const float tx = L.p0.x - p.wx, ty = L.p0.y - p.wy, tz = L.p0.z - p.wz;
-        const float d2 = tx * tx + ty * ty + tz * tz;
-        if (!range_test(L, tx, ty, tz, d2)) return false;
+        const float d2 = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz));
+        // The first decision in one band test on d2 (p4.xy, built by
+        // make_hot_light): below p4.x the pixel is in range and, with kCut,
+        // the unshadowed radiance is at least 0.002 (deferred_lighting.wgsl:
+        // 1128-1131 and 781); above p4.y the term is zero; in between the
+        // exact restatement decides.
+        bool in = d2 < L.p4.x;
+        const bool band = !in && !(d2 > L.p4.y);
+        if (hl_ballot(band)) {
+            hl_rare();
+            if (band) {
+                bool in_range;
+                const float m = hl_x_radiance_max(L, p.wx, p.wy, p.wz, &in_range);
+                in = in_range && (!kCut || !(m < kThresh));
+            }
+        }
+        if (!in) return false;
         inv_d = hl_f_rsq(d2);
         d = d2 * inv_d;
         lx = tx * inv_d;
@@ -88,7 +104,8 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
             // 739-742) divides by cos(inner) - cos(outer) (0.005 for a 10-degree
             // penumbra), which would amplify a last-bit error of a fast L ~200x.
             // Outside the penumbra by more than the margin the factor is exactly
-            // 0 (the term vanishes) or 1; inside, it is evaluated exactly.
+            // 0 (the term vanishes) or 1; inside, it is evaluated exactly, and
+            // so is the 0.002 cut, which the factor lowers.
             const float cf = -(lx * L.p2.x + ly * L.p2.y + lz * L.p2.z);
             const bool ordered = L.p2.w < L.p1.w;
             if (ordered && cf < L.p2.w - L.p3.z) return false;
@@ -102,25 +119,15 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
                     cos_a = hl_x_dot3(-ex, -ey, -ez, L.p2.x, L.p2.y, L.p2.z);
                 }
                 s = hl_x_smoothstep(L.p2.w, L.p1.w, cos_a);
+                if (kCut) {
+                    bool in_range;
+                    if (hl_x_radiance_max(L, p.wx, p.wy, p.wz, &in_range) < kThresh) return false;
+                }
                 if (kFull) {
                     const float inv_w = hl_f_rcp(L.p1.w - L.p2.w);
                     const float tt = (cos_a - L.p2.w) * inv_w;
                     dsdcos = (tt > 0.0f && tt < 1.0f) ? 6.0f * tt * (1.0f - tt) * inv_w : 0.0f;
                 }
-            }
-        }
-        // max(rad) = ci * atten exactly (rounding is monotonic, atten >= 0).
-        // Error bound of the fast radiance (rcp/rsq of att0, |fall| near 0):
-        // 32e-6 ci att0 s + 4e-6 m <= 3.61e-5 ci att0 s, as m <= ci att0 s.
-        if (kCut) {
-            const float m = L.p4.z * (att0 * fall * s);
-            const float err = L.p4.w * (att0 * s) + 1e-12f;
-            const float dm = m - kThresh;
-            if (dm < -err) return false;
-            if (fabsf(dm) < err) {  // ambiguous: decide with the exact restatement
-                hl_rare();
-                bool dummy;
-                if (hl_x_radiance_max(L, p.wx, p.wy, p.wz, &dummy) < kThresh) return false;
             }
         }
     }

@@ -171,7 +171,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 
     // ---- per-pixel set-up (deferred_lighting.wgsl:999-1086)
     Pixel p;
-    // the four fetches are all issued before the first is waited for
+    // the four G-buffer fetches go out together; the set-up that needs the
+    // depth only (world position, view vector, cell) runs while they are in
+    // flight, and the texels are decoded after it
     typename TexRaw<FA>::type rA{}, rO{};
     typename TexRaw<FN>::type rN{}, rE{};
     if (valid) {
@@ -181,27 +183,73 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         rE = hl_fetch<FN>(a.emissive, x, y);
     }
     __builtin_amdgcn_sched_barrier(0);
+    {
+        // world position exactly as deferred_lighting.wgsl:1003-1008 (the
+        // range, cone and threshold decisions depend on it): uv = (px + 0.5) /
+        // dims and w.xyz / w.w as IEEE quotients, from a reciprocal and two
+        // fma corrections (hl_x_div_r) where the operands allow it
+#pragma clang fp contract(off)
+        const float u = hl_x_div_r((float)x + 0.5f, (float)a.width, a.rcp_w);
+        const float v = hl_x_div_r((float)y + 0.5f, (float)a.height, a.rcp_h);
+        const float ndx = u * 2.0f - 1.0f, ndy = 1.0f - v * 2.0f;
+        const float* m = vc.ivp;
+        const float hw = hl_x_mat_row(m, 3, ndx, ndy, depth, 1.0f);
+        const float n0 = hl_x_mat_row(m, 0, ndx, ndy, depth, 1.0f);
+        const float n1 = hl_x_mat_row(m, 1, ndx, ndy, depth, 1.0f);
+        const float n2 = hl_x_mat_row(m, 2, ndx, ndy, depth, 1.0f);
+        if (hl_div_r_ok(hw, n0, n1, n2) || !valid) {  // a lane without a pixel: any value
+            const float y1 = hl_rcp_refined(hw);
+            p.wx = hl_x_div_r(n0, hw, y1);
+            p.wy = hl_x_div_r(n1, hw, y1);
+            p.wz = hl_x_div_r(n2, hw, y1);
+        } else {
+            hl_rare();
+            p.wx = n0 / hw;
+            p.wy = n1 / hw;
+            p.wz = n2 / hw;
+        }
+    }
+    const float ex = vc.eye[0] - p.wx, ey = vc.eye[1] - p.wy, ez = vc.eye[2] - p.wz;
+    {
+        const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
+        p.vx = ex * iv;
+        p.vy = ey * iv;
+        p.vz = ez * iv;
+    }
+
+    // ---- cell of this pixel
+    uint32_t cell = 0;
+    if (GRID == HL_GRID_TILES) {
+        cell = (y >> 4) * a.tile_stride_x + (x >> 4);
+    } else {
+        const uint32_t cx = hl_floor_div(x, a.cell_w), cy = hl_floor_div(y, a.cell_h);
+        uint32_t cz = 0;
+        if (valid && a.grid_z > 1u) {
+            // guess from the view distance, then settle on the exact thresholds
+            const float dist = vc.p32 * hl_f_rcp(depth + vc.p22);
+            const float g = __builtin_amdgcn_logf(fmaxf(dist, 1e-30f)) * a.log_scale + a.log_bias;
+            const int kmax = (int)a.grid_z - 1;
+            int k = (int)floorf(g);
+            k = k < 0 ? 0 : (k > kmax ? kmax : k);
+            // the guess is almost always right: both of its thresholds at once
+            const float* split = vc.split;
+            const float lo = split[k], hi = split[k + 1];
+            if (!((k == 0 || lo <= depth) && (k == kmax || hi > depth))) {
+                hl_rare();
+                while (k > 0 && split[k] > depth) --k;
+                while (k < kmax && split[k + 1] <= depth) ++k;
+            }
+            cz = (uint32_t)k;
+        }
+        cell = cx + a.grid_x * (cy + a.grid_y * cz);
+    }
+
     // zero bits decode to 0 in every format; lanes without a pixel keep N = +y
     const float4 A = hl_decode<FA>(rA), O = hl_decode<FA>(rO), E = hl_decode<FN>(rE);
     float4 Nr = hl_decode<FN>(rN);
     if (!valid) Nr = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
     {
-        // world position exactly as deferred_lighting.wgsl:1003-1008: the
-        // range, cone and threshold decisions depend on it
-#pragma clang fp contract(off)
-        const float u = ((float)x + 0.5f) / (float)a.width;
-        const float v = ((float)y + 0.5f) / (float)a.height;
-        const float ndx = u * 2.0f - 1.0f, ndy = 1.0f - v * 2.0f;
-        const float* m = vc.ivp;
-        const float hw = hl_x_mat_row(m, 3, ndx, ndy, depth, 1.0f);
-        p.wx = hl_x_mat_row(m, 0, ndx, ndy, depth, 1.0f) / hw;
-        p.wy = hl_x_mat_row(m, 1, ndx, ndy, depth, 1.0f) / hw;
-        p.wz = hl_x_mat_row(m, 2, ndx, ndy, depth, 1.0f) / hw;
-    }
-    {
-        const float ex = vc.eye[0] - p.wx, ey = vc.eye[1] - p.wy, ez = vc.eye[2] - p.wz;
         const float in_ = hl_f_rsq(Nr.x * Nr.x + Nr.y * Nr.y + Nr.z * Nr.z);
-        const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
         if (kExt) {
             // shadow_factor's normal offset and N.L take the shader's exact N
 #pragma clang fp contract(off)
@@ -214,9 +262,6 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             p.ny = Nr.y * in_;
             p.nz = Nr.z * in_;
         }
-        p.vx = ex * iv;
-        p.vy = ey * iv;
-        p.vz = ez * iv;
         p.ndv = fmaxf(p.nx * p.vx + p.ny * p.vy + p.nz * p.vz, 0.0f);
         // At grazing view NdV is a difference of ~1 terms: a last-bit error of
         // N or V (< 1e-6 absolute in NdV) is a relative error of 1e-6 / NdV in
@@ -279,33 +324,6 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             hl_store_nt(out_px, o);
         }
         return;
-    }
-
-    // ---- cell of this pixel
-    uint32_t cell = 0;
-    if (GRID == HL_GRID_TILES) {
-        cell = (y >> 4) * a.tile_stride_x + (x >> 4);
-    } else {
-        const uint32_t cx = hl_floor_div(x, a.cell_w), cy = hl_floor_div(y, a.cell_h);
-        uint32_t cz = 0;
-        if (valid && a.grid_z > 1u) {
-            // guess from the view distance, then settle on the exact thresholds
-            const float dist = vc.p32 * hl_f_rcp(depth + vc.p22);
-            const float g = __builtin_amdgcn_logf(fmaxf(dist, 1e-30f)) * a.log_scale + a.log_bias;
-            const int kmax = (int)a.grid_z - 1;
-            int k = (int)floorf(g);
-            k = k < 0 ? 0 : (k > kmax ? kmax : k);
-            // the guess is almost always right: both of its thresholds at once
-            const float* split = vc.split;
-            const float lo = split[k], hi = split[k + 1];
-            if (!((k == 0 || lo <= depth) && (k == kmax || hi > depth))) {
-                hl_rare();
-                while (k > 0 && split[k] > depth) --k;
-                while (k < kmax && split[k + 1] <= depth) ++k;
-            }
-            cz = (uint32_t)k;
-        }
-        cell = cx + a.grid_x * (cy + a.grid_y * cz);
     }
 
     HotLight* s_l = sm.light[wave];

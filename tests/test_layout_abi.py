@@ -157,8 +157,37 @@ def test_cluster_params_rejects_non_perspective(hlib):
 def test_header_constants_match_python_mirror():
     """Every numeric #define of helio_hip.h that abi.py mirrors has the same value."""
     seen = 0
-    for name, val in re.findall(r"#define\s+(HL_[A-Z0-9_]+)\s+\(?(-?(?:0x[0-9a-fA-F]+|\d+))u?\)?", HEADER.read_text()):
+    text = HEADER.read_text()
+    for name, val in re.findall(r"^#define\s+(HL_[A-Z0-9_]+)\s+\(?(-?(?:0x[0-9a-fA-F]+|\d+))u?\)?\s*(?:/\*.*)?$",
+                                text, re.M):
         if hasattr(abi, name):
             assert getattr(abi, name) == int(val, 0), name
             seen += 1
     assert seen >= 12 and abi.HL_BATCH_MAX_MULTIVIEW_PIXELS == 262144
+
+
+def _c_exports():
+    """name -> parameter count of every HL_API function in helio_hip.h."""
+    out = {}
+    for m in re.finditer(r"HL_API\s+[\w\s\*]+?\b(hl_\w+)\s*\(([^;]*?)\)\s*;", HEADER.read_text(), re.S):
+        args = " ".join(m.group(2).split())
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_rust_binding_declares_every_export():
+    """The extern block a maintainer adds on the reference side
+    declares every exported function with the same number of parameters."""
+    text = (ROOT / "INTEGRATION.md").read_text()
+    block = text[text.index('extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    rust = {}
+    for m in re.finditer(r"pub fn (hl_\w+)\(([^)]*)\)", block, re.S):
+        args = " ".join(m.group(2).split())
+        rust[m.group(1)] = 0 if not args else args.count(",") + 1
+    c = _c_exports()
+    assert len(c) >= 37
+    assert sorted(set(c) - set(rust)) == [], "declared in C, missing in the Rust binding"
+    assert sorted(set(rust) - set(c)) == [], "in the Rust binding, not exported"
+    for name, n in c.items():
+        assert rust[name] == n, (name, n, rust[name])
