@@ -26,6 +26,8 @@ HL_SHADE_LIGHTS_FROM_CULL = 2
 HL_OVERFLOW_CAP, HL_OVERFLOW_POOL = 1, 2
 HL_PIXEL_GRAD_PLANES = 5
 HL_BATCH_MAX_VIEWS, HL_BATCH_MAX_MULTIVIEW_PIXELS = 255, 262144
+HL_NOALLOC_MAX_LIGHTS = 65536
+HL_NOALLOC_SCRATCH_BYTES = 16384 + 80 * HL_NOALLOC_MAX_LIGHTS
 
 # GpuLight (crates/libhelio/src/light.rs:60-106), 128 bytes.
 GPU_LIGHT_DTYPE = np.dtype([
@@ -244,6 +246,8 @@ SIGNATURES: dict[str, tuple] = {
     "hl_gbuffer_pack": (None, [vp, P(hl_surface), P(hl_gbuffer)]),
     "hl_deferred_forward": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                    P(hl_light_grid), vp, u32, u32]),
+    "hl_deferred_forward_noalloc": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
+                                           P(hl_light_grid), vp, u32, u32]),
     "hl_deferred_forward_shadowed": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                             P(hl_light_grid), P(hl_shadow_maps), vp, u32, u32]),
     "hl_deferred_forward_ex": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32, P(hl_light_grid),

@@ -7,6 +7,7 @@
 
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "hl_internal.hpp"
 
@@ -292,6 +293,42 @@ hl_status shade_launch(hl_context* c, int set, int mode, const ShadeArgs& a) {
 
 }  // namespace
 
+// ------------------------------------------- scratch of the no-alloc entry
+// hl_deferred_forward_noalloc keeps its scratch -- two sets of claim counters
+// and the shading records of up to HL_NOALLOC_MAX_LIGHTS lights -- in this
+// file-scope device array rather than in context allocations.  Each device
+// holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: it uses the device address hipGetSymbolAddress
+// returns for the current device, looked up once per device (noalloc_setup,
+// from hl_context_create) and kept in g_scratch_ptr and the context.
+constexpr size_t kNoAllocLightsOff = 2 * 4 * (size_t)kShadeQueueWords;
+constexpr size_t kNoAllocBytes = kNoAllocLightsOff + sizeof(HotLight) * (size_t)HL_NOALLOC_MAX_LIGHTS;
+static_assert(kNoAllocLightsOff % 256 == 0, "light records start 256-B aligned");
+static_assert(kNoAllocBytes == HL_NOALLOC_SCRATCH_BYTES, "HL_NOALLOC_SCRATCH_BYTES states the array's size");
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device ordinal; null until looked up
+
+// The device of c is current.
+hl_status noalloc_setup(hl_context* c) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)c->device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)c->device + 1, nullptr);
+    if (!g_scratch_ptr[(size_t)c->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(c, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                   "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[(size_t)c->device] = p;
+    }
+    c->noalloc = g_scratch_ptr[(size_t)c->device];
+    return HL_OK;
+}
+
+}  // namespace
+
 // ------------------------------------------------------------------ exports
 extern "C" {
 
@@ -322,7 +359,7 @@ HL_API hl_status hl_context_create(int device, hl_context** out) {
         delete c;
         return HL_ENOMEM;
     }
-    if (hipMemset(c->words.p, 0, c->words.bytes) != hipSuccess) {
+    if (hipMemset(c->words.p, 0, c->words.bytes) != hipSuccess || noalloc_setup(c) != HL_OK) {
         (void)hipFree(c->words.p);
         delete c;
         return HL_EHIP;
@@ -704,6 +741,43 @@ HL_API hl_status hl_deferred_forward(hl_context* c, const GpuCameraUniforms* cam
                                      uint32_t out_pitch_bytes, uint32_t flags) {
     return run_forward(c, camera_d, g, gb, lights_d, num_lights, grid, nullptr, nullptr, out_d, out_pitch_bytes,
                        flags);
+}
+
+HL_API hl_status hl_deferred_forward_noalloc(hl_context* c, const GpuCameraUniforms* camera_d,
+                                             const DeferredGlobals* g, const hl_gbuffer* gb,
+                                             const GpuLight* lights_d, uint32_t num_lights,
+                                             const hl_light_grid* grid, float* out_d,
+                                             uint32_t out_pitch_bytes, uint32_t flags) {
+    if (!c) return HL_EINVAL;
+    if (!c->noalloc) return fail(c, HL_EHIP, "the no-alloc scratch of device %d is not set up", c->device);
+    if (!out_d) return fail(c, HL_EINVAL, "output is null");
+    if (flags & ~(uint32_t)HL_SHADE_CLEAR_SKY)
+        return fail(c, HL_EINVAL, "hl_deferred_forward_noalloc: flags 0x%x (HL_SHADE_CLEAR_SKY only)", flags);
+    if (num_lights > HL_NOALLOC_MAX_LIGHTS)
+        return fail(c, HL_EINVAL, "hl_deferred_forward_noalloc: %u lights, at most %u (HL_NOALLOC_MAX_LIGHTS)",
+                    num_lights, HL_NOALLOC_MAX_LIGHTS);
+    if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
+    ShadeArgs a;
+    int set;
+    hl_status s = build_shade_args(c, camera_d, g, gb, num_lights, grid, &a, &set, false);
+    if (s != HL_OK) return s;
+    if (out_pitch_bytes < 16u * gb->width || out_pitch_bytes % 16 != 0 || ((uintptr_t)out_d) % 16)
+        return fail(c, HL_EINVAL, "output pitch %u invalid for width %u (RGBA32F, 16-B aligned)",
+                    out_pitch_bytes, gb->width);
+    a.out = out_d;
+    a.out_pitch = out_pitch_bytes;
+    a.flags = flags;
+    HL_HIP(c, hipSetDevice(c->device));
+    uint8_t* base = (uint8_t*)c->noalloc;  // this device's address of `scratch`
+    uint32_t* queue = (uint32_t*)base;
+    HotLight* hot = (HotLight*)(base + kNoAllocLightsOff);
+    a.lights = hot;
+    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, num_lights, camera_d, hot, nullptr, nullptr));
+    // the array keeps the last call's counters: clear this launch's set (a
+    // fill kernel, so that the call can be captured into a graph)
+    HL_HIP(c, launch_fill_u32(c->stream, queue, 0, kShadeQueueWords));
+    HL_HIP(c, launch_shade(c->stream, set, kModeForward, a, queue, queue + kShadeQueueWords));
+    return HL_OK;
 }
 
 HL_API hl_status hl_deferred_forward_shadowed(hl_context* c, const GpuCameraUniforms* camera_d,

@@ -35,6 +35,9 @@ struct hl_context {
     std::string err;
     DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum, vogel, rcirr;
     bool vogel_ready = false;  // vogel holds the Vogel disk tables
+    // this device's address of the no-alloc scratch array (hl_api.hip),
+    // looked up once per device when the context is created
+    void* noalloc = nullptr;
     uint32_t shade_parity = 0;  // which half of shadeq the next shading launch claims from
     // the lights the records in `hot` were last built from (stream order)
     const GpuLight* hot_src = nullptr;

@@ -271,6 +271,25 @@ HL_API hl_status hl_deferred_forward(hl_context* ctx, const GpuCameraUniforms* c
                                      const hl_light_grid* grid, float* out_d,
                                      uint32_t out_pitch_bytes, uint32_t flags);
 
+/* No counterpart in the reference: hl_deferred_forward for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same arguments and the
+ * same shading (bit-identical output); the light records and the claim
+ * counters live in a fixed device array of the library
+ * (HL_NOALLOC_SCRATCH_BYTES on each device, held until the process ends)
+ * instead of context scratch, so the call allocates nothing.
+ * Limits: num_lights <= HL_NOALLOC_MAX_LIGHTS (HL_EINVAL beyond); flags may
+ * hold HL_SHADE_CLEAR_SKY only.  Every call on a device shares that device's
+ * array: calls that overlap on one device (different streams or contexts,
+ * not ordered by events) race and must be serialised by the caller. */
+#define HL_NOALLOC_MAX_LIGHTS 65536u
+#define HL_NOALLOC_SCRATCH_BYTES (16384u + 80u * HL_NOALLOC_MAX_LIGHTS)
+HL_API hl_status hl_deferred_forward_noalloc(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                             const DeferredGlobals* globals, const hl_gbuffer* gbuffer,
+                                             const GpuLight* lights_d, uint32_t num_lights,
+                                             const hl_light_grid* grid, float* out_d,
+                                             uint32_t out_pitch_bytes, uint32_t flags);
+
 /* Shadow maps shadow_factor samples (deferred_lighting.wgsl:234-565): the
  * dynamic and the static shadow atlas (Depth32Float texture arrays bound at
  * :160-161), num_layers square layers of atlas_size texels (ATLAS_SIZE = 1024

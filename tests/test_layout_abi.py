@@ -164,6 +164,8 @@ def test_header_constants_match_python_mirror():
             assert getattr(abi, name) == int(val, 0), name
             seen += 1
     assert seen >= 12 and abi.HL_BATCH_MAX_MULTIVIEW_PIXELS == 262144
+    m = re.search(r"#define HL_NOALLOC_SCRATCH_BYTES \((\d+)u \+ (\d+)u \* HL_NOALLOC_MAX_LIGHTS\)", text)
+    assert m and int(m.group(1)) + int(m.group(2)) * abi.HL_NOALLOC_MAX_LIGHTS == abi.HL_NOALLOC_SCRATCH_BYTES
 
 
 def _c_exports():
@@ -186,7 +188,7 @@ def test_rust_binding_declares_every_export():
         args = " ".join(m.group(2).split())
         rust[m.group(1)] = 0 if not args else args.count(",") + 1
     c = _c_exports()
-    assert len(c) >= 37
+    assert len(c) >= 38
     assert sorted(set(c) - set(rust)) == [], "declared in C, missing in the Rust binding"
     assert sorted(set(rust) - set(c)) == [], "in the Rust binding, not exported"
     for name, n in c.items():
