@@ -15,6 +15,8 @@ is the C restatement of the reference shaders (oracle/) on the host cores.
 Further legs, each reported in its own object of the same line:
   batched_4k      the same 4K frame as a one-view batched plan (persistent
                   shading launch), eager and replayed from its HIP graph
+  far_4k          the same lights and grid seen from 420 m out with a 90-degree
+                  field of view: cluster lists of 0 to 2000+ lights
   fwd_adjoint_512 512x512 fp32 G-buffer, 256 point lights: cluster cull +
                   fused forward / L2 loss / adjoint per step (and forward only)
   gbuffer_pack_4k the G-buffer producer's format contract (hl_gbuffer_pack) at
@@ -165,7 +167,7 @@ def bench_4k(ctx, args):
     from helio_amd import abi
     total, (cull_ms, shade_ms) = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
                                             lambda: shade.execute(ds, grid, flags=abi.HL_SHADE_LIGHTS_FROM_CULL))
-    res = {"scene": sc, "params": params, "pool_cap": pool_cap, "q": q, "ncell": ncell, "nlist": nlist,
+    res = {"scene": sc, "params": params, "pool_cap": pool_cap, "q": q, "ncell": ncell, "nlist": nlist, "counts": cnt,
            "ms_frame": total, "cull_ms": cull_ms, "shade_ms": shade_ms}
     if not args.no_batched:
         out = ctx.alloc(sc.width * sc.height * 16)
@@ -184,6 +186,23 @@ def bench_4k(ctx, args):
                           "note": "one-view plan on the same cluster grid: batched cull + persistent shading launch"}
         log(f"4K batched: graph {graph:.3f} ms, eager {eager:.3f} ms")
     return res
+
+
+def bench_far(ctx, args):
+    """The 4K/16k lights and grid seen from 420 m out with a 90-degree field of
+    view (c3_4k_16k_far): cluster lists of 0 to 2000+ lights."""
+    import argparse
+    a = argparse.Namespace(**vars(args))
+    a.workload, a.no_batched = "c3_4k_16k_far", True
+    r = bench_4k(ctx, a)
+    cnt = r["counts"]
+    log(f"4K far camera: {r['ms_frame']:.4f} ms/frame (shade {r['shade_ms'] * 1e3:.1f} us), Q={r['q']}, "
+        f"longest list {int(cnt.max())}")
+    return {"workload": "c3_4k_16k_far: the 4K/16k lights and 32x18x48 grid from (0, 60, 420) with a 90-degree "
+                        "field of view", "ms_per_frame": r["ms_frame"], "cull_ms": r["cull_ms"],
+            "shade_ms": r["shade_ms"], "pairs_q": r["q"], "evals_per_s": r["q"] / (r["ms_frame"] * 1e-3),
+            "cluster_entries": r["nlist"], "longest_list": int(cnt.max()),
+            "clusters_2000_plus": int((cnt >= 2000).sum())}
 
 
 def bench_gbuffer_pack(ctx, args):
@@ -372,6 +391,7 @@ def main() -> int:
     ap.add_argument("--no-forward", action="store_true", help="skip the 4K forward-lit leg")
     ap.add_argument("--no-shadows", action="store_true", help="skip the 4K shadowed leg")
     ap.add_argument("--no-reflection", action="store_true", help="skip the 4K reflection-composite leg")
+    ap.add_argument("--no-far", action="store_true", help="skip the 4K far-camera (long cluster lists) leg")
     ap.add_argument("--multiview", action="store_true",
                     help="the full-size 50-view batched leg: refused (its plan hit a device memory fault "
                          "once, cause not found; multi-view plans are limited to "
@@ -409,6 +429,7 @@ def main() -> int:
 
     with Context(0) as ctx:
         r4k = bench_4k(ctx, args)
+        far = None if args.no_far else bench_far(ctx, args)
         adj = None if args.no_adjoint else bench_adjoint(ctx, args)
         pack = None if args.no_pack else bench_gbuffer_pack(ctx, args)
         fwd = None if args.no_forward else bench_forward_lit(ctx, args)
@@ -465,6 +486,8 @@ def main() -> int:
     }
     if "batched" in r4k:
         res["batched_4k"] = r4k["batched"]
+    if far is not None:
+        res["far_4k"] = far
     if adj is not None:
         res["fwd_adjoint_512"] = adj
     if pack is not None:

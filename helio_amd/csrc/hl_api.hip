@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -141,6 +143,18 @@ hl_status check_cluster_params(hl_context* c, const hl_cluster_params* p) {
     for (uint32_t k = 1; k < p->grid_z; ++k)
         if (!(p->depth_split[k] >= p->depth_split[k - 1]))
             return fail(c, HL_EINVAL, "cluster depth_split is not non-decreasing at slice %u", k);
+    // the cull bounds a light's slices by searching slice_near and slice_far
+    // as sorted tables, under the padding of the farthest slice in reach
+    for (uint32_t k = 0; k < p->grid_z; ++k) {
+        if (!std::isfinite(p->slice_near[k]) || !std::isfinite(p->slice_far[k]) ||
+            !std::isfinite(p->slice_r_pad[k]) || !(p->slice_r_pad[k] >= 0.0f))
+            return fail(c, HL_EINVAL, "cluster slice %u: slice_near, slice_far and slice_r_pad must be finite, "
+                        "the padding non-negative", k);
+        if (k > 0 && !(p->slice_near[k] >= p->slice_near[k - 1] && p->slice_far[k] >= p->slice_far[k - 1] &&
+                       p->slice_r_pad[k] >= p->slice_r_pad[k - 1]))
+            return fail(c, HL_EINVAL, "cluster slice_near, slice_far and slice_r_pad must be non-decreasing "
+                        "(slice %u)", k);
+    }
     return HL_OK;
 }
 
@@ -535,6 +549,16 @@ HL_API hl_status hl_cluster_params_init(const GpuCameraUniforms* cam, uint32_t w
         out->slice_r_pad[k] = (float)pad;
         if (pad > maxpad) maxpad = pad;
     }
+    // Non-decreasing tables (the cull searches them): a slice's near limit is
+    // lowered to the next one's, its far limit and padding raised to the
+    // previous one's where FP32 depths round to equal distances -- wider, so
+    // still conservative.
+    for (uint32_t k = gz - 1; k-- > 0;)
+        out->slice_near[k] = std::min(out->slice_near[k], out->slice_near[k + 1]);
+    for (uint32_t k = 1; k < gz; ++k) {
+        out->slice_far[k] = std::max(out->slice_far[k], out->slice_far[k - 1]);
+        out->slice_r_pad[k] = std::max(out->slice_r_pad[k], out->slice_r_pad[k - 1]);
+    }
     out->r_pad_column = (float)maxpad;
     return HL_OK;
 }
@@ -591,7 +615,7 @@ HL_API hl_status hl_light_cull_clusters(hl_context* c, const GpuCameraUniforms* 
     if ((s = ensure(c, c->markq, mark_queue_bytes(n))) != HL_OK) return s;
     HL_HIP(c, hipSetDevice(c->device));
     // the overflow word(s) are reset by the first cull kernel and OR-ed by the
-    // scan: the caller's word when given, and always the context's copy
+    // list kernel: the caller's word when given, and always the context's copy
     uint32_t* flags = overflow_d ? overflow_d : ctx_words(c);
     uint32_t* flags2 = overflow_d ? ctx_words(c) : nullptr;
     ClusterArgs a = make_cluster_args(p, n);

@@ -13,11 +13,13 @@
 //                     (bits + a summary bit per non-empty word); past its first
 //                     chunks a light queues the rest
 //  k_cluster_mark_tail the queued chunks, spread over all CUs
-//  k_cluster_count    per cluster (16 lanes): popcount over its non-empty words
-//  k_cluster_scan     exclusive scan of the capped counts, a block per tile of
-//                     1024 clusters (tile sums come from the count kernel)
-//  k_cluster_fill     per cluster (16 lanes): ascending indices from the
+//  k_cluster_count    per cluster (16 lanes): popcount over its non-empty
+//                     words; sums of the capped counts per scan tile of 1024
+//  k_cluster_fill     per block of 16 clusters: its offsets from the tile
+//                     sums and the counts before it (the exclusive scan, no
+//                     separate launch), then the ascending indices from the
 //                     bitmask, which it clears again for the next call
+//  k_batch_*          the same steps for many views in one launch each
 // Membership is decided by the same exact FP32 tests whatever the order of
 // evaluation, so the lists equal the per-cluster brute force bit for bit.
 #include "hl_internal.hpp"
@@ -231,58 +233,74 @@ __device__ __forceinline__ bool cell_member(const CullLight& c, const ColumnGeom
     return in;
 }
 
-// Candidate cells of a light: the slices its view distance +- (radius +
-// padding) spans and the columns whose lateral planes it can pass at its own
-// depth, each widened by one.  False when there are none.
+// Candidate cells of a light: the columns and slices left by necessary
+// conditions of the membership rule, rounded outwards.
+//  * slices: the box test needs slice_near[k] <= zc + rc and slice_far[k] >=
+//    zc - rc (zc the light's view distance, rc = |radius| + pad_k, at most rr,
+//    the radius plus the largest padding of the slices in reach: paddings do
+//    not decrease with k);
+//  * columns: a column with NDC edges l < r (|l|, |r| <= 1) passes its left
+//    plane (normal (1, 0, l / p00) normalised) only if
+//    l <= p00 (cx + rc sqrt(1 + l^2 / p00^2)) / zc <= p00 (cx + rr fx) / zc
+//    with fx = sqrt(1 + 1 / p00^2), and its right plane only if
+//    r >= p00 (cx - rr fx) / zc (zc > 0, p00 > 0); in pixels, the columns
+//    floor(px0 / cell_w) .. floor(px1 / cell_w).  Rows alike with p11.
+// The FP32 tests can pass a few ulp beyond these real-number bounds, and the
+// bounds are computed with fast reciprocals and square roots (a few ulp of the
+// terms): rr is widened by 1e-5 of itself and of |cx| + |cy| + zc, the pixel
+// bounds by 1e-5 of their magnitude plus a quarter pixel, the slice limits by
+// 1e-5 relative -- each many times those errors.  Directional lights, lights
+// with a non-finite centre and a camera with p00 or p11 <= 0 take the whole
+// grid (or the whole screen); the exact tests decide there.
 struct MarkBox {
     uint32_t x0, y0, s0, nx, ny, nz;
 };
 
-// first k in [0, n) with v[k] >= x, for ascending v held one per lane (lane
-// k holds v[k], n <= 64): the number of lanes below n with v[k] < x
-__device__ __forceinline__ int lane_lower_bound(float vk, int n, float x) {
-    return (int)__popcll(hl_ballot((int)hl_lane() < n && vk < x));
-}
-
-// All 64 lanes must be active (the slice bounds are found by ballots); sp is
-// slice_r_pad[lane] (lanes below grid_z).
+// All 64 lanes must be active (the slice limits are found by ballots); sp is
+// slice_r_pad[lane] (lanes below grid_z; slice_near, slice_far and the
+// paddings are non-decreasing, checked by the host).
 __device__ __forceinline__ bool light_cells(const ClusterArgs& a,
                                             const GpuCameraUniforms* __restrict__ cam,
                                             const CullLight& L, float sp, MarkBox& m) {
     const int gx = (int)a.grid_x, gy = (int)a.grid_y, gz = (int)a.grid_z;
     const uint32_t lane = hl_lane();
-    const float sn = lane < a.grid_z ? a.slice_near[lane] : 0.0f;
-    const float sf = lane < a.grid_z ? a.slice_far[lane] : 0.0f;
+    const bool sl = (int)lane < gz;
+    const float vn = sl ? a.slice_near[lane] : 0.0f, vf = sl ? a.slice_far[lane] : 0.0f;
+    const float sn = vn - 1e-5f * fabsf(vn), sf = vf + 1e-5f * fabsf(vf);
     int x0 = 0, x1 = gx - 1, y0 = 0, y1 = gy - 1, s0 = 0, s1 = gz - 1;
-    if (!__builtin_isinf(L.vpos.w)) {
-        const float zc = -L.vpos.z;
-        // slices under the largest padding, then again under the largest
-        // padding of those slices (paddings grow with distance)
-        float rr = L.vpos.w + a.r_pad_column;
-        s1 = clampi(lane_lower_bound(sn, gz, zc + rr) + 1, 0, gz - 1);
-        rr = L.vpos.w + hl_readlane(sp, (uint32_t)s1);
-        s0 = lane_lower_bound(sf, gz, zc - rr) - 1;
-        s1 = lane_lower_bound(sn, gz, zc + rr) + 1;
-        if (zc > 1e-4f) {
-            // A lateral plane through the origin at angle alpha from the view
-            // axis passes a centre up to rr / cos(alpha) beyond the column's
-            // edge at the centre's own depth; 1/cos(alpha) <= sqrt(1 + 1/p00^2).
-            // fast arithmetic: a bound of candidates, widened by a whole cell
-            // on each side below, far beyond its rounding
-            const float p00 = cam->proj[0], p11 = cam->proj[5];
+    const float zc = -L.vpos.z;
+    if (!__builtin_isinf(L.vpos.w) && __builtin_isfinite(zc) && __builtin_isfinite(L.vpos.x) &&
+        __builtin_isfinite(L.vpos.y)) {
+        const float slack = 1e-5f * (fabsf(L.vpos.x) + fabsf(L.vpos.y) + fabsf(zc));
+        const float r = fabsf(L.vpos.w);
+        // slices in reach under the largest padding, then under the largest
+        // padding of those slices
+        float rr = (r + hl_readlane(sp, (uint32_t)(gz - 1))) * (1.0f + 1e-5f) + slack;
+        s1 = (int)__popcll(hl_ballot(sl && sn <= zc + rr)) - 1;
+        if (s1 < 0) return false;
+        rr = (r + hl_readlane(sp, (uint32_t)s1)) * (1.0f + 1e-5f) + slack;
+        s0 = (int)__popcll(hl_ballot(sl && sf < zc - rr));
+        s1 = (int)__popcll(hl_ballot(sl && sn <= zc + rr)) - 1;
+        const float p00 = cam->proj[0], p11 = cam->proj[5];
+        if (zc > 1e-4f && p00 > 0.0f && p11 > 0.0f) {
             const float fx = __builtin_amdgcn_sqrtf(1.0f + hl_f_rcp(p00 * p00));
             const float fy = __builtin_amdgcn_sqrtf(1.0f + hl_f_rcp(p11 * p11));
             const float iz = hl_f_rcp(zc);
             const float nx0 = p00 * (L.vpos.x - rr * fx) * iz, nx1 = p00 * (L.vpos.x + rr * fx) * iz;
             const float ny0 = p11 * (L.vpos.y - rr * fy) * iz, ny1 = p11 * (L.vpos.y + rr * fy) * iz;
             const float fw = (float)a.width, fh = (float)a.height;
-            const float px0 = (nx0 + 1.0f) * 0.5f * fw, px1 = (nx1 + 1.0f) * 0.5f * fw;
-            const float py0 = (1.0f - ny1) * 0.5f * fh, py1 = (1.0f - ny0) * 0.5f * fh;
+            float px0 = (nx0 + 1.0f) * 0.5f * fw, px1 = (nx1 + 1.0f) * 0.5f * fw;
+            float py0 = (1.0f - ny1) * 0.5f * fh, py1 = (1.0f - ny0) * 0.5f * fh;
+            px0 -= 1e-5f * fabsf(px0) + 0.25f;
+            px1 += 1e-5f * fabsf(px1) + 0.25f;
+            py0 -= 1e-5f * fabsf(py0) + 0.25f;
+            py1 += 1e-5f * fabsf(py1) + 0.25f;
             const float icw = hl_f_rcp((float)a.cell_w), ich = hl_f_rcp((float)a.cell_h);
-            x0 = (int)fmaxf(floorf(px0 * icw) - 1.0f, -1.0f);
-            x1 = (int)fminf(floorf(px1 * icw) + 1.0f, (float)gx);
-            y0 = (int)fmaxf(floorf(py0 * ich) - 1.0f, -1.0f);
-            y1 = (int)fminf(floorf(py1 * ich) + 1.0f, (float)gy);
+            // clamped to [-1, g] before the conversion (NaN: the whole range)
+            x0 = (int)fminf(fmaxf(floorf(px0 * icw), -1.0f), (float)gx);
+            x1 = (int)fmaxf(fminf(floorf(px1 * icw), (float)gx), -1.0f);
+            y0 = (int)fminf(fmaxf(floorf(py0 * ich), -1.0f), (float)gy);
+            y1 = (int)fmaxf(fminf(floorf(py1 * ich), (float)gy), -1.0f);
         }
     }
     x0 = clampi(x0, 0, gx - 1);
@@ -370,15 +388,25 @@ __device__ __forceinline__ ChunkCell chunk_fetch(const ClusterArgs& a, const Mar
     return r;
 }
 
+__device__ __forceinline__ size_t cluster_cells(const ClusterArgs& a) {
+    return (size_t)a.grid_x * a.grid_y * a.grid_z;
+}
+
+// The bitmask is stored word-major: word w of cluster c (lights 32 w .. 32 w
+// + 31) at bits[w * ncell + c], summary word s (bitmask words 32 s .. 32 s +
+// 31) at summary[s * ncell + c].  A light's member cells in one row of one
+// slice are then adjacent words (fewer lines for its atomics), and the list
+// kernel's 16 consecutive clusters read their words of one light range from
+// one line.
 __device__ __forceinline__ void chunk_mark(const CullLight& L, const ChunkCell& r, uint32_t li,
                                            uint32_t* __restrict__ bits, uint32_t* __restrict__ summary,
-                                           uint32_t W, uint32_t SW) {
+                                           size_t ncell) {
     if (!r.on || !cell_member(L, r.g, r.bx, r.pad)) return;
     const uint32_t word = li >> 5;
-    __hip_atomic_fetch_or(hl_g(&bits[r.cell * W + word]), 1u << (li & 31u), __ATOMIC_RELAXED,
+    __hip_atomic_fetch_or(hl_g(&bits[(size_t)word * ncell + r.cell]), 1u << (li & 31u), __ATOMIC_RELAXED,
                           __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_or(hl_g(&summary[r.cell * SW + (word >> 5)]), 1u << (word & 31u), __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(hl_g(&summary[(size_t)(word >> 5) * ncell + r.cell]), 1u << (word & 31u),
+                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight& L,
@@ -386,9 +414,8 @@ __device__ __forceinline__ void mark_chunk(const ClusterArgs& a, const CullLight
                                            const ColumnGeom* __restrict__ geom,
                                            const CellBox* __restrict__ boxes,
                                            uint32_t* __restrict__ bits,
-                                           uint32_t* __restrict__ summary, uint32_t W,
-                                           uint32_t SW, uint32_t chunk) {
-    chunk_mark(L, chunk_fetch(a, m, geom, boxes, chunk), li, bits, summary, W, SW);
+                                           uint32_t* __restrict__ summary, uint32_t chunk) {
+    chunk_mark(L, chunk_fetch(a, m, geom, boxes, chunk), li, bits, summary, cluster_cells(a));
 }
 
 __device__ __forceinline__ CullLight load_light_uniform(const CullLight* __restrict__ cl, uint32_t li) {
@@ -414,7 +441,7 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
                                            const ColumnGeom* __restrict__ geom,
                                            const CellBox* __restrict__ boxes,
                                            uint32_t* __restrict__ bits,
-                                           uint32_t* __restrict__ summary, uint32_t W, uint32_t SW,
+                                           uint32_t* __restrict__ summary,
                                            uint32_t li, const MarkQueue& q, uint32_t qlight,
                                            uint32_t qview) {
     if (li >= a.num_lights) return;
@@ -447,10 +474,10 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
         }
     }
     // chunks [0, qlo) and [qhi, nchunk) stay here (qlo >= 2 whenever nchunk >= 2)
-    chunk_mark(L, r0, li, bits, summary, W, SW);
-    if (nchunk > 1u) chunk_mark(L, r1, li, bits, summary, W, SW);
-    for (uint32_t c = 2u; c < qlo; ++c) mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, c);
-    for (uint32_t c = qhi; c < nchunk; ++c) mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, c);
+    chunk_mark(L, r0, li, bits, summary, cluster_cells(a));
+    if (nchunk > 1u) chunk_mark(L, r1, li, bits, summary, cluster_cells(a));
+    for (uint32_t c = 2u; c < qlo; ++c) mark_chunk(a, L, m, li, geom, boxes, bits, summary, c);
+    for (uint32_t c = qhi; c < nchunk; ++c) mark_chunk(a, L, m, li, geom, boxes, bits, summary, c);
 }
 
 // Count and fill work on 16 lanes per cluster (four clusters per wavefront).
@@ -477,15 +504,18 @@ __device__ __forceinline__ uint32_t group_incl_scan(uint32_t v) {
 }
 
 // Calls f(list, n) for each window of up to kWordWin non-empty word indices of
-// the cluster, in ascending order, uniformly over the cluster's 16 lanes (SW
-// is uniform over them).  clear: zero the summary words afterwards.
+// cluster cell (when on), in ascending order, uniformly over the cluster's 16
+// lanes (SW is uniform over them).  clear: zero the summary words afterwards.
 template <class F>
-__device__ __forceinline__ void for_nonempty_words(uint32_t* __restrict__ summary_cell, uint32_t SW,
-                                                   bool on, bool clear, uint32_t* list, F&& f) {
+__device__ __forceinline__ void for_nonempty_words(uint32_t* __restrict__ summary, size_t ncell,
+                                                   uint32_t cell, uint32_t SW, bool on, bool clear,
+                                                   uint32_t* list, F&& f) {
     const uint32_t sl = threadIdx.x & (kCellLanes - 1u);
     for (uint32_t s0 = 0; s0 < SW; s0 += kCellLanes) {
         const uint32_t j = s0 + sl;
-        const uint32_t sw = (on && j < SW) ? summary_cell[j] : 0u;
+        const bool in = on && j < SW;
+        uint32_t* const sp = summary + (in ? (size_t)j * ncell + cell : 0);
+        const uint32_t sw = in ? *sp : 0u;
         const uint32_t c = (uint32_t)__popc(sw);
         const uint32_t incl = group_incl_scan(c);
         const uint32_t T = __shfl(incl, kCellLanes - 1u, kCellLanes);
@@ -498,28 +528,28 @@ __device__ __forceinline__ void for_nonempty_words(uint32_t* __restrict__ summar
             f(list, T - wb < kWordWin ? T - wb : kWordWin);
             hl_wave_sync();
         }
-        if (clear && sw) summary_cell[j] = 0u;
+        if (clear && sw) *sp = 0u;
     }
 }
 
 // number of members of a cluster (before the cap), returned to every lane of
 // the cluster's group; lane 0 of the group also stores it in raw[cell]
-__device__ __forceinline__ uint32_t count_cell(uint32_t ncell, uint32_t W, uint32_t SW,
+__device__ __forceinline__ uint32_t count_cell(uint32_t ncell, uint32_t SW,
                                                const uint32_t* __restrict__ bits,
                                                const uint32_t* __restrict__ summary,
                                                uint32_t* __restrict__ raw, uint32_t cell,
                                                CellSmem& sm) {
     const uint32_t sl = threadIdx.x & (kCellLanes - 1u);
     const bool on = cell < ncell;
-    const uint32_t* cb = bits + (size_t)(on ? cell : 0u) * W;
+    const uint32_t* cb = bits + (on ? cell : 0u);  // word w at cb[w * ncell]
     uint32_t total = 0;
-    for_nonempty_words(const_cast<uint32_t*>(summary) + (size_t)(on ? cell : 0u) * SW, SW, on, false,
+    for_nonempty_words(const_cast<uint32_t*>(summary), ncell, cell, SW, on, false,
                        sm.words[threadIdx.x / kCellLanes], [&](const uint32_t* list, uint32_t n) {
                            uint32_t wv[kWordsPerLane];
 #pragma unroll
                            for (uint32_t q = 0; q < kWordsPerLane; ++q) {
                                const uint32_t e = kWordsPerLane * sl + q;
-                               wv[q] = cb[list[e < n ? e : 0u]];
+                               wv[q] = cb[(size_t)list[e < n ? e : 0u] * ncell];
                            }
 #pragma unroll
                            for (uint32_t q = 0; q < kWordsPerLane; ++q)
@@ -527,7 +557,7 @@ __device__ __forceinline__ uint32_t count_cell(uint32_t ncell, uint32_t W, uint3
                        });
 #pragma unroll
     for (uint32_t o = kCellLanes / 2; o >= 1; o >>= 1) total += __shfl_xor(total, o, kCellLanes);
-    if (on && sl == 0) raw[cell] = total;
+    if (raw && on && sl == 0) raw[cell] = total;
     return total;
 }
 
@@ -561,115 +591,26 @@ __device__ __forceinline__ void add_tile_sums(TileSumSmem& sm, unsigned long lon
     }
 }
 
-// Block per scan tile (256 threads x 4 consecutive clusters): offsets are
-// the exclusive scan of the capped counts in cluster order, from the sums of
-// the preceding tiles plus the scan within the tile; pool truncation and
-// overflow flags (OR-ed into flags_out and, when given, flags_out2).
-constexpr uint32_t kScanVPT = kScanTile / 256u;
-struct ScanSmem {
-    unsigned long long wave[4];
-    unsigned long long red[4];
-    uint32_t flags;
-};
+constexpr uint32_t kScanVPT = kScanTile / 256u;  // clusters per thread when a block reads a scan tile
 
-__device__ __forceinline__ void scan_tile(const uint32_t* __restrict__ raw, uint32_t ncell,
-                                          uint32_t cap, uint32_t pool_cap,
-                                          uint32_t* __restrict__ offsets,
-                                          uint32_t* __restrict__ counts,
-                                          uint32_t* __restrict__ flags_out,
-                                          uint32_t* __restrict__ flags_out2, uint32_t tile,
-                                          ScanSmem& sm) {
-    const unsigned long long* tile_sums =
-        (const unsigned long long*)((const uint8_t*)raw + cluster_tile_sums_offset(ncell));
-    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    if (t == 0) sm.flags = 0;
-    // sums of the preceding tiles
-    unsigned long long before = 0;
-    for (uint32_t j = t; j < tile; j += 256u) before += tile_sums[j];
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) before += __shfl_xor(before, o, 64);
-    if (lane == 0) sm.red[wave] = before;
-    const uint32_t c0 = tile * kScanTile + kScanVPT * t;
-    uint32_t v[kScanVPT];
-    if (c0 + kScanVPT <= ncell) {  // raw is 256-B aligned, c0 a multiple of 4
-#pragma unroll
-        for (uint32_t q = 0; q < kScanVPT; q += 4) {
-            const uint4 u = *reinterpret_cast<const uint4*>(raw + c0 + q);
-            v[q] = u.x;
-            v[q + 1] = u.y;
-            v[q + 2] = u.z;
-            v[q + 3] = u.w;
-        }
-    } else {
-#pragma unroll
-        for (uint32_t q = 0; q < kScanVPT; ++q) v[q] = c0 + q < ncell ? raw[c0 + q] : 0u;
-    }
-    uint32_t fl = 0;
-    unsigned long long local = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < kScanVPT; ++q) {
-        if (cap && v[q] > cap) {
-            v[q] = cap;
-            fl |= HL_OVERFLOW_CAP;
-        }
-        local += v[q];
-    }
-    unsigned long long incl = local;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long u = __shfl_up(incl, o, 64);
-        if (lane >= (uint32_t)o) incl += u;
-    }
-    if (lane == 63) sm.wave[wave] = incl;
-    __syncthreads();
-    unsigned long long off = sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3] + (incl - local);
-    for (uint32_t w = 0; w < wave; ++w) off += sm.wave[w];
-#pragma unroll
-    for (uint32_t q = 0; q < kScanVPT; ++q) {
-        const uint32_t c = c0 + q;
-        if (c < ncell) {
-            const uint32_t o = off > pool_cap ? pool_cap : (uint32_t)off;
-            const uint32_t room = pool_cap - o;
-            uint32_t n = v[q];
-            if (n > room) {
-                n = room;
-                fl |= HL_OVERFLOW_POOL;
-            }
-            offsets[c] = o;
-            counts[c] = n;
-        }
-        off += v[q];
-    }
-    if (fl) atomicOr(&sm.flags, fl);
-    __syncthreads();
-    if (t == 0 && sm.flags) {
-        atomicOr(flags_out, sm.flags);
-        if (flags_out2) atomicOr(flags_out2, sm.flags);
-    }
-}
-
-// members of a cluster in ascending order up to its (capped, pool-truncated)
-// count; clears the cluster's bitmask words for the next call
-__device__ __forceinline__ void fill_cell(uint32_t ncell, uint32_t W, uint32_t SW,
-                                          uint32_t* __restrict__ bits,
-                                          uint32_t* __restrict__ summary,
-                                          const uint32_t* __restrict__ offsets,
-                                          const uint32_t* __restrict__ counts,
-                                          uint32_t* __restrict__ pool, uint32_t cell, CellSmem& sm) {
+// members of a cluster in ascending order, the first cnt of them written to
+// dst (its capped, pool-truncated count); clears the cluster's bitmask words
+// for the next call
+__device__ __forceinline__ void fill_cell_at(uint32_t ncell, uint32_t SW, uint32_t* __restrict__ bits,
+                                             uint32_t* __restrict__ summary, uint32_t cell,
+                                             uint32_t cnt, uint32_t* __restrict__ dst, CellSmem& sm) {
     const uint32_t sl = threadIdx.x & (kCellLanes - 1u);
     const bool on = cell < ncell;
-    const uint32_t cnt = on ? counts[cell] : 0u;
-    uint32_t* dst = on ? pool + offsets[cell] : pool;
-    uint32_t* cb = bits + (size_t)(on ? cell : 0u) * W;
+    uint32_t* cb = bits + (on ? cell : 0u);  // word w at cb[w * ncell]
     uint32_t written = 0;
-    for_nonempty_words(summary + (size_t)(on ? cell : 0u) * SW, SW, on, true,
+    for_nonempty_words(summary, ncell, cell, SW, on, true,
                        sm.words[threadIdx.x / kCellLanes], [&](const uint32_t* list, uint32_t n) {
                            uint32_t wi[kWordsPerLane], wv[kWordsPerLane];
 #pragma unroll
                            for (uint32_t q = 0; q < kWordsPerLane; ++q) {
                                const uint32_t e = kWordsPerLane * sl + q;
                                wi[q] = list[e < n ? e : 0u];
-                               wv[q] = cb[wi[q]];
+                               wv[q] = cb[(size_t)wi[q] * ncell];
                            }
                            uint32_t lt = 0;
 #pragma unroll
@@ -682,12 +623,100 @@ __device__ __forceinline__ void fill_cell(uint32_t ncell, uint32_t W, uint32_t S
 #pragma unroll
                            for (uint32_t q = 0; q < kWordsPerLane; ++q) {
                                if (kWordsPerLane * sl + q >= n) continue;
-                               cb[wi[q]] = 0u;
+                               cb[(size_t)wi[q] * ncell] = 0u;
                                for (uint32_t m = wv[q]; m; m &= m - 1u, ++pos)
                                    if (pos < cnt) dst[pos] = 32u * wi[q] + (uint32_t)__builtin_ctz(m);
                            }
                            written += __shfl(incl, kCellLanes - 1u, kCellLanes);
                        });
+}
+
+__device__ __forceinline__ void fill_cell(uint32_t ncell, uint32_t SW,
+                                          uint32_t* __restrict__ bits,
+                                          uint32_t* __restrict__ summary,
+                                          const uint32_t* __restrict__ offsets,
+                                          const uint32_t* __restrict__ counts,
+                                          uint32_t* __restrict__ pool, uint32_t cell, CellSmem& sm) {
+    const bool on = cell < ncell;
+    fill_cell_at(ncell, SW, bits, summary, cell, on ? counts[cell] : 0u, on ? pool + offsets[cell] : pool, sm);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ------------------------------------------------------ scan fused into fill
+// A fill block (16 consecutive clusters, in one scan tile of kScanTile) finds
+// its clusters' offsets itself: the sums of the preceding scan tiles (from the
+// count kernel), the capped raw counts of the tile's clusters before the
+// block, and the prefix within the block -- at most 27 + 1008 values at 4K,
+// read in one round trip -- then the pool truncation and overflow flags, then
+// the lists.  The separate scan launch is gone.
+struct ScanFillSmem {
+    uint32_t capped[256 / kCellLanes];
+    unsigned long long part[4];
+    uint32_t flags;
+};
+
+__device__ __forceinline__ void scan_fill_block(const uint32_t* __restrict__ raw, uint32_t ncell,
+                                                uint32_t SW, uint32_t* __restrict__ bits,
+                                                uint32_t* __restrict__ summary, uint32_t cap,
+                                                uint32_t pool_cap, uint32_t* __restrict__ offsets,
+                                                uint32_t* __restrict__ counts, uint32_t* __restrict__ pool,
+                                                uint32_t* __restrict__ flags_out,
+                                                uint32_t* __restrict__ flags_out2, uint32_t first,
+                                                CellSmem& cs, ScanFillSmem& sm) {
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t grp = t / kCellLanes, sl = t & (kCellLanes - 1u);
+    if (first >= ncell) return;  // a batched view's padding cells (uniform)
+    const unsigned long long* tile_sums =
+        (const unsigned long long*)((const uint8_t*)raw + cluster_tile_sums_offset(ncell));
+    const uint32_t tile = first / kScanTile, t0 = tile * kScanTile;
+    if (t == 0) sm.flags = 0u;
+    // sums of the preceding tiles, and the tile's clusters [t0, first)
+    unsigned long long acc = 0;
+    for (uint32_t j = t; j < tile; j += 256u) acc += tile_sums[j];
+    const uint32_t c0 = t0 + kScanVPT * t;  // 4 clusters per thread, 16-B aligned
+    if (c0 < first) {
+        const uint4 u = *reinterpret_cast<const uint4*>(raw + c0);  // first <= ncell, a multiple of 16
+        const uint32_t v[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) acc += (cap && v[q] > cap) ? cap : v[q];
+    }
+    acc = wave_sum_u64(acc);
+    if (lane == 0) sm.part[wave] = acc;
+    // the block's own clusters
+    const uint32_t cell = first + grp;
+    const bool on = cell < ncell;
+    uint32_t n = on ? raw[cell] : 0u;
+    uint32_t fl = 0;
+    if (cap && n > cap) {
+        n = cap;
+        fl |= HL_OVERFLOW_CAP;
+    }
+    if (sl == 0) sm.capped[grp] = n;
+    __syncthreads();
+    unsigned long long off = sm.part[0] + sm.part[1] + sm.part[2] + sm.part[3];
+    for (uint32_t j = 0; j < grp; ++j) off += sm.capped[j];
+    const uint32_t o = off > pool_cap ? pool_cap : (uint32_t)off;
+    uint32_t cnt = n;
+    if (cnt > pool_cap - o) {
+        cnt = pool_cap - o;
+        fl |= HL_OVERFLOW_POOL;
+    }
+    if (on && sl == 0) {
+        offsets[cell] = o;
+        counts[cell] = cnt;
+    }
+    fill_cell_at(ncell, SW, bits, summary, cell, cnt, pool + o, cs);
+    if (fl && sl == 0) atomicOr(&sm.flags, fl);
+    __syncthreads();
+    if (t == 0 && sm.flags) {
+        atomicOr(flags_out, sm.flags);
+        if (flags_out2) atomicOr(flags_out2, sm.flags);
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
@@ -697,8 +726,8 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
 }
 
 // ------------------------------------------------------ one view per launch
-// The geometry kernel also resets the mark queue and the overflow words (it
-// runs before every kernel that reads them).  Its blocks past the cells'
+// The geometry kernel also resets the mark queue, the overflow words and the
+// scan tile sums (it runs before every kernel that reads them).  Its blocks past the cells'
 // prepare the lights' records (independent work that the mark kernel needs
 // too), so that both take one launch.
 __global__ __launch_bounds__(256) void k_cluster_geom(ClusterArgs a,
@@ -736,10 +765,9 @@ __global__ __launch_bounds__(256) void k_cluster_mark(ClusterArgs a,
                                                       const ColumnGeom* __restrict__ geom,
                                                       const CellBox* __restrict__ boxes,
                                                       uint32_t* __restrict__ bits,
-                                                      uint32_t* __restrict__ summary, uint32_t W,
-                                                      uint32_t SW, MarkQueue q) {
+                                                      uint32_t* __restrict__ summary, MarkQueue q) {
     const uint32_t li = blockIdx.x * 4u + (threadIdx.x >> 6);
-    mark_light(a, cam, cl, geom, boxes, bits, summary, W, SW, li, q, li, 0u);
+    mark_light(a, cam, cl, geom, boxes, bits, summary, li, q, li, 0u);
 }
 
 // Queued chunks over a fixed grid of wavefronts: nw / kMarkQueues waves per
@@ -749,8 +777,7 @@ __global__ __launch_bounds__(256) void k_cluster_mark_tail(ClusterArgs a,
                                                            const ColumnGeom* __restrict__ geom,
                                                            const CellBox* __restrict__ boxes,
                                                            uint32_t* __restrict__ bits,
-                                                           uint32_t* __restrict__ summary,
-                                                           uint32_t W, uint32_t SW, MarkQueue q) {
+                                                           uint32_t* __restrict__ summary, MarkQueue q) {
     const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6), nw = gridDim.x * 4u;
     const uint32_t sub = w % kMarkQueues, step = nw / kMarkQueues;  // nw: a multiple of kMarkQueues
     const uint32_t n = min(hl_uniform(q.count[sub * kMarkCountStride]), q.seg);
@@ -762,42 +789,34 @@ __global__ __launch_bounds__(256) void k_cluster_mark_tail(ClusterArgs a,
         const uint4 pb = q.bounds[li];
         const MarkBox m = unpack_box(make_uint4(hl_uniform(pb.x), hl_uniform(pb.y), hl_uniform(pb.z),
                                                 hl_uniform(pb.w)));
-        mark_chunk(a, L, m, li, geom, boxes, bits, summary, W, SW, chunk);
+        mark_chunk(a, L, m, li, geom, boxes, bits, summary, chunk);
     }
 }
 
-__global__ __launch_bounds__(256) void k_cluster_count(uint32_t ncell, uint32_t W, uint32_t SW,
+__global__ __launch_bounds__(256) void k_cluster_count(uint32_t ncell, uint32_t SW,
                                                        const uint32_t* __restrict__ bits,
                                                        const uint32_t* __restrict__ summary,
                                                        uint32_t* __restrict__ raw, uint32_t cap) {
     __shared__ TileSumSmem sm;
     __shared__ CellSmem cs;
     const uint32_t cell = (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes;
-    const uint32_t n = count_cell(ncell, W, SW, bits, summary, raw, cell, cs);
+    const uint32_t n = count_cell(ncell, SW, bits, summary, raw, cell, cs);
     unsigned long long* ts = (unsigned long long*)((uint8_t*)raw + cluster_tile_sums_offset(ncell));
     add_tile_sums(sm, cell < ncell ? ts + cell / kScanTile : nullptr, (cap && n > cap) ? cap : n);
 }
 
-__global__ __launch_bounds__(256) void k_cluster_scan(const uint32_t* __restrict__ raw,
-                                                      uint32_t ncell, uint32_t cap,
-                                                      uint32_t pool_cap,
-                                                      uint32_t* __restrict__ offsets,
+__global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t SW, const uint32_t* __restrict__ raw,
+                                                      uint32_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ summary, uint32_t cap,
+                                                      uint32_t pool_cap, uint32_t* __restrict__ offsets,
                                                       uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ pool,
                                                       uint32_t* __restrict__ flags_out,
                                                       uint32_t* __restrict__ flags_out2) {
-    __shared__ ScanSmem sm;
-    scan_tile(raw, ncell, cap, pool_cap, offsets, counts, flags_out, flags_out2, blockIdx.x, sm);
-}
-
-__global__ __launch_bounds__(256) void k_cluster_fill(uint32_t ncell, uint32_t W, uint32_t SW,
-                                                      uint32_t* __restrict__ bits,
-                                                      uint32_t* __restrict__ summary,
-                                                      const uint32_t* __restrict__ offsets,
-                                                      const uint32_t* __restrict__ counts,
-                                                      uint32_t* __restrict__ pool) {
     __shared__ CellSmem cs;
-    fill_cell(ncell, W, SW, bits, summary, offsets, counts, pool,
-              (blockIdx.x * blockDim.x + threadIdx.x) / kCellLanes, cs);
+    __shared__ ScanFillSmem sm;
+    scan_fill_block(raw, ncell, SW, bits, summary, cap, pool_cap, offsets, counts, pool, flags_out, flags_out2,
+                    blockIdx.x * (256u / kCellLanes), cs, sm);
 }
 
 // ------------------------------------------------- all views in one launch
@@ -855,7 +874,7 @@ __global__ __launch_bounds__(256) void k_batch_mark(const BatchCullView* __restr
     const uint32_t vi = find_view(light_base, nviews, g);
     const BatchCullView& v = views[vi];
     mark_light(v.ca, v.cam, v.cull, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary,
-               v.W, v.SW, g - light_base[vi], q, g, vi);
+               g - light_base[vi], q, g, vi);
 }
 
 __global__ __launch_bounds__(256) void k_batch_mark_tail(const BatchCullView* __restrict__ views,
@@ -875,8 +894,7 @@ __global__ __launch_bounds__(256) void k_batch_mark_tail(const BatchCullView* __
         const uint4 pb = q.bounds[g];
         const MarkBox m = unpack_box(make_uint4(hl_uniform(pb.x), hl_uniform(pb.y), hl_uniform(pb.z),
                                                 hl_uniform(pb.w)));
-        mark_chunk(v.ca, L, m, li, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary, v.W,
-                   v.SW, chunk);
+        mark_chunk(v.ca, L, m, li, (const ColumnGeom*)v.geom, view_boxes(v), v.bits, v.summary, chunk);
     }
 }
 
@@ -889,23 +907,13 @@ __global__ __launch_bounds__(256) void k_batch_count(const BatchCullView* __rest
     const uint32_t vi = find_view(cell_base, nviews, blockIdx.x * kCellsPerBlock);
     const BatchCullView& v = views[vi];
     const uint32_t c = blockIdx.x * kCellsPerBlock + threadIdx.x / kCellLanes - cell_base[vi];
-    const uint32_t n = count_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.raw, c, cs);
+    const uint32_t n = count_cell(v.ncell, v.SW, v.bits, v.summary, v.raw, c, cs);
     unsigned long long* ts = (unsigned long long*)((uint8_t*)v.raw + cluster_tile_sums_offset(v.ncell));
     const uint32_t cap = v.ca.cap;
     add_tile_sums(sm, c < v.ncell ? ts + c / kScanTile : nullptr, (cap && n > cap) ? cap : n);
 }
 
-// one block per scan tile of any view
-__global__ __launch_bounds__(256) void k_batch_scan(const BatchCullView* __restrict__ views,
-                                                    const uint32_t* __restrict__ tile_base,
-                                                    uint32_t nviews) {
-    __shared__ ScanSmem sm;
-    const uint32_t vi = find_view(tile_base, nviews, blockIdx.x);
-    const BatchCullView& v = views[vi];
-    scan_tile(v.raw, v.ncell, v.ca.cap, v.pool_cap, v.offsets, v.counts, v.flags, nullptr,
-              blockIdx.x - tile_base[vi], sm);
-}
-
+// scan and fill of 16 clusters of one view (scan_fill_block)
 __global__ __launch_bounds__(256) void k_batch_fill(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ cell_base,
                                                     uint32_t nviews) {
@@ -913,8 +921,9 @@ __global__ __launch_bounds__(256) void k_batch_fill(const BatchCullView* __restr
     const uint32_t vi = find_view(cell_base, nviews, blockIdx.x * kCellsPerBlock);
     const BatchCullView& v = views[vi];
     __shared__ CellSmem cs;
-    fill_cell(v.ncell, v.W, v.SW, v.bits, v.summary, v.offsets, v.counts, v.pool,
-              blockIdx.x * kCellsPerBlock + threadIdx.x / kCellLanes - cell_base[vi], cs);
+    __shared__ ScanFillSmem sm;
+    scan_fill_block(v.raw, v.ncell, v.SW, v.bits, v.summary, v.ca.cap, v.pool_cap, v.offsets, v.counts, v.pool,
+                    v.flags, nullptr, blockIdx.x * kCellsPerBlock - cell_base[vi], cs, sm);
 }
 
 // grid of the queued-chunk launches: 64 wavefronts per sub-queue, enough to
@@ -940,8 +949,6 @@ hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCul
     }
     const uint32_t cblocks = total_cells / (256u / kCellLanes);  // total_cells: kBatchCellAlign multiple
     k_batch_count<<<cblocks, 256, 0, s>>>(views, cell_base, nviews);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_scan<<<total_tiles, 256, 0, s>>>(views, tile_base, nviews);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_batch_fill<<<cblocks, 256, 0, s>>>(views, cell_base, nviews);
     return hipGetLastError();
@@ -997,18 +1004,17 @@ hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCam
     if (e != hipSuccess) return e;
     if (a.num_lights) {
         k_cluster_mark<<<(a.num_lights + 3) / 4, 256, 0, s>>>(a, cam, cl, geom, boxes, w.bits,
-                                                                  w.summary, W, SW, w.mq);
+                                                                  w.summary, w.mq);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         k_cluster_mark_tail<<<tail_blocks(a.num_lights), 256, 0, s>>>(a, cl, geom, boxes, w.bits,
-                                                                       w.summary, W, SW, w.mq);
+                                                                       w.summary, w.mq);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const uint32_t blocks = (ncell + 256u / kCellLanes - 1u) / (256u / kCellLanes);
-    k_cluster_count<<<blocks, 256, 0, s>>>(ncell, W, SW, w.bits, w.summary, w.raw, a.cap);
+    k_cluster_count<<<blocks, 256, 0, s>>>(ncell, SW, w.bits, w.summary, w.raw, a.cap);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_cluster_scan<<<ntiles, 256, 0, s>>>(w.raw, ncell, a.cap, pool_cap, offsets, counts, flags, flags2);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_cluster_fill<<<blocks, 256, 0, s>>>(ncell, W, SW, w.bits, w.summary, offsets, counts, pool);
+    k_cluster_fill<<<blocks, 256, 0, s>>>(ncell, SW, w.raw, w.bits, w.summary, a.cap, pool_cap, offsets, counts,
+                                          pool, flags, flags2);
     return hipGetLastError();
 }
 

@@ -351,6 +351,24 @@ __device__ __forceinline__ bool hl_div_r_ok(float b, float a0, float a1, float a
     return ab >= 0x1p-40f && ab <= 0x1p40f && amax <= 0x1p40f && amin >= 0x1p-60f;
 }
 
+// hl_x_normalize with the three quotients from one refined reciprocal
+// (hl_x_div_r) where that is exact: length in [2^-40, 2^40] and each
+// component zero or at least 2^-60 in magnitude (a component's magnitude never
+// exceeds the length).  A -0 component comes out +0; otherwise the bits of
+// hl_x_normalize.  len (optional): the length.
+__device__ __forceinline__ float3 hl_x_normalize_r(float x, float y, float z, float* len = nullptr) {
+#pragma clang fp contract(off)
+    const float l = sqrtf(hl_x_dot3(x, y, z, x, y, z));
+    if (len) *len = l;
+    const float m = fminf(fminf(x == 0.0f ? 1.0f : fabsf(x), y == 0.0f ? 1.0f : fabsf(y)),
+                          z == 0.0f ? 1.0f : fabsf(z));
+    if (l >= 0x1p-40f && l <= 0x1p40f && m >= 0x1p-60f) {
+        const float y1 = hl_rcp_refined(l);
+        return make_float3(hl_x_div_r(x, l, y1), hl_x_div_r(y, l, y1), hl_x_div_r(z, l, y1));
+    }
+    return make_float3(x / l, y / l, z / l);
+}
+
 // -------------------------------------------------------------- fast math
 __device__ __forceinline__ float hl_f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float hl_f_rsq(float x) { return __builtin_amdgcn_rsqf(x); }

@@ -90,7 +90,6 @@ constexpr int kFwdModeShade = 0, kFwdModeCount = 1;
 
 struct FwdSmem {
     HotLight light[4][kStage];
-    uint32_t index[4][kStage];
     float eye[3];
     float vp_z[4], vp_w[4];  // rows 2 and 3 of view_proj
     float split[HL_MAX_SLICES + 1];
@@ -137,23 +136,14 @@ __global__ __launch_bounds__(256, 5) void k_forward_lit(ForwardArgs a) {
     p.ny = r1.y;
     p.nz = r1.z;
     {
-        const float ex = sm.eye[0] - p.wx, ey = sm.eye[1] - p.wy, ez = sm.eye[2] - p.wz;
-        const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
-        p.vx = ex * iv;
-        p.vy = ey * iv;
-        p.vz = ez * iv;
-        p.ndv = fmaxf(p.nx * p.vx + p.ny * p.vy + p.nz * p.vz, 0.0f);
-        // grazing view: a last-bit error of V is a large relative error of
-        // G1(NdV); below 0.02 V and NdV are recomputed exactly
-        if (keep && p.ndv < 0.02f) {
-#pragma clang fp contract(off)
-            hl_rare();
-            const float3 vv = hl_x_normalize(ex, ey, ez);
-            p.vx = vv.x;
-            p.vy = vv.y;
-            p.vz = vv.z;
-            p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
-        }
+        // the shader's V, IEEE-exact, and NdV in its order: at grazing view a
+        // last-bit error of V is a large relative error of G1(NdV), and the
+        // near-peak GGX terms depend on the last bits of V
+        const float3 v = hl_x_normalize_r(sm.eye[0] - p.wx, sm.eye[1] - p.wy, sm.eye[2] - p.wz);
+        p.vx = v.x;
+        p.vy = v.y;
+        p.vz = v.z;
+        p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
     }
     p.alb[0] = r2.x;
     p.alb[1] = r2.y;
@@ -216,10 +206,9 @@ __global__ __launch_bounds__(256, 5) void k_forward_lit(ForwardArgs a) {
     const bool walk = keep && in_grid;
 
     HotLight* s_l = sm.light[wave];
-    uint32_t* s_i = sm.index[wave];
     if (MODE == kFwdModeCount) {
         uint32_t pairs = 0;
-        walk_lights<GRID>(a, s_l, s_i, p, walk, cell, [&](const HotLight& L, uint32_t, bool mine) {
+        walk_lights<GRID>(a, s_l, p, walk, cell, [&](const HotLight& L, uint32_t, bool mine) {
             if (!mine || __builtin_isinf(L.p0.w)) {
                 pairs += mine ? 1u : 0u;
                 return;
@@ -233,7 +222,7 @@ __global__ __launch_bounds__(256, 5) void k_forward_lit(ForwardArgs a) {
         return;
     }
     float lo[3] = {0.0f, 0.0f, 0.0f};
-    walk_lights<GRID>(a, s_l, s_i, p, walk, cell, [&](const HotLight& L, uint32_t, bool mine) {
+    walk_lights<GRID>(a, s_l, p, walk, cell, [&](const HotLight& L, uint32_t, bool mine) {
         if (!mine) return;
         Term t;
         if (eval_term<false, false>(L, p, t)) {

@@ -137,6 +137,7 @@ __host__ __device__ inline size_t cluster_tile_sums_offset(uint32_t ncell) {
 inline size_t cluster_raw_bytes(uint32_t ncell) {
     return cluster_tile_sums_offset(ncell) + 8 * (size_t)cluster_scan_tiles(ncell);
 }
+
 // One view of a batched cull (device memory): its cluster arguments, camera,
 // light subset, and its own per-view buffers.
 // Each view's cell range in a batched cull starts on a multiple of this (the

@@ -54,11 +54,47 @@ __device__ __forceinline__ bool range_test(const HotLight& L, float tx, float ty
     return in;
 }
 
+// The geometry of a term near the GGX peak in the shader's order
+// (deferred_lighting.wgsl:699-760, pbr_eval.wgsl:12-18): L from the light,
+// H = normalize(V + L) with the pixel's exact V, the three dot products and
+// the GGX denominator, IEEE throughout.  False when N.L is exactly zero there
+// (the shader returns before the BRDF).
+__device__ __forceinline__ bool exact_peak(const HotLight& L, const Pixel& p, Term& t) {
+#pragma clang fp contract(off)
+    const float3 l = __builtin_isinf(L.p0.w)
+                         ? hl_x_normalize_r(-L.p2.x, -L.p2.y, -L.p2.z)
+                         : hl_x_normalize_r(L.p0.x - p.wx, L.p0.y - p.wy, L.p0.z - p.wz);
+    const float ndl = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, l.x, l.y, l.z), 0.0f);
+    if (ndl == 0.0f) return false;
+    float hl;
+    const float3 h = hl_x_normalize_r(p.vx + l.x, p.vy + l.y, p.vz + l.z, &hl);
+    t.lx = l.x;
+    t.ly = l.y;
+    t.lz = l.z;
+    t.ndl = ndl;
+    t.inv_h = hl_f_rcp(hl);
+    t.hx = h.x;
+    t.hy = h.y;
+    t.hz = h.z;
+    t.hdv = fmaxf(hl_x_dot3(h.x, h.y, h.z, p.vx, p.vy, p.vz), 0.0f);
+    t.ndh = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, h.x, h.y, h.z), 0.0f);
+    t.den = t.ndh * t.ndh * (p.a2 - 1.0f) + 1.0f;
+    return true;
+}
+
 // Fast arithmetic, exact decisions.  Returns false when the term is exactly
 // zero in the restated shader.
 // kCut: apply the all(radiance < 0.002) early-out of the deferred shader
 // (deferred_lighting.wgsl:781); the forward-lit shader has none
 // (forward_lit.wgsl:374-438).
+// Near the GGX peak of a smooth surface the denominator NdH^2 (a2 - 1) + 1
+// is a difference of terms near 1 and as small as a2 (6e-6 for roughness
+// 0.05): a last-bit error of the fast L or H (a few 1e-7 in NdH, N and V being
+// exact from the pixel set-up) moves D by up to ~1e-6 / den relative.  Below
+// kDenExact L, H, N.L, H.V, N.H and the denominator are recomputed in the
+// shader's order with IEEE operations.
+constexpr float kDenExact = 0.02f;
+
 template <bool kFull, bool kCut = true>
 __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Term& t) {
     // Both branches compute locals and the record is written once below:
@@ -162,11 +198,15 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     t.hz = hz * inv_h;
     t.hdv = fmaxf(t.hx * p.vx + t.hy * p.vy + t.hz * p.vz, 0.0f);
     t.ndh = fmaxf(p.nx * t.hx + p.ny * t.hy + p.nz * t.hz, 0.0f);
+    t.den = t.ndh * t.ndh * (p.a2 - 1.0f) + 1.0f;
+    if (hl_ballot(t.den < kDenExact)) {
+        hl_rare();
+        if (t.den < kDenExact && !exact_peak(L, p, t)) return false;
+    }
     t.u = fminf(fmaxf(1.0f - t.hdv, 0.0f), 1.0f);
     const float u2 = t.u * t.u;
     t.P = u2 * u2 * t.u;
     t.a2 = p.a2;
-    t.den = t.ndh * t.ndh * (p.a2 - 1.0f) + 1.0f;
     t.Q = kPi * t.den * t.den + 0.0001f;
     t.el = t.ndl * (1.0f - p.kk) + p.kk1;
     t.den2 = 4.0f * p.ndv * t.ndl + 0.0001f;
@@ -190,13 +230,16 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
 }
 
 // Walks the lights of every distinct cell among the wave's valid pixels:
-// per cell, refines its list against the AABB of those pixels, compacts the
-// survivors into LDS and calls f(light, index, mine) for each, uniformly
-// across the wave.  a provides the grid (lists, counts, offsets,
-// lists_capacity) and the light records (lights, num_lights).
+// per cell, refines its list against the AABB of those pixels and calls
+// f(light, index, mine) for each light that passes, in list order, uniformly
+// across the wave.  The cell's list is streamed 64 entries at a time, one per
+// lane; a lane whose light passes stores the record in its own LDS slot, and
+// the wave then visits the set bits of the ballot (ascending lanes, i.e. list
+// order) with uniform reads of those slots.  a provides the grid (lists,
+// counts, offsets, lists_capacity) and the light records (lights, num_lights).
 template <int GRID, class A, class F>
-__device__ __forceinline__ void walk_lights(const A& a, HotLight* s_l, uint32_t* s_i,
-                                            const Pixel& p, bool valid, uint32_t cell, F&& f) {
+__device__ __forceinline__ void walk_lights(const A& a, HotLight* s_l, const Pixel& p, bool valid,
+                                            uint32_t cell, F&& f) {
     const uint32_t lane = hl_lane();
     uint64_t todo = hl_ballot(valid);
     while (todo) {
@@ -217,41 +260,35 @@ __device__ __forceinline__ void walk_lights(const A& a, HotLight* s_l, uint32_t*
             const uint32_t j = j0 + lane;
             bool pass = false;
             uint32_t idx = 0;
-            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), l1 = q, l2 = q, l3 = q, l4 = q;
             if (j < cnt) {
                 idx = hl_g(list)[j];
                 if (idx < a.num_lights) {
-                    // the whole record now, so that staging it does not wait
-                    // on a second load after the test
+                    // the whole record at once, so that staging it does not
+                    // wait on a second load after the test
                     const HotLight* hp = a.lights + idx;
-                    q = hl_ld4(&hp->p0);
-                    l1 = hl_ld4(&hp->p1);
-                    l2 = hl_ld4(&hp->p2);
-                    l3 = hl_ld4(&hp->p3);
-                    l4 = hl_ld4(&hp->p4);
+                    const float4 q = hl_ld4(&hp->p0), l1 = hl_ld4(&hp->p1), l2 = hl_ld4(&hp->p2),
+                                 l3 = hl_ld4(&hp->p3), l4 = hl_ld4(&hp->p4);
                     const float dx = fmaxf(fmaxf(bx0 - q.x, q.x - bx1), 0.0f);
                     const float dy = fmaxf(fmaxf(by0 - q.y, q.y - by1), 0.0f);
                     const float dz = fmaxf(fmaxf(bz0 - q.z, q.z - bz1), 0.0f);
                     const float d2 = dx * dx + dy * dy + dz * dz;
                     pass = d2 <= q.w * q.w * 1.0001f + 1e-12f;
+                    if (pass) {
+                        s_l[lane].p0 = q;
+                        s_l[lane].p1 = l1;
+                        s_l[lane].p2 = l2;
+                        s_l[lane].p3 = l3;
+                        s_l[lane].p4 = l4;
+                    }
                 }
             }
-            const uint64_t m = hl_ballot(pass);
-            if (pass) {
-                const uint32_t pos = hl_prefix(m);
-                s_l[pos].p0 = q;
-                s_l[pos].p1 = l1;
-                s_l[pos].p2 = l2;
-                s_l[pos].p3 = l3;
-                s_l[pos].p4 = l4;
-                s_i[pos] = idx;
-            }
-            const uint32_t n = (uint32_t)__popcll(m);
+            uint64_t m = hl_ballot(pass);
             hl_wave_sync();
-            for (uint32_t i = 0; i < n; ++i) {
-                const HotLight L = s_l[i];
-                const uint32_t li = s_i[i];
-                f(L, li, mine);
+            while (m) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1u;
+                const HotLight L = s_l[k];
+                f(L, hl_readlane(idx, k), mine);
             }
             hl_wave_sync();
         }

@@ -122,7 +122,6 @@ struct ViewConst {
 
 struct ShadeSmem {
     HotLight light[4][kStage];
-    uint32_t index[4][kStage];
     ViewConst vc;
 };
 
@@ -209,12 +208,13 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             p.wz = n2 / hw;
         }
     }
-    const float ex = vc.eye[0] - p.wx, ey = vc.eye[1] - p.wy, ez = vc.eye[2] - p.wz;
     {
-        const float iv = hl_f_rsq(ex * ex + ey * ey + ez * ez);
-        p.vx = ex * iv;
-        p.vy = ey * iv;
-        p.vz = ez * iv;
+        // the shader's V, IEEE-exact (as N below): NdV at grazing view and the
+        // near-peak GGX terms depend on its last bits
+        const float3 v = hl_x_normalize_r(vc.eye[0] - p.wx, vc.eye[1] - p.wy, vc.eye[2] - p.wz);
+        p.vx = v.x;
+        p.vy = v.y;
+        p.vz = v.z;
     }
 
     // ---- cell of this pixel
@@ -249,36 +249,16 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     float4 Nr = hl_decode<FN>(rN);
     if (!valid) Nr = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
     {
-        const float in_ = hl_f_rsq(Nr.x * Nr.x + Nr.y * Nr.y + Nr.z * Nr.z);
-        if (kExt) {
-            // shadow_factor's normal offset and N.L take the shader's exact N
-#pragma clang fp contract(off)
-            const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
-            p.nx = n.x;
-            p.ny = n.y;
-            p.nz = n.z;
-        } else {
-            p.nx = Nr.x * in_;
-            p.ny = Nr.y * in_;
-            p.nz = Nr.z * in_;
-        }
-        p.ndv = fmaxf(p.nx * p.vx + p.ny * p.vy + p.nz * p.vz, 0.0f);
-        // At grazing view NdV is a difference of ~1 terms: a last-bit error of
-        // N or V (< 1e-6 absolute in NdV) is a relative error of 1e-6 / NdV in
-        // G1(NdV).  Below 0.02 N, V and NdV are recomputed exactly.
-        if (valid && p.ndv < 0.02f) {
-#pragma clang fp contract(off)
-            hl_rare();
-            const float3 n = hl_x_normalize(Nr.x, Nr.y, Nr.z);
-            const float3 vv = hl_x_normalize(ex, ey, ez);
-            p.nx = n.x;
-            p.ny = n.y;
-            p.nz = n.z;
-            p.vx = vv.x;
-            p.vy = vv.y;
-            p.vz = vv.z;
-            p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
-        }
+        // the shader's N, IEEE-exact (the near-peak GGX terms and
+        // shadow_factor's normal offset rely on it); NdV in the shader's order
+        // (at grazing view NdV is a difference of terms near 1, and G1(NdV)
+        // would carry a last-bit error of N or V as a relative error of
+        // 1e-7 / NdV)
+        const float3 n = hl_x_normalize_r(Nr.x, Nr.y, Nr.z);
+        p.nx = n.x;
+        p.ny = n.y;
+        p.nz = n.z;
+        p.ndv = fmaxf(hl_x_dot3(p.nx, p.ny, p.nz, p.vx, p.vy, p.vz), 0.0f);
     }
     p.alb[0] = A.x;
     p.alb[1] = A.y;
@@ -327,12 +307,11 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     }
 
     HotLight* s_l = sm.light[wave];
-    uint32_t* s_i = sm.index[wave];
     float lo[3] = {0.0f, 0.0f, 0.0f};
 
     if (MODE == kModeCount) {
         uint32_t pairs = 0;
-        walk_lights<GRID>(a, s_l, s_i, p, valid, cell,
+        walk_lights<GRID>(a, s_l, p, valid, cell,
                           [&](const HotLight& L, uint32_t, bool mine) {
                               if (!mine || __builtin_isinf(L.p0.w)) {
                                   pairs += mine ? 1u : 0u;
@@ -378,7 +357,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         }
     }
     if (MODE != kModeBackward) {
-        walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
+        walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
             if (!mine) return;
             Term t;
             if (eval_term<false>(L, p, t)) {
@@ -430,7 +409,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     // per-light gradients go to one of kGradSlots copies (by wave) so that the
     // FP64 atomics of many waves on one light do not serialise on one address
     const uint32_t slot = (b * 4u + wave) % kGradSlots;
-    walk_lights<GRID>(a, s_l, s_i, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
+    walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
         float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         bool act = false;
         if (mine) {
@@ -521,10 +500,20 @@ __device__ __forceinline__ void claim_items(uint32_t num_items, uint32_t* __rest
 // counters of the previous launch on the stream (finished by stream order)
 // are zeroed here for the next one, so consecutive launches alternate
 // between two counter sets without a separate reset.
-// Forward launches are held to 96 VGPRs (5 waves per SIMD): the tile's
-// memory round trips need the waves to hide them.
+// Forward launches on packed G-buffers are held to 96 VGPRs (5 waves per
+// SIMD), the shadowed ones and the FP32-G-buffer forward to 128 (4 waves):
+// the tile's memory round trips need the waves to hide them.  The pair count
+// and the adjoint modes keep their registers (no occupancy request): at 4
+// waves they would spill.
+template <int MODE, int FA>
+constexpr int shade_min_blocks() {
+    return MODE == kModeForward && FA != HL_FMT_RGBA32_FLOAT
+               ? 5
+               : (MODE == kModeForward || MODE == kModeShadowed || MODE == kModeExtended ? 4 : 1);
+}
+
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeShadowed || MODE == kModeExtended ? 4 : 1)) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(256, (shade_min_blocks<MODE, FA>())) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
                                                uint32_t* __restrict__ queue_next) {
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
@@ -544,7 +533,7 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : (MODE == kModeShado
 // finds the span by bisection and stages that view's arguments and constants
 // in LDS, where they need fewer registers than read from the view record.
 template <int FA, int FN, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_batch(
+__global__ __launch_bounds__(256, MODE == kModeForward ? 4 : 1) void k_shade_batch(
     const BatchShadeView* __restrict__ views, const uint2* __restrict__ spans, uint32_t nspans,
     uint32_t num_items, uint32_t* __restrict__ queue) {
     static_assert(sizeof(ShadeArgs) % 4 == 0, "copied as words");

@@ -128,6 +128,11 @@ SPECS: dict[str, SceneSpec] = {
                            seed=SEED_BASE + 16 * 3),
     "c4_512_adj": SceneSpec("c4_512_adj", 512, 512, GBUF_F32, 256, seed=SEED_BASE + 16 * 4),
 }
+# The 4K/16k lights and grid seen from 420 m out with a 90-degree field of
+# view: far froxels span the light box, so cluster lists run from 0 to 2000+
+# indices (the long-list regime of the 16384-light configuration).
+SPECS["c3_4k_16k_far"] = replace(SPECS["c3_4k_16k"], name="c3_4k_16k_far", eye=(0.0, 60.0, 420.0),
+                                 fov_y_deg=90.0)
 
 
 def spec(name: str, **over) -> SceneSpec:

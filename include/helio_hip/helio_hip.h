@@ -131,12 +131,14 @@ typedef struct hl_cluster_params {
     uint32_t width, height;       /* screen size */
     uint32_t cap;                 /* indices kept per cluster, lowest first; 0 = unbounded */
     float near_plane, far_plane;
-    float r_pad_column;           /* radius padding of the lateral-plane pre-test (max of slice_r_pad) */
+    float r_pad_column;           /* max of slice_r_pad (informational: the cull reads slice_r_pad) */
     float reserved;
     float depth_split[HL_MAX_SLICES + 1];  /* NDC depth where slice k begins */
     float slice_near[HL_MAX_SLICES];       /* padded view-distance range of slice k */
     float slice_far[HL_MAX_SLICES];
     float slice_r_pad[HL_MAX_SLICES];      /* radius padding of slice k (reconstruction error bound) */
+    /* slice_near, slice_far and slice_r_pad must be finite and non-decreasing
+     * over the grid_z slices, the paddings non-negative (HL_EINVAL otherwise) */
 } hl_cluster_params;
 
 /* Host-side helper: compute the slice thresholds from the camera (a HOST copy

@@ -78,11 +78,16 @@ def find_view(base: np.ndarray, g: np.ndarray) -> np.ndarray:
     return np.searchsorted(base[1:-1], g, side="right").astype(np.int64)
 
 
-def light_boxes(v: EmuView) -> tuple[np.ndarray, ...]:
+def light_boxes(v: EmuView, mistake: str | None = None) -> tuple[np.ndarray, ...]:
     """light_cells() of hl_cull.hip for every light of the view: the candidate
     box (x0, y0, s0, nx, ny, nz) and whether it is non-empty.  The fast
     arithmetic is evaluated in float32 numpy; the integer clamps that follow
-    it are the kernel's, and they alone keep the box inside the grid."""
+    it are the kernel's, and they alone keep the box inside the grid.
+    mistake plants an error for tests: "no_margins" (no rounding margins),
+    "no_pad" (the radius without the slice padding), "no_tilt" (fx = fy = 1,
+    the lateral planes taken as parallel to the view axis), "pad_of_first"
+    (the padding of the nearest slice in reach instead of the farthest)."""
+    margins = mistake != "no_margins"
     p = v.params
     gx, gy, gz = p.grid_x, p.grid_y, p.grid_z
     cam = v.camera
@@ -90,7 +95,9 @@ def light_boxes(v: EmuView) -> tuple[np.ndarray, ...]:
     pos = np.stack([v.lights["position_range"][:, k] for k in range(3)], 1).astype(np.float32)
     rng = np.where(v.lights["light_type"] == 0, np.float32(np.inf),
                    v.lights["position_range"][:, 3]).astype(np.float32)
-    vp = pos @ view[:3, :3] + view[3, :3]  # view-space position (x, y, z)
+    # view-space position (x, y, z), columns accumulated in order (hl_x_mat_row)
+    vp = np.stack([((view[0, r] * pos[:, 0] + view[1, r] * pos[:, 1]) + view[2, r] * pos[:, 2]) + view[3, r]
+                   for r in range(3)], 1).astype(np.float32)
     sn = np.array(p.slice_near[:gz], np.float32)
     sf = np.array(p.slice_far[:gz], np.float32)
     sp = np.array(p.slice_r_pad[:gz], np.float32)
@@ -101,42 +108,61 @@ def light_boxes(v: EmuView) -> tuple[np.ndarray, ...]:
     y1 = np.full(n, gy - 1, np.int64)
     s0 = np.zeros(n, np.int64)
     s1 = np.full(n, gz - 1, np.int64)
-    fin = ~np.isinf(rng)
     zc = (-vp[:, 2]).astype(np.float32)
+    f32 = np.float32
     with np.errstate(all="ignore"):
-        rr = (rng + np.float32(p.r_pad_column)).astype(np.float32)
-        t1 = np.clip((sn[None, :] < (zc + rr)[:, None]).sum(1) + 1, 0, gz - 1)
-        rr = (rng + sp[t1]).astype(np.float32)
-        ls0 = (sf[None, :] < (zc - rr)[:, None]).sum(1) - 1
-        ls1 = (sn[None, :] < (zc + rr)[:, None]).sum(1) + 1
+        fin0 = ~np.isinf(rng) & np.isfinite(zc) & np.isfinite(vp[:, 0]) & np.isfinite(vp[:, 1])
+        fin = fin0.copy()
+        e = f32(1e-5) if margins else f32(0)
+        snm = (sn - e * np.abs(sn)).astype(f32)
+        sfm = (sf + e * np.abs(sf)).astype(f32)
+        slack = (e * ((np.abs(vp[:, 0]) + np.abs(vp[:, 1])) + np.abs(zc))).astype(f32)
+        r = np.abs(rng).astype(f32)
+        spk = sp * f32(0) if mistake == "no_pad" else sp
+        rr = ((r + spk[gz - 1]) * (f32(1) + e) + slack).astype(f32)
+        t1 = (snm[None, :] <= (zc + rr)[:, None]).sum(1) - 1
+        fin &= t1 >= 0
+        rr = ((r + spk[np.maximum(t1, 0)]) * (f32(1) + e) + slack).astype(f32)
+        if mistake == "pad_of_first":
+            f0 = np.minimum((sfm[None, :] < (zc - rr)[:, None]).sum(1), gz - 1)
+            rr = ((r + spk[f0]) * (f32(1) + e) + slack).astype(f32)
+        ls0 = (sfm[None, :] < (zc - rr)[:, None]).sum(1)
+        ls1 = (snm[None, :] <= (zc + rr)[:, None]).sum(1) - 1
         s0 = np.where(fin, ls0, s0)
         s1 = np.where(fin, ls1, s1)
-        front = fin & (zc > np.float32(1e-4))
-        p00, p11 = np.float32(cam.proj[0]), np.float32(cam.proj[5])
-        fx = np.sqrt(np.float32(1) + np.float32(1) / (p00 * p00))
-        fy = np.sqrt(np.float32(1) + np.float32(1) / (p11 * p11))
-        iz = np.float32(1) / zc
+        none = fin0 & (t1 < 0)  # no slice in reach: no candidates
+        p00, p11 = f32(cam.proj[0]), f32(cam.proj[5])
+        front = fin & (zc > f32(1e-4)) & (p00 > 0) & (p11 > 0)
+        fx = np.sqrt(f32(1) + f32(1) / (p00 * p00))
+        fy = np.sqrt(f32(1) + f32(1) / (p11 * p11))
+        if mistake == "no_tilt":
+            fx = fy = f32(1)
+        iz = f32(1) / zc
         nx0 = p00 * (vp[:, 0] - rr * fx) * iz
         nx1 = p00 * (vp[:, 0] + rr * fx) * iz
         ny0 = p11 * (vp[:, 1] - rr * fy) * iz
         ny1 = p11 * (vp[:, 1] + rr * fy) * iz
-        fw, fh = np.float32(p.width), np.float32(p.height)
-        px0 = (nx0 + 1) * np.float32(0.5) * fw
-        px1 = (nx1 + 1) * np.float32(0.5) * fw
-        py0 = (1 - ny1) * np.float32(0.5) * fh
-        py1 = (1 - ny0) * np.float32(0.5) * fh
-        icw, ich = np.float32(1) / np.float32(p.cell_w), np.float32(1) / np.float32(p.cell_h)
+        fw, fh = f32(p.width), f32(p.height)
+        px0 = (nx0 + 1) * f32(0.5) * fw
+        px1 = (nx1 + 1) * f32(0.5) * fw
+        py0 = (1 - ny1) * f32(0.5) * fh
+        py1 = (1 - ny0) * f32(0.5) * fh
+        q = f32(0.25) if margins else f32(0)
+        px0 = px0 - (e * np.abs(px0) + q)
+        px1 = px1 + (e * np.abs(px1) + q)
+        py0 = py0 - (e * np.abs(py0) + q)
+        py1 = py1 + (e * np.abs(py1) + q)
+        icw, ich = f32(1) / f32(p.cell_w), f32(1) / f32(p.cell_h)
 
-        def to_int(f):  # v_cvt_i32_f32: saturating, NaN -> 0
-            f = np.nan_to_num(f.astype(np.float64), nan=0.0, posinf=2**31 - 1, neginf=-2**31)
-            return np.clip(np.trunc(f), -2**31, 2**31 - 1).astype(np.int64)
+        def to_int(f):  # v_cvt_i32_f32 of a value already clamped to [-1, g]
+            return np.trunc(f.astype(np.float64)).astype(np.int64)
 
         fmax = np.fmax  # fmaxf/fminf: a NaN operand yields the other one
         fmin = np.fmin
-        bx0 = to_int(fmax(np.floor(px0 * icw) - 1, np.float32(-1)))
-        bx1 = to_int(fmin(np.floor(px1 * icw) + 1, np.float32(gx)))
-        by0 = to_int(fmax(np.floor(py0 * ich) - 1, np.float32(-1)))
-        by1 = to_int(fmin(np.floor(py1 * ich) + 1, np.float32(gy)))
+        bx0 = to_int(fmin(fmax(np.floor(px0 * icw), f32(-1)), f32(gx)))
+        bx1 = to_int(fmax(fmin(np.floor(px1 * icw), f32(gx)), f32(-1)))
+        by0 = to_int(fmin(fmax(np.floor(py0 * ich), f32(-1)), f32(gy)))
+        by1 = to_int(fmax(fmin(np.floor(py1 * ich), f32(gy)), f32(-1)))
     x0 = np.where(front, bx0, x0)
     x1 = np.where(front, bx1, x1)
     y0 = np.where(front, by0, y0)
@@ -147,7 +173,7 @@ def light_boxes(v: EmuView) -> tuple[np.ndarray, ...]:
     y1 = np.clip(y1, -1, gy - 1)
     s0 = np.clip(s0, 0, gz - 1)
     s1 = np.clip(s1, -1, gz - 1)
-    ok = (x1 >= x0) & (y1 >= y0) & (s1 >= s0)
+    ok = (x1 >= x0) & (y1 >= y0) & (s1 >= s0) & ~none
     return ok, x0, y0, s0, x1 - x0 + 1, y1 - y0 + 1, s1 - s0 + 1
 
 
@@ -290,13 +316,14 @@ def check_mark(views, plan) -> dict:
             cmax = (x0 + nx - 1) + p.grid_x * (y0 + ny - 1) + p.grid_x * p.grid_y * (s0 + nz - 1)
             assert (cmax[ok] < a["ncell"]).all()
             li = np.nonzero(ok)[0]
-            assert (4 * (cmax[ok] * a["W"] + (li >> 5) + 1) <= a["bits"]).all()
-            assert (4 * (cmax[ok] * a["SW"] + (li >> 10) + 1) <= a["summary"]).all()
+            # word-major bitmask: word w of cluster c at w * ncell + c
+            assert (4 * ((li >> 5) * a["ncell"] + cmax[ok] + 1) <= a["bits"]).all()
+            assert (4 * ((li >> 10) * a["ncell"] + cmax[ok] + 1) <= a["summary"]).all()
             small = ok & (work <= 1 << 21)
             for d in np.unique(nz[small]):
                 assert_floor_div_exact(int(d), int(work[small & (nz == d)].max()))
             for d in np.unique(nx[small]):
-                assert_floor_div_exact(int(d), int((work // nz)[small & (nx == d)].max()))
+                assert_floor_div_exact(int(d), int((work // np.maximum(nz, 1))[small & (nx == d)].max()))
         stats["lights"] += int(ok.sum())
         stats["chunks"] += int(nchunk[ok].sum())
         for li in np.nonzero(ok & (nchunk > K_MARK_INLINE))[0]:
@@ -341,10 +368,11 @@ def scan_emulate(raw: np.ndarray, cap: int, pool_cap: int) -> tuple[np.ndarray, 
 
 
 def check_count_scan_fill(views, plan, aligned=True, rng=None) -> None:
-    """k_batch_count / k_batch_scan / k_batch_fill: each 16-lane group's view
-    and cluster, the bitmask words it may list (word indices below W: a
-    summary bit is set only for a word holding a member), the tile sums,
-    and the pool positions the fill writes for any raw counts."""
+    """k_batch_count / k_batch_fill: each 16-lane group's view and cluster,
+    the bitmask words it may list (word indices below W: a summary bit is set
+    only for a word holding a member), the tile sums and raw counts the fill
+    block reads for its offsets, and the pool positions the fill writes for
+    any raw counts."""
     cb, tb, al = plan["cell_base"], plan["tile_base"], plan["alloc"]
     total_cells = int(cb[-1])
     cblocks = total_cells // 16 if aligned else -(-total_cells // 16)
@@ -363,16 +391,36 @@ def check_count_scan_fill(views, plan, aligned=True, rng=None) -> None:
     SW = np.array([a["SW"] for a in al])[vi]
     bits = np.array([a["bits"] for a in al])[vi]
     summ = np.array([a["summary"] for a in al])[vi]
-    # summary words j < SW of the cluster; listed word indices 32 j + b < W
-    assert (4 * (c[live] * SW[live] + SW[live]) <= summ[live]).all()
-    assert (4 * (c[live] * W[live] + W[live]) <= bits[live]).all()
+    # summary words j < SW of the cluster, listed word indices 32 j + b < W
+    # (word-major: word w of cluster c at w * ncell + c)
+    assert (4 * ((SW[live] - 1) * ncell[live] + c[live] + 1) <= summ[live]).all()
+    assert (4 * ((W[live] - 1) * ncell[live] + c[live] + 1) <= bits[live]).all()
     # a group with no cluster reads only summary word range of cluster 0 with on=false (no loads)
-    # scan: one block per tile
-    blocks = np.arange(int(tb[-1]), dtype=np.int64)
-    sv = find_view(tb, blocks)
-    tile = blocks - tb[sv]
-    nt = np.array([scan_tiles(a["ncell"]) for a in al])[sv]
-    assert (tile < nt).all()
+    # scan inside the fill (scan_fill_block): a block whose first cluster f
+    # lies in the view reads the sums of scan tiles [0, f // 1024) and the raw
+    # counts [1024 (f // 1024), f) with 16-B loads; padding blocks return.
+    # (Views packed back to back, the layout before the alignment, had a
+    # separate scan launch: nothing to check there.)
+    if not aligned:
+        return check_pool_positions(views, al, rng)
+    first = (np.arange(cblocks, dtype=np.int64) * 16)
+    bv = vi[::16]
+    f = first - cb[bv]
+    nc = np.array([a["ncell"] for a in al])[bv]
+    live_b = (f >= 0) & (f < nc)
+    assert (f[live_b] % 16 == 0).all()
+    ntile = np.array([scan_tiles(a["ncell"]) for a in al])[bv]
+    assert ((f[live_b] // 1024) <= ntile[live_b]).all()
+    raw_b = np.array([a["raw"] for a in al])[bv]
+    ts_off = np.array([tile_sums_offset(a["ncell"]) for a in al])[bv]
+    # the last tile sum read, and the last raw word read (f - 1 < ncell), inside raw
+    assert ((ts_off + 8 * (f // 1024))[live_b] <= raw_b[live_b]).all()
+    assert (4 * f[live_b] <= ts_off[live_b]).all()
+    check_pool_positions(views, al, rng)
+
+
+def check_pool_positions(views, al, rng=None) -> None:
+    """The pool positions the fill writes, for extreme and random raw counts."""
     rng = rng or np.random.default_rng(1)
     for v, a in zip(views, al):
         n = v.n

@@ -79,7 +79,7 @@ def hip_clusters(ctx, sc, params, pool_cap):
 
 
 @pytest.mark.parametrize("name,pool_cap", [("c1_720p_64", 1 << 20), ("c2_1080p_1k", 1 << 21),
-                                           ("c3_4k_16k", 1 << 25)])
+                                           ("c3_4k_16k", 1 << 25), ("c3_4k_16k_far", 1 << 25)])
 def test_cluster_cull_bit_exact(ctx, hlib, name, pool_cap):
     sc = scene(name)
     print(f"[cluster cull] {name}", flush=True)
@@ -94,6 +94,8 @@ def test_cluster_cull_bit_exact(ctx, hlib, name, pool_cap):
     used = int(ooff[-1] + ocnt[-1]) if len(ocnt) else 0
     assert np.array_equal(pool[:used], opool[:used])
     print(f"  clusters={len(cnt)} entries={total} max={int(ocnt.max())} flags={flags}")
+    if name == "c3_4k_16k_far":
+        assert int(ocnt.max()) >= 2000  # the long-list regime
 
 
 def test_cluster_cap_and_pool_truncation(ctx, hlib):
@@ -128,8 +130,10 @@ def test_shade_tiles_parity(ctx, name):
 
 
 @pytest.mark.parametrize("name,rows", [("c1_720p_64", None), ("c2_1080p_1k", None),
-                                       ("c3_4k_16k", (1000, 1064))])
+                                       ("c3_4k_16k", None), ("c3_4k_16k_far", None)])
 def test_shade_clusters_parity(ctx, hlib, name, rows):
+    """Every pixel of the frame, 4K included; c3_4k_16k_far walks lists of up
+    to 2000+ lights."""
     sc = scene(name)
     print(f"[shade clusters] {name}", flush=True)
     params = synth.cluster_params(hlib, sc)
@@ -147,6 +151,30 @@ def test_shade_clusters_parity(ctx, hlib, name, rows):
     if rows is None:
         assert shade.count_pairs(ds, cull.publish()) == opairs
     print(f"  max rel err {r:.2e}")
+
+
+@pytest.mark.parametrize("name,over", [("c4_512_adj", {}),
+                                       ("c3_4k_16k", dict(width=960, height=540))])
+def test_smooth_surfaces_near_the_specular_peak(ctx, hlib, name, over):
+    """Roughness 0.03-0.25 on every pixel: many terms lie near the GGX peak,
+    where the denominator NdH^2 (a^2 - 1) + 1 is a few 1e-3 and a last-bit
+    difference of L or H would move D by percents; every pixel within 1e-4 of
+    the oracle (fp32 and fp16 G-buffers)."""
+    sc = synth.make_scene(name, **over)
+    dt = {abi.HL_FMT_RGBA32_FLOAT: np.float32, abi.HL_FMT_RGBA16_FLOAT: np.float16}[sc.formats[0]]
+    orm = sc.orm.view(dt).reshape(sc.height, sc.width, 4)
+    orm[..., 1] = np.random.default_rng(7).uniform(0.03, 0.25, (sc.height, sc.width)).astype(dt)
+    print(f"[smooth] {name} {sc.width}x{sc.height}", flush=True)
+    params = synth.cluster_params(hlib, sc)
+    ds, cull, off, cnt, pool, flags = hip_clusters(ctx, sc, params, 1 << 25)
+    assert flags == 0
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    shade.out.fill_u32(0)
+    shade.execute(ds, cull.publish())
+    ref, pairs = O.shade(sc, O.cluster_grid_host(params, off, cnt, pool))
+    r = check_radiance(shade.output(), ref, sc.depth < 1.0)
+    print(f"  max rel err {r:.2e}, Q={pairs}")
 
 
 def test_zero_lights(ctx):
@@ -217,6 +245,22 @@ def test_debug_views_parity(ctx, mode):
         check_radiance(out, ref, lit)
     else:
         assert np.array_equal(out[lit], ref[lit])
+
+
+def test_unsorted_slice_tables_are_rejected(ctx, hlib):
+    """The cull searches slice_near / slice_far as sorted tables under the
+    padding of the farthest slice in reach: tables that are not
+    non-decreasing, or a negative padding, are refused."""
+    sc = scene("c1_720p_64")
+    ds = upload_scene(ctx, sc)
+    for field, k, value in (("slice_near", 3, None), ("slice_far", 5, None), ("slice_r_pad", 2, -1.0)):
+        p = synth.cluster_params(hlib, sc)
+        arr = getattr(p, field)
+        arr[k] = arr[k - 1] * 0.5 if value is None else value
+        with pytest.raises(abi.HelioError, match="slice"):
+            cull = LightCullPass(ctx, sc.width, sc.height, clusters=p, pool_cap=1 << 20)
+            cull.prepare(ds)
+            cull.execute(ds)
 
 
 def test_unsupported_debug_view_is_rejected(ctx):

@@ -25,12 +25,18 @@ def run(ctx, sc, refl, base, flags=0, globals_=None):
     return out.download(np.float32).reshape(sc.height, sc.width, 4)
 
 
-@pytest.mark.parametrize("name,fmt,rc", [("c1_720p_64", abi.HL_FMT_RGBA16_FLOAT, False),
-                                         ("c2_1080p_1k", abi.HL_FMT_RGBA8_UNORM, True),
-                                         ("c4_512_adj", abi.HL_FMT_RGBA32_FLOAT, True)])
-def test_reflection_matches_oracle(ctx, name, fmt, rc):
-    print(f"[reflection] {name} cube format {fmt} rc={rc}", flush=True)
-    sc = synth.make_scene(name)
+# the 4K/16k-light scene (fp16 G-buffer set) at a quarter of its pixels
+C3_REDUCED = dict(width=960, height=540, num_lights=4096)
+
+
+@pytest.mark.parametrize("name,fmt,rc,over", [("c1_720p_64", abi.HL_FMT_RGBA16_FLOAT, False, {}),
+                                              ("c2_1080p_1k", abi.HL_FMT_RGBA8_UNORM, True, {}),
+                                              ("c3_4k_16k", abi.HL_FMT_RGBA16_FLOAT, True, C3_REDUCED),
+                                              ("c4_512_adj", abi.HL_FMT_RGBA32_FLOAT, True, {}),
+                                              ("c4_512_adj", abi.HL_FMT_RGBA16_FLOAT, False, {})])
+def test_reflection_matches_oracle(ctx, name, fmt, rc, over):
+    print(f"[reflection] {name} cube format {fmt} rc={rc} {over}", flush=True)
+    sc = synth.make_scene(name, **over)
     refl = synth.make_reflections(sc, n_box=5, n_sphere=5, face=64, mips=7, fmt=fmt, rc=rc)
     H, W = sc.height, sc.width
     sky = sc.depth >= 1.0
@@ -105,19 +111,21 @@ def test_rejects_bad_inputs(ctx):
     ReflectionPass(ctx).execute(ds, dr, out)  # and the restored inputs run
 
 
-@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("case", range(8))
 def test_random_configurations(ctx, case):
     """Ragged frame sizes, 0-19 boxes + 1-19 spheres, cube faces of 8-64
     texels with any mip count, the three cube formats, RC on and off, added
-    over a random frame: within 1e-4 of the restatement (in practice a few
-    1e-7, from exp2 in env_brdf_approx)."""
+    over a random frame; the scene (and so the G-buffer set: reference, fp16,
+    fp32) and the cube format are drawn independently: within 1e-4 of the
+    restatement (in practice a few 1e-7, from exp2 in env_brdf_approx)."""
     rng = np.random.default_rng(1000 + case)
-    name = ("c1_720p_64", "c2_1080p_1k", "c4_512_adj")[case % 3]
+    name = str(rng.choice(["c1_720p_64", "c2_1080p_1k", "c3_4k_16k", "c4_512_adj"]))
+    over = dict(num_lights=1024) if name == "c3_4k_16k" else {}
     W, H = int(rng.integers(97, 700)), int(rng.integers(61, 400))
-    sc = synth.make_scene(name, width=W, height=H)
+    sc = synth.make_scene(name, width=W, height=H, **over)
     face = int(rng.choice([8, 16, 32, 64]))
     mips = int(rng.integers(1, int(np.log2(face)) + 2))
-    fmt = (abi.HL_FMT_RGBA16_FLOAT, abi.HL_FMT_RGBA32_FLOAT, abi.HL_FMT_RGBA8_UNORM)[case % 3]
+    fmt = int(rng.choice([abi.HL_FMT_RGBA16_FLOAT, abi.HL_FMT_RGBA32_FLOAT, abi.HL_FMT_RGBA8_UNORM]))
     nb, ns = int(rng.integers(0, 20)), int(rng.integers(1, 20))
     print(f"[reflection] random case {case}: {name} {W}x{H}, {nb}+{ns} captures, face {face} x {mips} mips, "
           f"format {fmt}", flush=True)

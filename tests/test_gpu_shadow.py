@@ -44,13 +44,20 @@ def run(ctx, hlib, name, mode, quality, size, flags=0, **over):
     return sc, sh, out, grid, (ds, cull, shade, dsh)
 
 
-@pytest.mark.parametrize("name,mode,quality,size", [("c1_720p_64", "tiles", "medium", 256),
-                                                    ("c1_720p_64", "clusters", "high", 256),
-                                                    ("c2_1080p_1k", "tiles", "ultra", 512),
-                                                    ("c2_1080p_1k", "clusters", "low", 1024)])
-def test_shadowed_shading_matches_oracle(ctx, hlib, name, mode, quality, size):
-    print(f"[shadowed] {name} {mode} {quality} atlas {size}", flush=True)
-    sc, sh, out, grid, _ = run(ctx, hlib, name, mode, quality, size)
+# The 4K/16k-light scene at a quarter of its pixels and lights (fp16 G-buffer
+# set) and the 512^2 scene (fp32 set) build the other two G-buffer variants.
+C3_REDUCED = dict(width=960, height=540, num_lights=4096)
+
+
+@pytest.mark.parametrize("name,mode,quality,size,over", [("c1_720p_64", "tiles", "medium", 256, {}),
+                                                         ("c1_720p_64", "clusters", "high", 256, {}),
+                                                         ("c2_1080p_1k", "tiles", "ultra", 512, {}),
+                                                         ("c2_1080p_1k", "clusters", "low", 1024, {}),
+                                                         ("c3_4k_16k", "clusters", "medium", 512, C3_REDUCED),
+                                                         ("c4_512_adj", "tiles", "high", 256, {})])
+def test_shadowed_shading_matches_oracle(ctx, hlib, name, mode, quality, size, over):
+    print(f"[shadowed] {name} {mode} {quality} atlas {size} {over}", flush=True)
+    sc, sh, out, grid, _ = run(ctx, hlib, name, mode, quality, size, **over)
     ref, _ = O.shade(sc, grid, shadows=sh)
     unshadowed, _ = O.shade(sc, grid)
     sky = sc.depth >= 1.0
@@ -91,17 +98,22 @@ def test_shadow_layer_past_the_atlas_is_lit(ctx, hlib):
     check_radiance(shade.output(), ref, sc.depth < 1.0)
 
 
-@pytest.mark.parametrize("name,mode,fmt,shadows", [("c1_720p_64", "tiles", "f16", False),
-                                                   ("c2_1080p_1k", "clusters", "f32", True)])
-def test_rc_gi_matches_oracle(ctx, hlib, name, mode, fmt, shadows):
+@pytest.mark.parametrize("name,mode,fmt,shadows,over", [
+    ("c1_720p_64", "tiles", "f16", False, {}),
+    ("c2_1080p_1k", "clusters", "f32", True, {}),
+    ("c3_4k_16k", "clusters", "f16", True, C3_REDUCED),
+    ("c4_512_adj", "tiles", "f32", False, {})])
+def test_rc_gi_matches_oracle(ctx, hlib, name, mode, fmt, shadows, over):
     """hl_deferred_forward_ex with the radiance-cascade indirect diffuse (and
     shadows): the RC irradiance is exact, the combine within 1e-4."""
     from helio_amd.device import DeviceTexture
-    print(f"[rc gi] {name} {mode} {fmt} shadows={shadows}", flush=True)
-    sc = synth.make_scene(name)
+    print(f"[rc gi] {name} {mode} {fmt} shadows={shadows} {over}", flush=True)
+    sc = synth.make_scene(name, **over)
     sh = synth.add_shadow_casters(sc, n_point=4, n_spot=4, size=256) if shadows else None
+    box = ((-60.0, -1.0, -60.0), (60.0, 30.0, 60.0)) if name == "c3_4k_16k" else ((-25.0, -1.0, -25.0),
+                                                                                  (25.0, 14.0, 25.0))
     atlas = synth.make_rc_probes(sc, fmt=abi.HL_FMT_RGBA16_FLOAT if fmt == "f16" else abi.HL_FMT_RGBA32_FLOAT,
-                                 box=((-25.0, -1.0, -25.0), (25.0, 14.0, 25.0)))
+                                 box=box)
     ds = upload_scene(ctx, sc)
     if mode == "tiles":
         cull = LightCullPass(ctx, sc.width, sc.height)
