@@ -330,7 +330,7 @@ hl_status build_plan(hl_plan* p) {
             const uint32_t v = o.second;
             if (vset[v] * 2 + (is_adjoint(p->views[v]) ? 1 : 0) != g) continue;
             spans.push_back(make_uint2((uint32_t)n, v));
-            n += p->hshade[v].sa.num_blocks;
+            n += shade_items(is_adjoint(p->views[v]) ? kModeFusedL2 : kModeForward, p->hshade[v].sa.num_blocks);
         }
         if (n >= 0x80000000ull) return fail(c, HL_ECAPACITY, "%llu block tiles in one launch", (unsigned long long)n);
         p->grp_spans[g] = (uint32_t)spans.size() - p->grp_first[g];

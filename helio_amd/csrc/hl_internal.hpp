@@ -185,6 +185,13 @@ enum ShadeMode : int {
     kModeShadowed = 4,
     kModeExtended = 5
 };
+// Work items of a shading launch over num_blocks block tiles: the adjoint
+// modes split each wave tile's lights over a block's four waves, so their
+// items are wave tiles (four per block tile).
+__host__ __device__ inline uint32_t shade_items(int mode, uint32_t num_blocks) {
+    return (mode == kModeBackward || mode == kModeFusedL2) ? 4u * num_blocks : num_blocks;
+}
+
 // Claim counters of one persistent shading launch: kShadeQueues counters,
 // kShadeQueueStride words apart, zeroed beforehand (launch_fill_u32).
 constexpr uint32_t kShadeQueues = 64;

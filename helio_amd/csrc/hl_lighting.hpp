@@ -229,24 +229,40 @@ __device__ __forceinline__ bool eval_term(const HotLight& L, const Pixel& p, Ter
     return true;
 }
 
+// The lights a wave's walk visited, kept for a second pass over the same
+// lights (the fused adjoint): their records stay where the walk staged them,
+// compacted in visiting order (s_l[0, n)), with each one's index and the lanes
+// of its cell group.  ok turns false when they do not fit in kStage slots.
+struct KeptLights {
+    uint32_t idx[kStage];
+    uint64_t lanes[kStage];
+};
+struct KeptState {
+    uint32_t n = 0;
+    bool ok = true;
+};
+
 // Walks the lights of every distinct cell among the wave's valid pixels:
 // per cell, refines its list against the AABB of those pixels and calls
 // f(light, index, mine) for each light that passes, in list order, uniformly
 // across the wave.  The cell's list is streamed 64 entries at a time, one per
-// lane; a lane whose light passes stores the record in its own LDS slot, and
-// the wave then visits the set bits of the ballot (ascending lanes, i.e. list
-// order) with uniform reads of those slots.  a provides the grid (lists,
-// counts, offsets, lists_capacity) and the light records (lights, num_lights).
+// lane; a lane whose light passes stores the record in an LDS slot (its own,
+// or with kept the next free one of the kept range), and the wave then visits
+// the set bits of the ballot (ascending lanes, i.e. list order) with uniform
+// reads of those slots.  a provides the grid (lists, counts, offsets,
+// lists_capacity) and the light records (lights, num_lights).
 template <int GRID, class A, class F>
 __device__ __forceinline__ void walk_lights(const A& a, HotLight* s_l, const Pixel& p, bool valid,
-                                            uint32_t cell, F&& f) {
+                                            uint32_t cell, F&& f, KeptLights* kept = nullptr,
+                                            KeptState* ks = nullptr) {
     const uint32_t lane = hl_lane();
     uint64_t todo = hl_ballot(valid);
     while (todo) {
         const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
         const uint32_t ucell = hl_readlane(cell, leader);
         const bool mine = valid && cell == ucell;
-        todo &= ~hl_ballot(mine);
+        const uint64_t group = hl_ballot(mine);
+        todo &= ~group;
         const float bx0 = dpp_min_if(mine, p.wx), bx1 = dpp_max_if(mine, p.wx);
         const float by0 = dpp_min_if(mine, p.wy), by1 = dpp_max_if(mine, p.wy);
         const float bz0 = dpp_min_if(mine, p.wz), bz1 = dpp_max_if(mine, p.wz);
@@ -260,34 +276,54 @@ __device__ __forceinline__ void walk_lights(const A& a, HotLight* s_l, const Pix
             const uint32_t j = j0 + lane;
             bool pass = false;
             uint32_t idx = 0;
+            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), l1 = q, l2 = q, l3 = q, l4 = q;
             if (j < cnt) {
                 idx = hl_g(list)[j];
                 if (idx < a.num_lights) {
                     // the whole record at once, so that staging it does not
                     // wait on a second load after the test
                     const HotLight* hp = a.lights + idx;
-                    const float4 q = hl_ld4(&hp->p0), l1 = hl_ld4(&hp->p1), l2 = hl_ld4(&hp->p2),
-                                 l3 = hl_ld4(&hp->p3), l4 = hl_ld4(&hp->p4);
+                    q = hl_ld4(&hp->p0);
+                    l1 = hl_ld4(&hp->p1);
+                    l2 = hl_ld4(&hp->p2);
+                    l3 = hl_ld4(&hp->p3);
+                    l4 = hl_ld4(&hp->p4);
                     const float dx = fmaxf(fmaxf(bx0 - q.x, q.x - bx1), 0.0f);
                     const float dy = fmaxf(fmaxf(by0 - q.y, q.y - by1), 0.0f);
                     const float dz = fmaxf(fmaxf(bz0 - q.z, q.z - bz1), 0.0f);
                     const float d2 = dx * dx + dy * dy + dz * dz;
                     pass = d2 <= q.w * q.w * 1.0001f + 1e-12f;
-                    if (pass) {
-                        s_l[lane].p0 = q;
-                        s_l[lane].p1 = l1;
-                        s_l[lane].p2 = l2;
-                        s_l[lane].p3 = l3;
-                        s_l[lane].p4 = l4;
-                    }
                 }
             }
             uint64_t m = hl_ballot(pass);
+            // slots: the lane's own, or the kept range's next ones in lane order
+            bool keep = false;
+            uint32_t first = 0;
+            if (kept) {
+                const uint32_t np = (uint32_t)__popcll(m);
+                if (ks->ok && ks->n + np > (uint32_t)kStage) ks->ok = false;
+                keep = ks->ok;
+                first = ks->n;
+                if (keep) ks->n += np;
+            }
+            if (pass) {
+                const uint32_t slot = keep ? first + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : lane;
+                s_l[slot].p0 = q;
+                s_l[slot].p1 = l1;
+                s_l[slot].p2 = l2;
+                s_l[slot].p3 = l3;
+                s_l[slot].p4 = l4;
+                if (keep) {
+                    kept->idx[slot] = idx;
+                    kept->lanes[slot] = group;
+                }
+            }
             hl_wave_sync();
+            uint32_t slot = first;
             while (m) {
                 const uint32_t k = (uint32_t)__builtin_ctzll(m);
                 m &= m - 1u;
-                const HotLight L = s_l[k];
+                const HotLight L = s_l[keep ? slot++ : k];
                 f(L, hl_readlane(idx, k), mine);
             }
             hl_wave_sync();

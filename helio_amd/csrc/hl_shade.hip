@@ -122,6 +122,8 @@ struct ViewConst {
 
 struct ShadeSmem {
     HotLight light[4][kStage];
+    KeptLights kept[4];    // the fused adjoint's second pass
+    float red[4][5][64];   // the adjoint's per-wave partial sums of a pixel tile
     ViewConst vc;
 };
 
@@ -142,15 +144,25 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     const ViewConst& vc = sm.vc;
     b = hl_uniform(b);  // the item is the block's: its tile arithmetic stays scalar
     const uint32_t lane = threadIdx.x & 63u, wave = hl_uniform(threadIdx.x >> 6);
+    constexpr bool kAdj = MODE == kModeBackward || MODE == kModeFusedL2;
+    // The adjoint modes split a wave tile's adjoint pass over the block's four
+    // waves (every fourth light each; the items are wave tiles, four per block
+    // tile): its per-light work is several times the forward's, and the
+    // heaviest tiles would otherwise set the launch's length.  The fused
+    // mode's forward pass stays whole in every wave, so that its output is the
+    // forward's bit for bit.  The other modes give each wave its own tile of
+    // the block tile.
+    constexpr bool kSplit = kAdj;
     const uint32_t twl = a.tw_log2, tw = 1u << twl;
     const uint32_t th = 64u >> twl;
-    const uint32_t bx = b % a.block_tiles_x, by = b / a.block_tiles_x;
-    const uint32_t x = bx * (4u * tw) + wave * tw + (lane & (tw - 1u));
+    const uint32_t bt = kSplit ? b >> 2 : b, sub = kSplit ? (b & 3u) : wave;
+    const uint32_t bx = bt % a.block_tiles_x, by = bt / a.block_tiles_x;
+    const uint32_t x = bx * (4u * tw) + sub * tw + (lane & (tw - 1u));
     const uint32_t y = by * th + (lane >> twl);
     const bool inside = x < a.width && y < a.height;
     const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
     const bool valid = inside && depth < 1.0f;
-    constexpr bool kAdj = MODE == kModeBackward || MODE == kModeFusedL2;
+    const bool lead = !kSplit || wave == 0u;  // writes the tile's shared results
     constexpr bool kRC = MODE == kModeExtended;
     constexpr bool kExt = MODE == kModeShadowed || kRC;  // exact N, shadow_factor
     constexpr bool kFwd = MODE == kModeForward || kExt;
@@ -160,7 +172,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
                          : nullptr;
     const size_t plane = (size_t)a.width * a.height;
     const size_t pix = (size_t)y * a.width + x;
-    if (inside && !valid) {
+    if (inside && !valid && lead) {
         if (kWrites && (a.flags & HL_SHADE_CLEAR_SKY))
             hl_store_nt(out_px, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         if (kAdj)
@@ -308,6 +320,10 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 
     HotLight* s_l = sm.light[wave];
     float lo[3] = {0.0f, 0.0f, 0.0f};
+    // the fused adjoint revisits the forward walk's lights from LDS when they
+    // fit (kStage per wave), else walks the lists again
+    KeptState ks;
+    KeptLights* kept = MODE == kModeFusedL2 ? &sm.kept[wave] : nullptr;
 
     if (MODE == kModeCount) {
         uint32_t pairs = 0;
@@ -375,7 +391,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
                 lo[1] += t.B[1] * (L.p1.y * aw);
                 lo[2] += t.B[2] * (L.p1.z * aw);
             }
-        });
+        }, kept, &ks);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = lo[c] + amb[c];
@@ -389,7 +405,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     if (MODE == kModeFusedL2) {
         float l2 = 0.0f;
         if (valid) {
-            hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+            if (lead) hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
             const float4 tg = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
             const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
             gl[0] = 2.0f * e0;
@@ -398,7 +414,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             l2 = e0 * e0 + e1 * e1 + e2 * e2;
         }
         const float wl = dpp_sum(l2);
-        if (lane == 0) atomicAdd(a.loss, (double)wl);
+        if (lane == 0 && lead) atomicAdd(a.loss, (double)wl);
     } else if (valid) {
         const float4 g = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
         gl[0] = g.x;
@@ -409,7 +425,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     // per-light gradients go to one of kGradSlots copies (by wave) so that the
     // FP64 atomics of many waves on one light do not serialise on one address
     const uint32_t slot = (b * 4u + wave) % kGradSlots;
-    walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
+    const auto adj = [&](const HotLight& L, uint32_t li, bool mine) {
         float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         bool act = false;
         if (mine) {
@@ -424,8 +440,37 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
             const float v = lane == 0u ? sx : (lane == 1u ? sy : (lane == 2u ? sz : si));
             atomicAdd(a.d_light + 4 * ((size_t)slot * a.num_lights + li) + lane, (double)v);
         }
-    });
-    if (valid) {
+    };
+    if (MODE == kModeFusedL2 && ks.ok) {
+        for (uint32_t i = kSplit ? wave : 0u; i < ks.n; i += kSplit ? 4u : 1u) {  // uniform
+            const HotLight L = s_l[i];
+            const uint32_t li = hl_uniform(kept->idx[i]);
+            const uint64_t lanes = kept->lanes[i];
+            adj(L, li, ((lanes >> lane) & 1ull) != 0ull);
+        }
+    } else {
+        uint32_t ord2 = 0;
+        walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
+            if (kSplit && (ord2++ & 3u) != wave) return;
+            adj(L, li, mine);
+        });
+    }
+    if (kSplit) {
+        // the waves' partial per-pixel gradients, added in wave order
+        sm.red[wave][0][lane] = dA[0];
+        sm.red[wave][1][lane] = dA[1];
+        sm.red[wave][2][lane] = dA[2];
+        sm.red[wave][3][lane] = dR;
+        sm.red[wave][4][lane] = dM;
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            dA[c] = ((sm.red[0][c][lane] + sm.red[1][c][lane]) + sm.red[2][c][lane]) + sm.red[3][c][lane];
+        dR = ((sm.red[0][3][lane] + sm.red[1][3][lane]) + sm.red[2][3][lane]) + sm.red[3][3][lane];
+        dM = ((sm.red[0][4][lane] + sm.red[1][4][lane]) + sm.red[2][4][lane]) + sm.red[3][4][lane];
+        __syncthreads();
+    }
+    if (valid && lead) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) dA[c] += gl[c] * mixv[c] * p.ao;
         const hl_gptr<float> dp = hl_g(a.d_pixel);
@@ -519,7 +564,7 @@ __global__ __launch_bounds__(256, (shade_min_blocks<MODE, FA>())) void k_shade(S
     __shared__ ClaimSmem cs;
     if (blockIdx.x == 0 && threadIdx.x < kShadeQueues) queue_next[threadIdx.x * kShadeQueueStride] = 0u;
     load_view_const(a, sm);  // published by the first claim's barrier
-    claim_items(a.num_blocks, queue, cs, [&](uint32_t t) { shade_tile<FA, FN, GRID, MODE>(a, t, sm); });
+    claim_items(shade_items(MODE, a.num_blocks), queue, cs, [&](uint32_t t) { shade_tile<FA, FN, GRID, MODE>(a, t, sm); });
 }
 
 // Many views, one persistent launch: the items are the block tiles of the
@@ -531,9 +576,9 @@ __global__ __launch_bounds__(256, (shade_min_blocks<MODE, FA>())) void k_shade(S
 // keeps the item range of its current view; when an item falls outside it
 // (rare: each sub-queue's items run through one view at a time) the block
 // finds the span by bisection and stages that view's arguments and constants
-// in LDS, where they need fewer registers than read from the view record.
+// in LDS; each item then reads the arguments into scalar registers.
 template <int FA, int FN, int MODE>
-__global__ __launch_bounds__(256, MODE == kModeForward ? 4 : 1) void k_shade_batch(
+__global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_batch(
     const BatchShadeView* __restrict__ views, const uint2* __restrict__ spans, uint32_t nspans,
     uint32_t num_items, uint32_t* __restrict__ queue) {
     static_assert(sizeof(ShadeArgs) % 4 == 0, "copied as words");
@@ -558,7 +603,14 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 4 : 1) void k_shade_bat
             for (uint32_t i = threadIdx.x; i < sizeof(ShadeArgs) / 4; i += blockDim.x) ((uint32_t*)&sa)[i] = src[i];
             __syncthreads();
         }
-        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(sa, it - first, sm);
+        // the view's arguments as wave-uniform values (scalar registers), as
+        // k_shade has them from its kernel arguments: read from LDS they
+        // would take vector registers and cap the forward at 4 waves per SIMD
+        ShadeArgs ra;
+#pragma unroll
+        for (uint32_t i = 0; i < sizeof(ShadeArgs) / 4; ++i)
+            ((uint32_t*)&ra)[i] = __builtin_amdgcn_readfirstlane(((const uint32_t*)&sa)[i]);
+        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(ra, it - first, sm);
     });
 }
 
@@ -614,11 +666,12 @@ template <int FA, int FN, int MODE>
 static hipError_t launch_grid(hipStream_t s, const ShadeArgs& a, uint32_t* queue, uint32_t* queue_next) {
     if (a.grid_mode == HL_GRID_TILES)
         k_shade<FA, FN, HL_GRID_TILES, MODE>
-            <<<persistent_blocks<FA, FN, HL_GRID_TILES, MODE>(a.num_blocks), 256, 0, s>>>(a, queue, queue_next);
+            <<<persistent_blocks<FA, FN, HL_GRID_TILES, MODE>(shade_items(MODE, a.num_blocks)), 256, 0, s>>>(
+                a, queue, queue_next);
     else
         k_shade<FA, FN, HL_GRID_CLUSTERS, MODE>
-            <<<persistent_blocks<FA, FN, HL_GRID_CLUSTERS, MODE>(a.num_blocks), 256, 0, s>>>(a, queue,
-                                                                                           queue_next);
+            <<<persistent_blocks<FA, FN, HL_GRID_CLUSTERS, MODE>(shade_items(MODE, a.num_blocks)), 256, 0, s>>>(
+                a, queue, queue_next);
     return hipGetLastError();
 }
 
