@@ -142,3 +142,34 @@ def test_rc_gi_matches_oracle(ctx, hlib, name, mode, fmt, shadows, over):
     changed = np.mean(np.any(ref[~sky][:, :3] != plain[~sky][:, :3], axis=1))
     print(f"  max rel err {r:.2e}, pixels with RC ambient {changed:.2f}")
     assert changed > 0.1
+
+
+def test_rc_ambient_matches_float64(ctx):
+    """The kernel's radiance-cascade ambient (k_rc_irradiance, then the
+    extended shading launch) against the float64 restatement of
+    sample_rc_irradiance and the combine (tests/rc64.py), on a frame without
+    lights and random probes: within 2e-5 relative plus 2^-17 of the atlas's
+    largest value, as the oracle (test_rc_gi.py)."""
+    import rc64
+    from helio_amd.device import DeviceTexture
+    print("[rc gi] kernel ambient against float64", flush=True)
+    sc = synth.make_scene("c4_512_adj", width=128, height=128, num_lights=0)
+    lo, hi = (-40.0, -5.0, -40.0), (40.0, 25.0, 40.0)
+    atlas = synth.make_rc_probes(sc, fmt=abi.HL_FMT_RGBA32_FLOAT, box=(lo, hi))
+    ds = upload_scene(ctx, sc)
+    cull = LightCullPass(ctx, sc.width, sc.height)
+    cull.prepare(ds)
+    cull.execute(ds)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    shade.execute(ds, cull.publish(), rc=DeviceTexture(ctx, atlas))
+    out = shade.output()
+    ys, xs = np.nonzero(sc.depth < 1.0)
+    want, cw = rc64.ambient64(sc, atlas, lo, hi, xs, ys)
+    keep = ~np.any(np.abs(cw - 0.001) < 1e-6, axis=1)
+    amax = np.abs(atlas[..., :3]).max()
+    err = np.abs(out[ys, xs, :3] - want)[keep]
+    assert keep.sum() > 5000
+    bound = 2e-5 * np.abs(want[keep]) + 2.0 ** -17 * amax
+    print(f"  max err / bound {float((err / bound).max()):.3f}")
+    assert np.all(err <= bound)

@@ -10,7 +10,12 @@ The oracle (oracle/rc_gi.c) is pinned by
   along x is reproduced between probe centres (trilinear);
 * the shaded frame: has_rc_gi = 0 is bit-identical to the plain shading; with
   RC, pixels outside the box keep the hemisphere ambient bit for bit, and
-  inside it the ambient follows mix(hemi, kD_ibl rc albedo, clamp(4|rc|,0,1)).
+  inside it the ambient follows mix(hemi, kD_ibl rc albedo, clamp(4|rc|,0,1));
+* a float64 restatement written from the shader's description (tests/rc64.py):
+  the oracle's irradiance is within 2^-17 of the atlas's largest value (64 ulp
+  of it; 1.6e-6 observed) at 20 000 random points, normals and probe atlases of
+  both formats, and so is its shaded ambient; a transposed direction texel or a
+  dropped volume fade in the restatement is off by 0.19 and 0.73 of it.
 """
 import ctypes as C
 import math
@@ -20,6 +25,7 @@ import pytest
 
 from helio_amd import abi, synth
 import oracle_lib as O
+import rc64
 
 cf = C.c_float
 
@@ -174,3 +180,57 @@ def test_rc_combine_closed_form():
             np.testing.assert_allclose(out[y, x, :3], want, rtol=2e-5, atol=1e-7)
             checked += 1
     assert checked > 100
+
+
+RC_BOUND = 2.0 ** -17  # of the atlas's largest value
+
+
+def oracle_irradiance(atlas, lo, hi, wp, N):
+    t = synth.rc_texture_host(atlas)
+    out = np.zeros((len(wp), 3))
+    o = (cf * 3)()
+    for i in range(len(wp)):
+        O.oracle().or_rc_irradiance(C.byref(t), (cf * 3)(*lo), (cf * 3)(*hi), (cf * 3)(*wp[i]), (cf * 3)(*N[i]), o)
+        out[i] = o[:]
+    return out
+
+
+@pytest.mark.parametrize("fmt", [abi.HL_FMT_RGBA16_FLOAT, abi.HL_FMT_RGBA32_FLOAT])
+def test_irradiance_against_float64(fmt):
+    rng = np.random.default_rng(3 + fmt)
+    sc = synth.make_scene("c1_720p_64", width=64, height=64)
+    lo, hi = np.array([-10.0, -2.0, -10.0]), np.array([10.0, 8.0, 10.0])
+    atlas = synth.make_rc_probes(sc, fmt=fmt, box=(tuple(lo), tuple(hi)), seed=int(rng.integers(1, 1000)))
+    n = 20000
+    wp = rng.uniform(lo - 1.0, hi + 1.0, (n, 3)).astype(np.float32)
+    N = rng.normal(size=(n, 3))
+    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(np.float32)
+    got = oracle_irradiance(atlas, lo, hi, wp, N)
+    ref, cw = rc64.rc_irradiance64(atlas, lo, hi, wp, N)
+    margin = np.any(np.abs(cw - 0.001) < 1e-6, axis=1)  # the w > 0.001 selection decides by rounding
+    amax = np.abs(np.asarray(atlas, np.float64)[..., :3]).max()
+    err = np.abs(got - ref)[~margin]
+    assert (~margin).sum() > 19000 and (ref[~margin] > 0).mean() > 0.5
+    assert err.max() <= RC_BOUND * amax, err.max() / amax
+    for mistake in (dict(texel_order="yx"), dict(fade=False)):
+        bad, _ = rc64.rc_irradiance64(atlas, lo, hi, wp, N, **mistake)
+        assert np.abs(got - bad)[~margin].max() > 100 * RC_BOUND * amax, mistake
+
+
+def test_shaded_ambient_against_float64():
+    """Random probes, no lights, fp32 G-buffer: the oracle's frame against the
+    float64 ambient of every fifth pixel in the box, within 2e-5 relative plus
+    the irradiance bound."""
+    sc = synth.make_scene("c4_512_adj", width=128, height=128, num_lights=0)
+    lo, hi = (-40.0, -5.0, -40.0), (40.0, 25.0, 40.0)
+    atlas = synth.make_rc_probes(sc, fmt=abi.HL_FMT_RGBA32_FLOAT, box=(lo, hi))
+    lists, counts = O.cull_tiles(sc)
+    out, _ = O.shade(sc, O.tile_grid_host(sc, lists.ravel(), counts), rc=atlas)
+    ys, xs = np.nonzero(sc.depth < 1.0)
+    ys, xs = ys[::5], xs[::5]
+    want, cw = rc64.ambient64(sc, atlas, lo, hi, xs, ys)
+    keep = ~np.any(np.abs(cw - 0.001) < 1e-6, axis=1)
+    amax = np.abs(atlas[..., :3]).max()
+    err = np.abs(out[ys, xs, :3] - want)[keep]
+    assert keep.sum() > 1000
+    assert np.all(err <= 2e-5 * np.abs(want[keep]) + RC_BOUND * amax)
