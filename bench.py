@@ -328,14 +328,17 @@ def bench_forward_lit(ctx, args):
                        ("clusters", LightCullPass(ctx, sc.width, sc.height, clusters=params, pool_cap=1 << 24))):
         cull.prepare(ds)
         cull.execute(ds)
+        cull_ms, _ = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds))
         grid = cull.publish()
         kw = dict(entity=d_ent, transforms=d_tr, num_transforms=len(transforms))
         pairs = fwd.count_pairs(ds, d_fr, n, grid, **kw)
         ms, _ = time_steps(ctx, args.steps, args.warmup, lambda: fwd.execute(ds, d_fr, n, grid, **kw))
         res[mode] = {"ms_per_call": ms, "pairs": pairs, "evals_per_s": pairs / (ms * 1e-3),
                      "fragments_per_s": n / (ms * 1e-3),
-                     "bytes_fragment_io": 96 * n, "fragment_io_GBs": 96 * n / (ms * 1e-3) / 1e9}
-        log(f"forward-lit 4K {mode}: {ms * 1e3:.1f} us, {pairs / (ms * 1e-3) / 1e9:.1f} G evals/s")
+                     "bytes_fragment_io": 96 * n, "fragment_io_GBs": 96 * n / (ms * 1e-3) / 1e9,
+                     "cull_ms": cull_ms}
+        log(f"forward-lit 4K {mode}: {ms * 1e3:.1f} us, {pairs / (ms * 1e-3) / 1e9:.1f} G evals/s; "
+            f"its light cull {cull_ms * 1e3:.1f} us")
     res["tiles"]["grid"] = "reference tile_light_lists (16 px, cap 64, ascending)"
     res["clusters"]["grid"] = f"{params.grid_x}x{params.grid_y}x{params.grid_z} log-depth clusters, uncapped"
     return res

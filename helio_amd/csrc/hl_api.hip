@@ -387,7 +387,7 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->markq, &c->shadeq,
-                      &c->words, &c->accum, &c->vogel, &c->rcirr})
+                      &c->words, &c->accum, &c->vogel, &c->rcirr, &c->tilebin})
         if (b->p) (void)hipFree(b->p);
     delete c;
     return HL_OK;
@@ -586,8 +586,24 @@ HL_API hl_status hl_light_cull_tiles(hl_context* c, const GpuCameraUniforms* cam
                           (HotLight*)c->hot.p, (CullLight*)c->cull.p, ctx_words(c) + 1));
     c->hot_src = lights_d;
     c->hot_n = params->num_lights;
-    HL_HIP(c, launch_cull_tiles(c->stream, camera_d, *params, (const CullLight*)c->cull.p, lists_d,
-                                counts_d));
+    const LightCullParams& p = *params;
+    const uint64_t n = p.num_lights, nsup = tile_bin_supers(p);
+    if (p.num_tiles_x == (p.screen_width + HL_TILE_SIZE - 1) / HL_TILE_SIZE &&
+        p.num_tiles_y == (p.screen_height + HL_TILE_SIZE - 1) / HL_TILE_SIZE && nsup * n <= 0xFFFFFFFFull) {
+        // the screen's tile grid: candidates binned per super-tile
+        if ((s = ensure(c, c->bits, 4 * cluster_bits_words((uint32_t)nsup, (uint32_t)n), true)) != HL_OK) return s;
+        if ((s = ensure(c, c->summary, 4 * cluster_summary_words((uint32_t)nsup, (uint32_t)n), true)) != HL_OK)
+            return s;
+        if ((s = ensure(c, c->raw, cluster_raw_bytes((uint32_t)nsup))) != HL_OK) return s;
+        if ((s = ensure(c, c->tilebin, 4 * (2 * nsup + nsup * n))) != HL_OK) return s;
+        uint32_t* tb = (uint32_t*)c->tilebin.p;
+        const TileBinScratch w{(uint32_t*)c->bits.p, (uint32_t*)c->summary.p, (uint32_t*)c->raw.p, tb, tb + nsup,
+                               tb + 2 * nsup, (uint32_t)(nsup * n), ctx_words(c) + 4};
+        HL_HIP(c, launch_cull_tiles_binned(c->stream, camera_d, p, (const CullLight*)c->cull.p, lists_d, counts_d,
+                                           w));
+    } else {
+        HL_HIP(c, launch_cull_tiles(c->stream, camera_d, p, (const CullLight*)c->cull.p, lists_d, counts_d));
+    }
     return HL_OK;
 }
 

@@ -80,7 +80,30 @@ __global__ __launch_bounds__(256) void k_prep_lights(const GpuLight* __restrict_
 }
 
 // -------------------------------------------------------------- tile cull
-// One wavefront per 16x16 tile; four tiles per 256-thread block.
+// The four planes of a 16x16 tile (light_cull.wgsl:123-150).
+__device__ __forceinline__ void tile_planes(const GpuCameraUniforms* __restrict__ cam, const LightCullParams& p,
+                                           uint32_t tile_x, uint32_t tile_y, float3 pl[4]) {
+#pragma clang fp contract(off)
+    const float fw = (float)p.screen_width, fh = (float)p.screen_height;
+    const float ts = (float)HL_TILE_SIZE;
+    float ndc_left = ((float)tile_x * ts / fw) * 2.0f - 1.0f;
+    float ndc_right = fminf(((float)(tile_x + 1u) * ts / fw) * 2.0f - 1.0f, 1.0f);
+    float ndc_top = 1.0f - ((float)tile_y * ts / fh) * 2.0f;
+    float ndc_bottom = fmaxf(1.0f - ((float)(tile_y + 1u) * ts / fh) * 2.0f, -1.0f);
+    const float p00 = cam->proj[0], p11 = cam->proj[5];
+    pl[0] = hl_x_plane(p00, p11, ndc_left, ndc_bottom, ndc_left, ndc_top);
+    pl[1] = hl_x_plane(p00, p11, ndc_right, ndc_top, ndc_right, ndc_bottom);
+    pl[2] = hl_x_plane(p00, p11, ndc_left, ndc_top, ndc_right, ndc_top);
+    pl[3] = hl_x_plane(p00, p11, ndc_right, ndc_bottom, ndc_left, ndc_bottom);
+}
+
+__device__ __forceinline__ bool tile_pass(const float3 pl[4], float4 c) {
+    return hl_x_plane_pass(pl[0], c, c.w) && hl_x_plane_pass(pl[1], c, c.w) && hl_x_plane_pass(pl[2], c, c.w) &&
+           hl_x_plane_pass(pl[3], c, c.w);
+}
+
+// One wavefront per 16x16 tile; four tiles per 256-thread block; every light
+// tested (any tile grid).
 __global__ __launch_bounds__(256) void k_cull_tiles(const GpuCameraUniforms* __restrict__ cam,
                                                     LightCullParams p,
                                                     const CullLight* __restrict__ cl,
@@ -92,30 +115,13 @@ __global__ __launch_bounds__(256) void k_cull_tiles(const GpuCameraUniforms* __r
     if (tile >= total) return;
     const uint32_t tile_x = tile % p.num_tiles_x, tile_y = tile / p.num_tiles_x;
     float3 pl[4];
-    {
-#pragma clang fp contract(off)
-        const float fw = (float)p.screen_width, fh = (float)p.screen_height;
-        const float ts = (float)HL_TILE_SIZE;
-        float ndc_left = ((float)tile_x * ts / fw) * 2.0f - 1.0f;
-        float ndc_right = fminf(((float)(tile_x + 1u) * ts / fw) * 2.0f - 1.0f, 1.0f);
-        float ndc_top = 1.0f - ((float)tile_y * ts / fh) * 2.0f;
-        float ndc_bottom = fmaxf(1.0f - ((float)(tile_y + 1u) * ts / fh) * 2.0f, -1.0f);
-        const float p00 = cam->proj[0], p11 = cam->proj[5];
-        pl[0] = hl_x_plane(p00, p11, ndc_left, ndc_bottom, ndc_left, ndc_top);
-        pl[1] = hl_x_plane(p00, p11, ndc_right, ndc_top, ndc_right, ndc_bottom);
-        pl[2] = hl_x_plane(p00, p11, ndc_left, ndc_top, ndc_right, ndc_top);
-        pl[3] = hl_x_plane(p00, p11, ndc_right, ndc_bottom, ndc_left, ndc_bottom);
-    }
+    tile_planes(cam, p, tile_x, tile_y, pl);
     uint32_t count = 0;
     uint32_t* out = lists + (size_t)tile * HL_MAX_LIGHTS_PER_TILE;
     for (uint32_t base = 0; base < p.num_lights; base += 64u) {
         const uint32_t i = base + lane;
         bool pass = false;
-        if (i < p.num_lights) {
-            const float4 c = cl[i].vpos;
-            pass = hl_x_plane_pass(pl[0], c, c.w) && hl_x_plane_pass(pl[1], c, c.w) &&
-                   hl_x_plane_pass(pl[2], c, c.w) && hl_x_plane_pass(pl[3], c, c.w);
-        }
+        if (i < p.num_lights) pass = tile_pass(pl, cl[i].vpos);
         const uint64_t m = hl_ballot(pass);
         const uint32_t pos = count + hl_prefix(m);
         if (pass && pos < HL_MAX_LIGHTS_PER_TILE) out[pos] = i;
@@ -954,6 +960,142 @@ hipError_t launch_batch_cull(hipStream_t s, const GpuLight* pool, const BatchCul
     return hipGetLastError();
 }
 
+// ------------------------------------------------------ tile cull, binned
+// Candidate tiles of a light for the tile planes' test (inclusive rectangle;
+// false when empty).  Necessary conditions of passing a tile's four planes
+// (normals (1, 0, l/p00), -(1, 0, r/p00), (0, -1, -t/p11), (0, 1, b/p11),
+// normalised by s = sqrt(1 + e^2/p00^2) <= fx for an edge |e| <= 1), with
+// the light's centre c, view distance zc = -c.z and radius r (|r| <= rr):
+//  - in front (zc > 0): l <= p00 (cx + rr fx) / zc and r >= p00 (cx - rr fx) / zc,
+//    rows alike with p11 (the cluster cull's bound);
+//  - behind the camera (zc < 0, |zc| = -zc): the planes' half-spaces meet
+//    only near the mirrored projection, l >= p00 (-cx - rr fx) / |zc| and
+//    r <= p00 (rr fx - cx) / |zc|, t <= p11 (rr fy - cy) / |zc| and
+//    b >= p11 (-rr fy - cy) / |zc|;
+//  - within 1e-4 of the camera plane, a non-finite centre or radius, or a
+//    camera with p00 or p11 <= 0: every tile.
+// Rounded outwards as the cluster cull's candidates: rr widened by 1e-5 of
+// itself and of |cx| + |cy| + |zc|, the pixel bounds by 1e-5 of their
+// magnitude plus a quarter pixel.
+__device__ __forceinline__ bool light_tiles(const GpuCameraUniforms* __restrict__ cam, const LightCullParams& p,
+                                            float4 c, int* tx0, int* tx1, int* ty0, int* ty1) {
+    const int ntx = (int)p.num_tiles_x, nty = (int)p.num_tiles_y;
+    *tx0 = 0;
+    *tx1 = ntx - 1;
+    *ty0 = 0;
+    *ty1 = nty - 1;
+    const float zc = -c.z, p00 = cam->proj[0], p11 = cam->proj[5];
+    if (!__builtin_isfinite(c.w) || !__builtin_isfinite(c.x) || !__builtin_isfinite(c.y) ||
+        !__builtin_isfinite(zc) || !(p00 > 0.0f && p11 > 0.0f) || fabsf(zc) <= 1e-4f)
+        return ntx > 0 && nty > 0;
+    const float slack = 1e-5f * (fabsf(c.x) + fabsf(c.y) + fabsf(zc));
+    const float rr = fabsf(c.w) * (1.0f + 1e-5f) + slack;
+    const float fx = __builtin_amdgcn_sqrtf(1.0f + hl_f_rcp(p00 * p00));
+    const float fy = __builtin_amdgcn_sqrtf(1.0f + hl_f_rcp(p11 * p11));
+    const float iz = hl_f_rcp(fabsf(zc));
+    float nx0, nx1, nt, nb;  // column edges: l <= nx1, r >= nx0 (front) or l >= nx0, r <= nx1 (behind)
+    if (zc > 0.0f) {
+        nx0 = p00 * (c.x - rr * fx) * iz;
+        nx1 = p00 * (c.x + rr * fx) * iz;
+        nb = p11 * (c.y - rr * fy) * iz;  // t >= nb
+        nt = p11 * (c.y + rr * fy) * iz;  // b <= nt
+    } else {
+        nx0 = p00 * (-c.x - rr * fx) * iz;
+        nx1 = p00 * (rr * fx - c.x) * iz;
+        nt = p11 * (rr * fy - c.y) * iz;   // t <= nt
+        nb = p11 * (-rr * fy - c.y) * iz;  // b >= nb
+    }
+    const float fw = (float)p.screen_width, fh = (float)p.screen_height;
+    // in both cases the candidate pixel columns lie in [px0, px1] and rows in
+    // [py0, py1]: front, from l <= nx1 and r >= nx0; behind, from l >= nx0 and
+    // r <= nx1 (the same interval, reached from the other side)
+    float px0 = (nx0 + 1.0f) * 0.5f * fw, px1 = (nx1 + 1.0f) * 0.5f * fw;
+    float py0 = (1.0f - nt) * 0.5f * fh, py1 = (1.0f - nb) * 0.5f * fh;
+    px0 -= 1e-5f * fabsf(px0) + 0.25f;
+    px1 += 1e-5f * fabsf(px1) + 0.25f;
+    py0 -= 1e-5f * fabsf(py0) + 0.25f;
+    py1 += 1e-5f * fabsf(py1) + 0.25f;
+    const float its = 1.0f / (float)HL_TILE_SIZE;  // exact
+    // clamped to [-1, n] before the conversion (NaN: the whole range)
+    const int x0 = (int)fminf(fmaxf(floorf(px0 * its), -1.0f), (float)ntx);
+    const int x1 = (int)fmaxf(fminf(floorf(px1 * its), (float)ntx), -1.0f);
+    const int y0 = (int)fminf(fmaxf(floorf(py0 * its), -1.0f), (float)nty);
+    const int y1 = (int)fmaxf(fminf(floorf(py1 * its), (float)nty), -1.0f);
+    *tx0 = clampi(x0, 0, ntx - 1);
+    *tx1 = clampi(x1, -1, ntx - 1);
+    *ty0 = clampi(y0, 0, nty - 1);
+    *ty1 = clampi(y1, -1, nty - 1);
+    return *tx1 >= *tx0 && *ty1 >= *ty0;
+}
+
+// One wavefront per light: the super-tiles its candidate tiles touch get
+// its bit (word-major bitmask over nsup cells, as the cluster cull's).
+// Block 0 also zeroes the scan tile sums that the count kernel adds to.
+__global__ __launch_bounds__(256) void k_tile_bin(const GpuCameraUniforms* __restrict__ cam, LightCullParams p,
+                                                  const CullLight* __restrict__ cl, uint32_t* __restrict__ bits,
+                                                  uint32_t* __restrict__ summary,
+                                                  unsigned long long* __restrict__ tile_sums, uint32_t ntile_sums) {
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < ntile_sums; i += blockDim.x) tile_sums[i] = 0ull;
+    const uint32_t li = hl_uniform(blockIdx.x * 4u + (threadIdx.x >> 6));
+    if (li >= p.num_lights) return;
+    const float4 c = cl[li].vpos;
+    const float4 cu = make_float4(hl_uniform(c.x), hl_uniform(c.y), hl_uniform(c.z), hl_uniform(c.w));
+    int tx0, tx1, ty0, ty1;
+    if (!light_tiles(cam, p, cu, &tx0, &tx1, &ty0, &ty1)) return;
+    const uint32_t sx = (p.num_tiles_x + kSuperTile - 1u) / kSuperTile;
+    const uint32_t nsup = sx * ((p.num_tiles_y + kSuperTile - 1u) / kSuperTile);
+    const uint32_t s0x = (uint32_t)tx0 / kSuperTile, s1x = (uint32_t)tx1 / kSuperTile;
+    const uint32_t s0y = (uint32_t)ty0 / kSuperTile, s1y = (uint32_t)ty1 / kSuperTile;
+    const uint32_t nx = s1x - s0x + 1u, work = nx * (s1y - s0y + 1u);
+    const uint32_t word = li >> 5;
+    for (uint32_t t = threadIdx.x & 63u; t < work; t += 64u) {
+        const uint32_t iy = t / nx, ix = t - iy * nx;
+        const size_t sup = (size_t)(s0x + ix) + (size_t)sx * (s0y + iy);
+        __hip_atomic_fetch_or(hl_g(&bits[(size_t)word * nsup + sup]), 1u << (li & 31u), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_or(hl_g(&summary[(size_t)(word >> 5) * nsup + sup]), 1u << (word & 31u),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One wavefront per tile: its super-tile's candidates (ascending) through
+// the exact plane test, the first HL_MAX_LIGHTS_PER_TILE kept in order.
+__global__ __launch_bounds__(256) void k_tile_lists(const GpuCameraUniforms* __restrict__ cam, LightCullParams p,
+                                                    const CullLight* __restrict__ cl,
+                                                    const uint32_t* __restrict__ soff,
+                                                    const uint32_t* __restrict__ scnt,
+                                                    const uint32_t* __restrict__ spool,
+                                                    uint32_t* __restrict__ lists, uint32_t* __restrict__ counts) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t total = p.num_tiles_x * p.num_tiles_y;
+    if (tile >= total) return;
+    const uint32_t tile_x = tile % p.num_tiles_x, tile_y = tile / p.num_tiles_x;
+    const uint32_t sup = tile_x / kSuperTile + ((p.num_tiles_x + kSuperTile - 1u) / kSuperTile) * (tile_y / kSuperTile);
+    float3 pl[4];
+    tile_planes(cam, p, tile_x, tile_y, pl);
+    const uint32_t n = hl_uniform(scnt[sup]);
+    const uint32_t* cand = spool + hl_uniform(soff[sup]);
+    uint32_t count = 0;
+    uint32_t* out = lists + (size_t)tile * HL_MAX_LIGHTS_PER_TILE;
+    for (uint32_t base = 0; base < n; base += 64u) {
+        const uint32_t j = base + lane;
+        bool pass = false;
+        uint32_t i = 0;
+        if (j < n) {
+            i = cand[j];
+            pass = tile_pass(pl, cl[i].vpos);
+        }
+        const uint64_t m = hl_ballot(pass);
+        const uint32_t pos = count + hl_prefix(m);
+        if (pass && pos < HL_MAX_LIGHTS_PER_TILE) out[pos] = i;
+        count += (uint32_t)__popcll(m);
+        if (count >= HL_MAX_LIGHTS_PER_TILE) break;
+    }
+    if (lane == 0) counts[tile] = count < HL_MAX_LIGHTS_PER_TILE ? count : HL_MAX_LIGHTS_PER_TILE;
+}
+
 // ---------------------------------------------------------------- launchers
 hipError_t launch_prep(hipStream_t s, const GpuLight* lights, const uint32_t* subset, uint32_t n,
                        const GpuCameraUniforms* cam, HotLight* hot, CullLight* cull,
@@ -976,6 +1118,29 @@ hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const 
     const uint32_t total = p.num_tiles_x * p.num_tiles_y;
     if (total == 0) return hipSuccess;
     k_cull_tiles<<<(total + 3) / 4, 256, 0, s>>>(cam, p, cl, lists, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_cull_tiles_binned(hipStream_t s, const GpuCameraUniforms* cam, const LightCullParams& p,
+                                    const CullLight* cl, uint32_t* lists, uint32_t* counts,
+                                    const TileBinScratch& w) {
+    const uint32_t total = p.num_tiles_x * p.num_tiles_y;
+    if (total == 0) return hipSuccess;
+    const uint32_t nsup = tile_bin_supers(p);
+    const uint32_t W = (p.num_lights + 31u) / 32u, SW = (W + 31u) / 32u;
+    hipError_t e;
+    k_tile_bin<<<(p.num_lights + 3) / 4, 256, 0, s>>>(
+        cam, p, cl, w.bits, w.summary, (unsigned long long*)((uint8_t*)w.raw + cluster_tile_sums_offset(nsup)),
+        cluster_scan_tiles(nsup));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint32_t blocks = (nsup + 256u / kCellLanes - 1u) / (256u / kCellLanes);
+    k_cluster_count<<<blocks, 256, 0, s>>>(nsup, SW, w.bits, w.summary, w.raw, 0u);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the pool holds every (super-tile, light) pair: nothing is truncated
+    k_cluster_fill<<<blocks, 256, 0, s>>>(nsup, SW, w.raw, w.bits, w.summary, 0u, w.pool_cap, w.offsets,
+                                          w.counts, w.pool, w.flags, nullptr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_tile_lists<<<(total + 3) / 4, 256, 0, s>>>(cam, p, cl, w.offsets, w.counts, w.pool, lists, counts);
     return hipGetLastError();
 }
 

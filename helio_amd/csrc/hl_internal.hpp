@@ -33,7 +33,7 @@ struct hl_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum, vogel, rcirr;
+    DevBuf hot, cull, geom, bits, summary, raw, markq, shadeq, words, accum, vogel, rcirr, tilebin;
     bool vogel_ready = false;  // vogel holds the Vogel disk tables
     // this device's address of the no-alloc scratch array (hl_api.hip),
     // looked up once per device when the context is created
@@ -73,6 +73,31 @@ hipError_t launch_prep(hipStream_t s, const GpuLight* lights, const uint32_t* su
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
 hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const LightCullParams& p,
                              const CullLight* cl, uint32_t* lists, uint32_t* counts);
+// The tile cull over candidates: each light's candidate tiles (a conservative
+// rectangle from necessary conditions of the plane test) are binned into
+// super-tiles of kSuperTile x kSuperTile tiles; the super-tiles' ascending
+// candidate lists come from the cluster count and fill kernels; each tile then
+// tests its super-tile's candidates in order with the exact plane test.  Same
+// lists as launch_cull_tiles when the tile grid is the screen's
+// (num_tiles = ceil(size / 16)).  bits and summary: as the cluster cull's
+// (zero before, zero after), over nsup = super-tiles cells.
+constexpr uint32_t kSuperTile = 8;
+struct TileBinScratch {
+    uint32_t* bits;     // nsup * ceil(L/32) words
+    uint32_t* summary;  // nsup * ceil(L/1024) words
+    uint32_t* raw;      // cluster_raw_bytes(nsup)
+    uint32_t* offsets;  // nsup
+    uint32_t* counts;   // nsup
+    uint32_t* pool;     // pool_cap
+    uint32_t pool_cap;  // nsup * L (no truncation)
+    uint32_t* flags;    // the fill's overflow word (never set: no cap, no truncation)
+};
+inline uint32_t tile_bin_supers(const LightCullParams& p) {
+    return ((p.num_tiles_x + kSuperTile - 1u) / kSuperTile) * ((p.num_tiles_y + kSuperTile - 1u) / kSuperTile);
+}
+hipError_t launch_cull_tiles_binned(hipStream_t s, const GpuCameraUniforms* cam, const LightCullParams& p,
+                                    const CullLight* cl, uint32_t* lists, uint32_t* counts,
+                                    const TileBinScratch& w);
 // Scratch of the clustered cull.  bits (ncell * ceil(L/32) words) and summary
 // (ncell * ceil(L/1024) words) must be zero before the first call; every call
 // leaves them zero again.

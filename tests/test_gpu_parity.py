@@ -60,16 +60,50 @@ def hip_tiles(ctx, sc):
     return ds, cull, lists, counts
 
 
-@pytest.mark.parametrize("name", ["c1_720p_64", "c2_1080p_1k"])
-def test_tile_cull_bit_exact(ctx, name):
-    sc = scene(name)
-    print(f"[tile cull] {name}", flush=True)
-    _, _, lists, counts = hip_tiles(ctx, sc)
-    ol, oc = O.cull_tiles(sc)
+def check_tile_lists(lists, counts, ol, oc):
     assert np.array_equal(counts, oc)
     for t in np.nonzero(oc)[0]:
         n = oc[t]
         assert np.array_equal(lists[t, :n], ol[t, :n]), f"tile {t}"
+
+
+@pytest.mark.parametrize("name", ["c1_720p_64", "c2_1080p_1k", "c3_4k_16k", "c3_4k_16k_far", "around_camera"])
+def test_tile_cull_bit_exact(ctx, name):
+    """The screen's tile grid takes the binned path (candidates per 8 x 8
+    super-tile, exact plane test per tile): the same lists as the reference's
+    brute force, including lights behind and beside the camera."""
+    if name == "around_camera":
+        from test_tile_candidates import around_camera
+        sc = around_camera(3, n=64, width=1280, height=720)
+    else:
+        sc = scene(name)
+    print(f"[tile cull] {name}", flush=True)
+    _, _, lists, counts = hip_tiles(ctx, sc)
+    ol, oc = O.cull_tiles(sc)
+    check_tile_lists(lists, counts, ol, oc)
+    print(f"  tiles {len(oc)}, full tiles {float(np.mean(oc == 64)):.2f}")
+
+
+def test_tile_cull_other_grid(ctx):
+    """A tile grid other than the screen's (one column and row more) takes the
+    brute-force kernel: the same lists as the oracle's for the same params."""
+    import ctypes as C
+    sc = scene("c2_1080p_1k")
+    print("[tile cull] a grid wider than the screen", flush=True)
+    p = light_cull_params(sc)
+    p.num_tiles_x += 1
+    p.num_tiles_y += 1
+    tiles = p.num_tiles_x * p.num_tiles_y
+    ds = upload_scene(ctx, sc)
+    lists_d, counts_d = ctx.alloc(4 * 64 * tiles), ctx.alloc(4 * tiles)
+    ctx.lib.call("hl_light_cull_tiles", ctx.h, ds.camera.ptr, C.byref(p), ds.lights.ptr, lists_d.ptr, counts_d.ptr)
+    lists = lists_d.download(np.uint32).reshape(tiles, 64)
+    counts = counts_d.download(np.uint32)
+    ol = np.zeros(tiles * 64, np.uint32)
+    oc = np.zeros(tiles, np.uint32)
+    O.oracle().or_light_cull_tiles(C.byref(sc.camera), C.byref(p), sc.lights.ctypes.data, ol.ctypes.data,
+                                   oc.ctypes.data)
+    check_tile_lists(lists, counts, ol.reshape(tiles, 64), oc)
 
 
 def hip_clusters(ctx, sc, params, pool_cap):
