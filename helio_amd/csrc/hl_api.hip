@@ -595,10 +595,12 @@ HL_API hl_status hl_light_cull_tiles(hl_context* c, const GpuCameraUniforms* cam
         if ((s = ensure(c, c->summary, 4 * cluster_summary_words((uint32_t)nsup, (uint32_t)n), true)) != HL_OK)
             return s;
         if ((s = ensure(c, c->raw, cluster_raw_bytes((uint32_t)nsup))) != HL_OK) return s;
-        if ((s = ensure(c, c->tilebin, 4 * (2 * nsup + nsup * n))) != HL_OK) return s;
-        uint32_t* tb = (uint32_t*)c->tilebin.p;
+        const uint64_t tiles_n = (uint64_t)p.num_tiles_x * p.num_tiles_y;
+        if ((s = ensure(c, c->tilebin, 4 * (12 * tiles_n + 2 * nsup + nsup * n))) != HL_OK) return s;
+        float* planes = (float*)c->tilebin.p;
+        uint32_t* tb = (uint32_t*)(planes + 12 * tiles_n);
         const TileBinScratch w{(uint32_t*)c->bits.p, (uint32_t*)c->summary.p, (uint32_t*)c->raw.p, tb, tb + nsup,
-                               tb + 2 * nsup, (uint32_t)(nsup * n), ctx_words(c) + 4};
+                               tb + 2 * nsup, (uint32_t)(nsup * n), ctx_words(c) + 4, planes};
         HL_HIP(c, launch_cull_tiles_binned(c->stream, camera_d, p, (const CullLight*)c->cull.p, lists_d, counts_d,
                                            w));
     } else {

@@ -1030,14 +1030,34 @@ __device__ __forceinline__ bool light_tiles(const GpuCameraUniforms* __restrict_
 
 // One wavefront per light: the super-tiles its candidate tiles touch get
 // its bit (word-major bitmask over nsup cells, as the cluster cull's).
-// Block 0 also zeroes the scan tile sums that the count kernel adds to.
+// Block 0 also zeroes the scan tile sums that the count kernel adds to; the
+// blocks past the lights' (plane_blocks of them) build each tile's four
+// planes once, one thread per tile, for the list kernel's waves (a tile's
+// planes cost its wave some 800 VALU instructions of exact divisions and
+// square roots, the same in every lane).
 __global__ __launch_bounds__(256) void k_tile_bin(const GpuCameraUniforms* __restrict__ cam, LightCullParams p,
                                                   const CullLight* __restrict__ cl, uint32_t* __restrict__ bits,
                                                   uint32_t* __restrict__ summary,
-                                                  unsigned long long* __restrict__ tile_sums, uint32_t ntile_sums) {
-    if (blockIdx.x == 0)
+                                                  unsigned long long* __restrict__ tile_sums, uint32_t ntile_sums,
+                                                  float* __restrict__ planes, uint32_t plane_blocks) {
+    if (blockIdx.x < plane_blocks) {
+        const uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
+        if (tile < p.num_tiles_x * p.num_tiles_y) {
+            float3 pl[4];
+            tile_planes(cam, p, tile % p.num_tiles_x, tile / p.num_tiles_x, pl);
+            float* o = planes + 12 * (size_t)tile;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                o[3 * k] = pl[k].x;
+                o[3 * k + 1] = pl[k].y;
+                o[3 * k + 2] = pl[k].z;
+            }
+        }
+        return;
+    }
+    if (blockIdx.x == plane_blocks)
         for (uint32_t i = threadIdx.x; i < ntile_sums; i += blockDim.x) tile_sums[i] = 0ull;
-    const uint32_t li = hl_uniform(blockIdx.x * 4u + (threadIdx.x >> 6));
+    const uint32_t li = hl_uniform((blockIdx.x - plane_blocks) * 4u + (threadIdx.x >> 6));
     if (li >= p.num_lights) return;
     const float4 c = cl[li].vpos;
     const float4 cu = make_float4(hl_uniform(c.x), hl_uniform(c.y), hl_uniform(c.z), hl_uniform(c.w));
@@ -1060,13 +1080,16 @@ __global__ __launch_bounds__(256) void k_tile_bin(const GpuCameraUniforms* __res
 }
 
 // One wavefront per tile: its super-tile's candidates (ascending) through
-// the exact plane test, the first HL_MAX_LIGHTS_PER_TILE kept in order.
-__global__ __launch_bounds__(256) void k_tile_lists(const GpuCameraUniforms* __restrict__ cam, LightCullParams p,
+// the exact plane test, the first HL_MAX_LIGHTS_PER_TILE kept in order.  The
+// candidates go through 256 at a time, four per lane (lane l takes the l-th
+// of each run of 64), their loads issued together.
+__global__ __launch_bounds__(256) void k_tile_lists(const float* __restrict__ planes, LightCullParams p,
                                                     const CullLight* __restrict__ cl,
                                                     const uint32_t* __restrict__ soff,
                                                     const uint32_t* __restrict__ scnt,
                                                     const uint32_t* __restrict__ spool,
                                                     uint32_t* __restrict__ lists, uint32_t* __restrict__ counts) {
+    constexpr uint32_t kPer = 4;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
     const uint32_t total = p.num_tiles_x * p.num_tiles_y;
@@ -1074,24 +1097,35 @@ __global__ __launch_bounds__(256) void k_tile_lists(const GpuCameraUniforms* __r
     const uint32_t tile_x = tile % p.num_tiles_x, tile_y = tile / p.num_tiles_x;
     const uint32_t sup = tile_x / kSuperTile + ((p.num_tiles_x + kSuperTile - 1u) / kSuperTile) * (tile_y / kSuperTile);
     float3 pl[4];
-    tile_planes(cam, p, tile_x, tile_y, pl);
+    {
+        const float* q = planes + 12 * (size_t)tile;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            pl[k] = make_float3(hl_uniform(q[3 * k]), hl_uniform(q[3 * k + 1]), hl_uniform(q[3 * k + 2]));
+    }
     const uint32_t n = hl_uniform(scnt[sup]);
     const uint32_t* cand = spool + hl_uniform(soff[sup]);
     uint32_t count = 0;
     uint32_t* out = lists + (size_t)tile * HL_MAX_LIGHTS_PER_TILE;
-    for (uint32_t base = 0; base < n; base += 64u) {
-        const uint32_t j = base + lane;
-        bool pass = false;
-        uint32_t i = 0;
-        if (j < n) {
-            i = cand[j];
-            pass = tile_pass(pl, cl[i].vpos);
+    for (uint32_t base = 0; base < n && count < HL_MAX_LIGHTS_PER_TILE; base += 64u * kPer) {
+        uint32_t idx[kPer];
+        float4 c[kPer];
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t j = base + 64u * q + lane;
+            idx[q] = j < n ? cand[j] : 0xFFFFFFFFu;
         }
-        const uint64_t m = hl_ballot(pass);
-        const uint32_t pos = count + hl_prefix(m);
-        if (pass && pos < HL_MAX_LIGHTS_PER_TILE) out[pos] = i;
-        count += (uint32_t)__popcll(m);
-        if (count >= HL_MAX_LIGHTS_PER_TILE) break;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q)
+            c[q] = idx[q] != 0xFFFFFFFFu ? cl[idx[q]].vpos : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const bool pass = idx[q] != 0xFFFFFFFFu && tile_pass(pl, c[q]);
+            const uint64_t m = hl_ballot(pass);
+            const uint32_t pos = count + hl_prefix(m);
+            if (pass && pos < HL_MAX_LIGHTS_PER_TILE) out[pos] = idx[q];
+            count += (uint32_t)__popcll(m);
+        }
     }
     if (lane == 0) counts[tile] = count < HL_MAX_LIGHTS_PER_TILE ? count : HL_MAX_LIGHTS_PER_TILE;
 }
@@ -1129,9 +1163,10 @@ hipError_t launch_cull_tiles_binned(hipStream_t s, const GpuCameraUniforms* cam,
     const uint32_t nsup = tile_bin_supers(p);
     const uint32_t W = (p.num_lights + 31u) / 32u, SW = (W + 31u) / 32u;
     hipError_t e;
-    k_tile_bin<<<(p.num_lights + 3) / 4, 256, 0, s>>>(
+    const uint32_t plane_blocks = (total + 255u) / 256u;
+    k_tile_bin<<<plane_blocks + (p.num_lights + 3) / 4, 256, 0, s>>>(
         cam, p, cl, w.bits, w.summary, (unsigned long long*)((uint8_t*)w.raw + cluster_tile_sums_offset(nsup)),
-        cluster_scan_tiles(nsup));
+        cluster_scan_tiles(nsup), w.planes, plane_blocks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const uint32_t blocks = (nsup + 256u / kCellLanes - 1u) / (256u / kCellLanes);
     k_cluster_count<<<blocks, 256, 0, s>>>(nsup, SW, w.bits, w.summary, w.raw, 0u);
@@ -1140,7 +1175,7 @@ hipError_t launch_cull_tiles_binned(hipStream_t s, const GpuCameraUniforms* cam,
     k_cluster_fill<<<blocks, 256, 0, s>>>(nsup, SW, w.raw, w.bits, w.summary, 0u, w.pool_cap, w.offsets,
                                           w.counts, w.pool, w.flags, nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_tile_lists<<<(total + 3) / 4, 256, 0, s>>>(cam, p, cl, w.offsets, w.counts, w.pool, lists, counts);
+    k_tile_lists<<<(total + 3) / 4, 256, 0, s>>>(w.planes, p, cl, w.offsets, w.counts, w.pool, lists, counts);
     return hipGetLastError();
 }
 

@@ -81,7 +81,7 @@ hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const 
 // lists as launch_cull_tiles when the tile grid is the screen's
 // (num_tiles = ceil(size / 16)).  bits and summary: as the cluster cull's
 // (zero before, zero after), over nsup = super-tiles cells.
-constexpr uint32_t kSuperTile = 8;
+constexpr uint32_t kSuperTile = 4;
 struct TileBinScratch {
     uint32_t* bits;     // nsup * ceil(L/32) words
     uint32_t* summary;  // nsup * ceil(L/1024) words
@@ -91,6 +91,7 @@ struct TileBinScratch {
     uint32_t* pool;     // pool_cap
     uint32_t pool_cap;  // nsup * L (no truncation)
     uint32_t* flags;    // the fill's overflow word (never set: no cap, no truncation)
+    float* planes;      // 12 floats per tile: its four planes
 };
 inline uint32_t tile_bin_supers(const LightCullParams& p) {
     return ((p.num_tiles_x + kSuperTile - 1u) / kSuperTile) * ((p.num_tiles_y + kSuperTile - 1u) / kSuperTile);

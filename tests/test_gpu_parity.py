@@ -69,7 +69,7 @@ def check_tile_lists(lists, counts, ol, oc):
 
 @pytest.mark.parametrize("name", ["c1_720p_64", "c2_1080p_1k", "c3_4k_16k", "c3_4k_16k_far", "around_camera"])
 def test_tile_cull_bit_exact(ctx, name):
-    """The screen's tile grid takes the binned path (candidates per 8 x 8
+    """The screen's tile grid takes the binned path (candidates per 4 x 4
     super-tile, exact plane test per tile): the same lists as the reference's
     brute force, including lights behind and beside the camera."""
     if name == "around_camera":

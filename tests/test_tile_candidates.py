@@ -1,6 +1,6 @@
 """The binned tile cull's candidate tiles contain every member.
 
-hl_light_cull_tiles bins each light into the super-tiles (8 x 8 tiles) that
+hl_light_cull_tiles bins each light into the super-tiles (4 x 4 tiles) that
 its candidate tiles touch (light_tiles in hl_cull.hip), and each tile tests
 only its super-tile's candidates, in ascending order, with the exact plane
 test of light_cull.wgsl.  The candidates come from necessary conditions of
