@@ -169,6 +169,13 @@ def bench_4k(ctx, args):
                                             lambda: shade.execute(ds, grid, flags=abi.HL_SHADE_LIGHTS_FROM_CULL))
     res = {"scene": sc, "params": params, "pool_cap": pool_cap, "q": q, "ncell": ncell, "nlist": nlist, "counts": cnt,
            "ms_frame": total, "cull_ms": cull_ms, "shade_ms": shade_ms}
+    # the same frame into the pass's RGBA16F pre_aa target (HL_SHADE_OUT_RGBA16F)
+    shade16 = DeferredLightPass(ctx, sc.width, sc.height, out_format=abi.HL_FMT_RGBA16_FLOAT)
+    shade16.prepare(ds)
+    total16, (_, shade16_ms) = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
+                                          lambda: shade16.execute(ds, grid, flags=abi.HL_SHADE_LIGHTS_FROM_CULL))
+    res["rgba16f"] = {"ms_per_frame": total16, "shade_ms": shade16_ms, "evals_per_s": q / (total16 * 1e-3)}
+    log(f"4K into RGBA16F: {total16:.4f} ms/frame (shade {shade16_ms * 1e3:.1f} us)")
     if not args.no_batched:
         out = ctx.alloc(sc.width * sc.height * 16)
         # the plain path's cluster grid, so both legs build the same lists
@@ -487,6 +494,7 @@ def main() -> int:
                           "frac_of_scalar_fp32": tflops / FP32_SCALAR_PEAK_TFLOPS,
                           "flops_per_launch": flops},
     }
+    res["rgba16f_target_4k"] = r4k["rgba16f"]
     if "batched" in r4k:
         res["batched_4k"] = r4k["batched"]
     if far is not None:

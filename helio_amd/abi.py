@@ -23,6 +23,7 @@ HL_GRID_TILES, HL_GRID_CLUSTERS = 1, 2
 HL_MAX_SLICES = 64
 HL_SHADE_CLEAR_SKY = 1
 HL_SHADE_LIGHTS_FROM_CULL = 2
+HL_SHADE_OUT_RGBA16F = 4
 HL_OVERFLOW_CAP, HL_OVERFLOW_POOL = 1, 2
 HL_PIXEL_GRAD_PLANES = 5
 HL_BATCH_MAX_VIEWS, HL_BATCH_MAX_MULTIVIEW_PIXELS = 255, 262144

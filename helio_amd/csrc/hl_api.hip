@@ -699,7 +699,7 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
     if (!out_d) return fail(c, HL_EINVAL, "output is null");
     ShadeArgs a;
     int set;
-    if (flags & ~(uint32_t)(HL_SHADE_CLEAR_SKY | HL_SHADE_LIGHTS_FROM_CULL))
+    if (flags & ~(uint32_t)(HL_SHADE_CLEAR_SKY | HL_SHADE_LIGHTS_FROM_CULL | HL_SHADE_OUT_RGBA16F))
         return fail(c, HL_EINVAL, "unknown shading flags 0x%x", flags);
     if (sh) {
         if (sh->atlas_size < 2 || sh->atlas_size > 16384)
@@ -726,11 +726,15 @@ hl_status run_forward(hl_context* c, const GpuCameraUniforms* camera_d, const De
     hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, num_lights, grid, &a, &set, false,
                                 (flags & HL_SHADE_LIGHTS_FROM_CULL) != 0);
     if (s != HL_OK) return s;
-    if (out_pitch_bytes < 16u * gb->width || out_pitch_bytes % 16 != 0 || ((uintptr_t)out_d) % 16)
-        return fail(c, HL_EINVAL, "output pitch %u invalid for width %u (RGBA32F, 16-B aligned)",
-                    out_pitch_bytes, gb->width);
+    const bool f16 = (flags & HL_SHADE_OUT_RGBA16F) != 0u;
+    const uint32_t px = f16 ? 8u : 16u;
+    if ((uint64_t)out_pitch_bytes < (uint64_t)px * gb->width || out_pitch_bytes % px != 0 ||
+        ((uintptr_t)out_d) % px)
+        return fail(c, HL_EINVAL, "output pitch %u invalid for width %u (%s, %u-B aligned)", out_pitch_bytes,
+                    gb->width, f16 ? "RGBA16F" : "RGBA32F", px);
     a.out = out_d;
     a.out_pitch = out_pitch_bytes;
+    a.out_f16 = f16 ? 1u : 0u;
     a.flags = flags;
     int mode = kModeForward;
     if (use_rc) {

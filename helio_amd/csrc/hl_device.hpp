@@ -185,7 +185,7 @@ struct ShadeArgs {
     uint32_t block_tiles_x;  // block tiles per row
     uint32_t num_blocks;     // block tiles of the view
     uint32_t tw_log2;        // wavefront tile: (1 << tw_log2) x (64 >> tw_log2) pixels
-    uint32_t _pad0;
+    uint32_t out_f16;        // the output is RGBA16F (8 B per pixel, IEEE round to nearest even)
     ShadowView shadow;       // kModeExtended only (layers 0: no shadows)
     // kModeExtended: radiance-cascade probe atlas (rc_fmt 0: no RC GI) and
     // its world box (min, max - min)
@@ -432,6 +432,21 @@ __device__ __forceinline__ hl_gptr<T> hl_g(T* p) {
 __device__ __forceinline__ void hl_store_nt(float4* p, float4 v) {
     hl_f32x4 w = {v.x, v.y, v.z, v.w};
     __builtin_nontemporal_store(w, hl_g(reinterpret_cast<hl_f32x4*>(p)));
+}
+
+// One output pixel in the target's format: RGBA32F, or RGBA16F with each
+// channel rounded to the nearest binary16 (ties to even, overflow to inf).
+__device__ __forceinline__ void hl_store_px(uint8_t* row, uint32_t x, bool f16, float4 v) {
+    if (f16) {
+        const _Float16 h0 = (_Float16)v.x, h1 = (_Float16)v.y, h2 = (_Float16)v.z, h3 = (_Float16)v.w;
+        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 u;
+        u.x = (uint32_t)__builtin_bit_cast(uint16_t, h0) | ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+        u.y = (uint32_t)__builtin_bit_cast(uint16_t, h2) | ((uint32_t)__builtin_bit_cast(uint16_t, h3) << 16);
+        __builtin_nontemporal_store(u, hl_g(reinterpret_cast<u32x2*>(row) + x));
+    } else {
+        hl_store_nt(reinterpret_cast<float4*>(row) + x, v);
+    }
 }
 
 __device__ __forceinline__ float4 hl_ld4(const float4* p) {

@@ -167,14 +167,13 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     constexpr bool kExt = MODE == kModeShadowed || kRC;  // exact N, shadow_factor
     constexpr bool kFwd = MODE == kModeForward || kExt;
     constexpr bool kWrites = kFwd || MODE == kModeFusedL2;
-    float4* out_px = (kWrites && inside)
-                         ? (float4*)(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x
-                         : nullptr;
+    uint8_t* out_row = (kWrites && inside) ? reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch : nullptr;
+    const bool out_f16 = a.out_f16 != 0u;
     const size_t plane = (size_t)a.width * a.height;
     const size_t pix = (size_t)y * a.width + x;
     if (inside && !valid && lead) {
         if (kWrites && (a.flags & HL_SHADE_CLEAR_SKY))
-            hl_store_nt(out_px, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            hl_store_px(out_row, x, out_f16, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         if (kAdj)
             for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) hl_g(a.d_pixel)[k * plane + pix] = 0.0f;
     }
@@ -313,7 +312,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
                 o = make_float4(p.nx * 0.5f + 0.5f, p.ny * 0.5f + 0.5f, p.nz * 0.5f + 0.5f, 1.0f);
             else
                 o = A;
-            hl_store_nt(out_px, o);
+            hl_store_px(out_row, x, out_f16, o);
         }
         return;
     }
@@ -396,7 +395,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = lo[c] + amb[c];
     if (kFwd) {
-        if (valid) hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+        if (valid) hl_store_px(out_row, x, out_f16, make_float4(o[0], o[1], o[2], A.w));
         return;
     }
 
@@ -405,7 +404,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     if (MODE == kModeFusedL2) {
         float l2 = 0.0f;
         if (valid) {
-            if (lead) hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+            if (lead) hl_store_px(out_row, x, out_f16, make_float4(o[0], o[1], o[2], A.w));
             const float4 tg = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
             const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
             gl[0] = 2.0f * e0;

@@ -317,10 +317,17 @@ class LightCullPass:
 
 
 class DeferredLightPass:
-    def __init__(self, ctx: Context, width: int, height: int):
+    """out_format: the pre_aa target's format, RGBA32F (default) or RGBA16F
+    (HL_SHADE_OUT_RGBA16F; forward shading only)."""
+
+    def __init__(self, ctx: Context, width: int, height: int, out_format: int = abi.HL_FMT_RGBA32_FLOAT):
+        if out_format not in (abi.HL_FMT_RGBA32_FLOAT, abi.HL_FMT_RGBA16_FLOAT):
+            raise ValueError("pre_aa must be RGBA32F or RGBA16F")
         self.ctx = ctx
         self.width, self.height = width, height
-        self.out = ctx.alloc(width * height * 16)
+        self.out_f16 = out_format == abi.HL_FMT_RGBA16_FLOAT
+        self.pitch = width * (8 if self.out_f16 else 16)
+        self.out = ctx.alloc(self.pitch * height)
         self.debug_mode = 0
         self.globals = None
 
@@ -339,24 +346,28 @@ class DeferredLightPass:
 
     def execute(self, scene: DeviceScene, grid: abi.hl_light_grid, flags: int = 0,
                 shadows: "DeviceShadows | None" = None, rc: "DeviceTexture | None" = None) -> None:
+        if self.out_f16:
+            flags |= abi.HL_SHADE_OUT_RGBA16F
         if rc is not None:
             self.ctx.lib.call("hl_deferred_forward_ex", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
                               C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
                               C.byref(shadows.desc) if shadows is not None else None, C.byref(rc.desc),
-                              self.out.ptr, self.width * 16, flags)
+                              self.out.ptr, self.pitch, flags)
         elif shadows is None:
             self.ctx.lib.call("hl_deferred_forward", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
                               C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
-                              self.out.ptr, self.width * 16, flags)
+                              self.out.ptr, self.pitch, flags)
         else:
             self.ctx.lib.call("hl_deferred_forward_shadowed", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
                               C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid),
-                              C.byref(shadows.desc), self.out.ptr, self.width * 16, flags)
+                              C.byref(shadows.desc), self.out.ptr, self.pitch, flags)
 
     def forward_backward_l2(self, scene: DeviceScene, grid: abi.hl_light_grid, target: DeviceArray,
                             grads: "AdjointBuffers") -> None:
         """Forward into self.out, then the L2 loss against target and its
         gradients (hl_deferred_forward_backward_l2)."""
+        if self.out_f16:
+            raise ValueError("the fused adjoint writes an RGBA32F output")
         self.ctx.lib.call("hl_deferred_forward_backward_l2", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
                           C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid), target.ptr,
                           self.width * 16, self.out.ptr, self.width * 16, grads.loss.ptr, grads.d_pixel.ptr,
@@ -376,6 +387,9 @@ class DeferredLightPass:
         return n.value
 
     def output(self) -> np.ndarray:
+        """The target as float32 (RGBA16F values widened exactly)."""
+        if self.out_f16:
+            return self.out.download(np.float16, (self.height, self.width, 4)).astype(np.float32)
         return self.out.download(np.float32, (self.height, self.width, 4))
 
 

@@ -246,11 +246,16 @@ HL_API hl_status hl_gbuffer_pack(hl_context* ctx, const hl_surface* surface, con
 /* ---------------------------------------------------------------- shading */
 enum {
     HL_SHADE_CLEAR_SKY = 1u,        /* write (0,0,0,0) to sky pixels (LoadOp::Clear); default leaves them */
-    HL_SHADE_LIGHTS_FROM_CULL = 2u  /* use the light records that this context's last light cull built
+    HL_SHADE_LIGHTS_FROM_CULL = 2u, /* use the light records that this context's last light cull built
                                        (hl_light_cull_clusters / hl_light_cull_tiles over the same lights_d
                                        and num_lights, the buffer unchanged since in stream order) instead
                                        of building them again; HL_EINVAL when the last cull was over other
                                        lights.  The records depend on the lights only, not the camera. */
+    HL_SHADE_OUT_RGBA16F = 4u       /* out_d is an RGBA16F target (the pass's pre_aa format,
+                                       helio-pass-deferred-light/src/lib.rs:423-427) instead of RGBA32F:
+                                       each channel of the FP32 result rounded to the nearest binary16
+                                       (ties to even, overflow to inf); row pitch >= 8 * width, 8-byte
+                                       aligned.  hl_deferred_forward, _shadowed and _ex only. */
 };
 
 /* DeferredLight fs_main over the G-buffer (deferred_lighting.wgsl:991-1259 with
@@ -261,8 +266,9 @@ enum {
  * globals: DeferredGlobals as written by DeferredLightPass::prepare (lib.rs:836-878);
  *   ambient_color/ambient_intensity/debug_mode/num_tiles_x are read.  Debug modes
  *   0, 1, 2, 4, 5, 20, 21 are supported; others return HL_EUNSUPPORTED.
- * out_d: RGBA32F target (pre_aa), row pitch out_pitch_bytes.  Sky pixels
- *   (depth >= 1) are left untouched unless flags has HL_SHADE_CLEAR_SKY.
+ * out_d: RGBA32F target (pre_aa), row pitch out_pitch_bytes, or RGBA16F with
+ *   HL_SHADE_OUT_RGBA16F.  Sky pixels (depth >= 1) are left untouched unless
+ *   flags has HL_SHADE_CLEAR_SKY.
  * Lights with shadow_index != HL_NO_SHADOW or IES/gobo indices >= 0 are shaded
  *   as unshadowed, untextured lights (those inputs are outside the hot path).
  * Radiance is within 1e-4 relative of the FP32 restatement evaluating the same

@@ -309,3 +309,29 @@ def test_unsupported_debug_view_is_rejected(ctx):
     with pytest.raises(abi.HelioError) as e:
         shade.execute(ds, cull.publish())
     assert e.value.status == abi.HL_EUNSUPPORTED
+
+
+@pytest.mark.parametrize("name,shadowed", [("c3_4k_16k", False), ("c2_1080p_1k", True)])
+def test_rgba16f_target(ctx, hlib, name, shadowed):
+    """HL_SHADE_OUT_RGBA16F (the pass's pre_aa format): every pixel of the
+    RGBA16F target is the same call's RGBA32F result rounded to the nearest
+    binary16 (ties to even), bit for bit, cleared sky included."""
+    from helio_amd.device import DeviceShadows
+    print(f"[rgba16f] {name} shadowed={shadowed}", flush=True)
+    sc = synth.make_scene(name)
+    sh = synth.add_shadow_casters(sc, n_point=4, n_spot=4, size=256) if shadowed else None
+    params = synth.cluster_params(hlib, sc)
+    ds, cull, off, cnt, pool, flags = hip_clusters(ctx, sc, params, 1 << 25)
+    dsh = DeviceShadows(ctx, sh) if shadowed else None
+    outs = []
+    for fmt in (abi.HL_FMT_RGBA32_FLOAT, abi.HL_FMT_RGBA16_FLOAT):
+        shade = DeferredLightPass(ctx, sc.width, sc.height, out_format=fmt)
+        shade.prepare(ds)
+        shade.out.fill_u32(0x7FC00000)  # a NaN pattern where nothing is written
+        shade.execute(ds, cull.publish(), flags=abi.HL_SHADE_CLEAR_SKY, shadows=dsh)
+        outs.append(shade.out.download(np.float32 if fmt == abi.HL_FMT_RGBA32_FLOAT else np.float16,
+                                       (sc.height, sc.width, 4)))
+    with np.errstate(over="ignore"):
+        want = outs[0].astype(np.float16)
+    assert np.array_equal(outs[1].view(np.uint16), want.view(np.uint16))
+    assert np.all(outs[1][sc.depth >= 1.0] == 0)
