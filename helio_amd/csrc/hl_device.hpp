@@ -436,16 +436,16 @@ __device__ __forceinline__ void hl_store_nt(float4* p, float4 v) {
 
 // One output pixel in the target's format: RGBA32F, or RGBA16F with each
 // channel rounded to the nearest binary16 (ties to even, overflow to inf).
-__device__ __forceinline__ void hl_store_px(uint8_t* row, uint32_t x, bool f16, float4 v) {
+__device__ __forceinline__ void hl_store_px(uint8_t* px, bool f16, float4 v) {
     if (f16) {
         const _Float16 h0 = (_Float16)v.x, h1 = (_Float16)v.y, h2 = (_Float16)v.z, h3 = (_Float16)v.w;
         typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
         u32x2 u;
         u.x = (uint32_t)__builtin_bit_cast(uint16_t, h0) | ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
         u.y = (uint32_t)__builtin_bit_cast(uint16_t, h2) | ((uint32_t)__builtin_bit_cast(uint16_t, h3) << 16);
-        __builtin_nontemporal_store(u, hl_g(reinterpret_cast<u32x2*>(row) + x));
+        __builtin_nontemporal_store(u, hl_g(reinterpret_cast<u32x2*>(px)));
     } else {
-        hl_store_nt(reinterpret_cast<float4*>(row) + x, v);
+        hl_store_nt(reinterpret_cast<float4*>(px), v);
     }
 }
 

@@ -209,7 +209,10 @@ enum ShadeMode : int {
     kModeBackward = 2,
     kModeFusedL2 = 3,
     kModeShadowed = 4,
-    kModeExtended = 5
+    kModeExtended = 5,
+    // flag on the forward-writing modes (forward, shadowed, extended): the
+    // output target is RGBA16F (ShadeArgs::out_f16) instead of RGBA32F
+    kModeOutF16 = 8
 };
 // Work items of a shading launch over num_blocks block tiles: the adjoint
 // modes split each wave tile's lights over a block's four waves, so their

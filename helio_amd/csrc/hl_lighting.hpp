@@ -251,7 +251,7 @@ struct KeptState {
 // the set bits of the ballot (ascending lanes, i.e. list order) with uniform
 // reads of those slots.  a provides the grid (lists, counts, offsets,
 // lists_capacity) and the light records (lights, num_lights).
-template <int GRID, class A, class F>
+template <int GRID, bool kKeep = false, class A, class F>
 __device__ __forceinline__ void walk_lights(const A& a, HotLight* s_l, const Pixel& p, bool valid,
                                             uint32_t cell, F&& f, KeptLights* kept = nullptr,
                                             KeptState* ks = nullptr) {
@@ -276,46 +276,57 @@ __device__ __forceinline__ void walk_lights(const A& a, HotLight* s_l, const Pix
             const uint32_t j = j0 + lane;
             bool pass = false;
             uint32_t idx = 0;
-            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), l1 = q, l2 = q, l3 = q, l4 = q;
+            [[maybe_unused]] float4 q, l1, l2, l3, l4;  // kKeep: staged after the ballot
             if (j < cnt) {
                 idx = hl_g(list)[j];
                 if (idx < a.num_lights) {
                     // the whole record at once, so that staging it does not
                     // wait on a second load after the test
                     const HotLight* hp = a.lights + idx;
-                    q = hl_ld4(&hp->p0);
-                    l1 = hl_ld4(&hp->p1);
-                    l2 = hl_ld4(&hp->p2);
-                    l3 = hl_ld4(&hp->p3);
-                    l4 = hl_ld4(&hp->p4);
-                    const float dx = fmaxf(fmaxf(bx0 - q.x, q.x - bx1), 0.0f);
-                    const float dy = fmaxf(fmaxf(by0 - q.y, q.y - by1), 0.0f);
-                    const float dz = fmaxf(fmaxf(bz0 - q.z, q.z - bz1), 0.0f);
+                    const float4 r0 = hl_ld4(&hp->p0), r1 = hl_ld4(&hp->p1), r2 = hl_ld4(&hp->p2),
+                                 r3 = hl_ld4(&hp->p3), r4 = hl_ld4(&hp->p4);
+                    const float dx = fmaxf(fmaxf(bx0 - r0.x, r0.x - bx1), 0.0f);
+                    const float dy = fmaxf(fmaxf(by0 - r0.y, r0.y - by1), 0.0f);
+                    const float dz = fmaxf(fmaxf(bz0 - r0.z, r0.z - bz1), 0.0f);
                     const float d2 = dx * dx + dy * dy + dz * dz;
-                    pass = d2 <= q.w * q.w * 1.0001f + 1e-12f;
+                    pass = d2 <= r0.w * r0.w * 1.0001f + 1e-12f;
+                    if constexpr (kKeep) {
+                        q = r0;
+                        l1 = r1;
+                        l2 = r2;
+                        l3 = r3;
+                        l4 = r4;
+                    } else if (pass) {  // the lane's own slot, stored while the rest are tested
+                        s_l[lane].p0 = r0;
+                        s_l[lane].p1 = r1;
+                        s_l[lane].p2 = r2;
+                        s_l[lane].p3 = r3;
+                        s_l[lane].p4 = r4;
+                    }
                 }
             }
             uint64_t m = hl_ballot(pass);
-            // slots: the lane's own, or the kept range's next ones in lane order
+            // kKeep: the kept range's next slots in lane order while they fit,
+            // else the lane's own
             bool keep = false;
             uint32_t first = 0;
-            if (kept) {
+            if constexpr (kKeep) {
                 const uint32_t np = (uint32_t)__popcll(m);
                 if (ks->ok && ks->n + np > (uint32_t)kStage) ks->ok = false;
                 keep = ks->ok;
                 first = ks->n;
                 if (keep) ks->n += np;
-            }
-            if (pass) {
-                const uint32_t slot = keep ? first + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : lane;
-                s_l[slot].p0 = q;
-                s_l[slot].p1 = l1;
-                s_l[slot].p2 = l2;
-                s_l[slot].p3 = l3;
-                s_l[slot].p4 = l4;
-                if (keep) {
-                    kept->idx[slot] = idx;
-                    kept->lanes[slot] = group;
+                if (pass) {
+                    const uint32_t slot = keep ? first + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : lane;
+                    s_l[slot].p0 = q;
+                    s_l[slot].p1 = l1;
+                    s_l[slot].p2 = l2;
+                    s_l[slot].p3 = l3;
+                    s_l[slot].p4 = l4;
+                    if (keep) {
+                        kept->idx[slot] = idx;
+                        kept->lanes[slot] = group;
+                    }
                 }
             }
             hl_wave_sync();

@@ -122,9 +122,13 @@ struct ViewConst {
 
 struct ShadeSmem {
     HotLight light[4][kStage];
-    KeptLights kept[4];    // the fused adjoint's second pass
-    float red[4][5][64];   // the adjoint's per-wave partial sums of a pixel tile
     ViewConst vc;
+};
+// LDS of the adjoint modes only (declared by their kernels alone, so that the
+// forward's occupancy is not charged for it)
+struct AdjSmem {
+    KeptLights kept[4];   // the fused adjoint's second pass
+    float red[4][5][64];  // the per-wave partial sums of a pixel tile
 };
 
 // Fills sm.vc from the view's arguments; the caller synchronises the block
@@ -139,8 +143,10 @@ __device__ __forceinline__ void load_view_const(const ShadeArgs& a, ShadeSmem& s
     else if (t >= 64u && t <= 64u + HL_MAX_SLICES) sm.vc.split[t - 64u] = a.depth_split[t - 64u];
 }
 
-template <int FA, int FN, int GRID, int MODE>
-__device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm) {
+// F16: the output target is RGBA16F (a compile-time choice, so that the
+// RGBA32F kernels hold no code of the other format)
+template <int FA, int FN, int GRID, int MODE, bool F16 = false>
+__device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem* as) {
     const ViewConst& vc = sm.vc;
     b = hl_uniform(b);  // the item is the block's: its tile arithmetic stays scalar
     const uint32_t lane = threadIdx.x & 63u, wave = hl_uniform(threadIdx.x >> 6);
@@ -167,13 +173,16 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     constexpr bool kExt = MODE == kModeShadowed || kRC;  // exact N, shadow_factor
     constexpr bool kFwd = MODE == kModeForward || kExt;
     constexpr bool kWrites = kFwd || MODE == kModeFusedL2;
-    uint8_t* out_row = (kWrites && inside) ? reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch : nullptr;
-    const bool out_f16 = a.out_f16 != 0u;
+    // the pixel's address, computed once (8 B per pixel for RGBA16F, else 16)
+    constexpr bool out_f16 = F16;
+    uint8_t* out_px = (kWrites && inside) ? reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch +
+                                                (size_t)x * (out_f16 ? 8u : 16u)
+                                          : nullptr;
     const size_t plane = (size_t)a.width * a.height;
     const size_t pix = (size_t)y * a.width + x;
     if (inside && !valid && lead) {
         if (kWrites && (a.flags & HL_SHADE_CLEAR_SKY))
-            hl_store_px(out_row, x, out_f16, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            hl_store_px(out_px, out_f16, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         if (kAdj)
             for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) hl_g(a.d_pixel)[k * plane + pix] = 0.0f;
     }
@@ -312,7 +321,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
                 o = make_float4(p.nx * 0.5f + 0.5f, p.ny * 0.5f + 0.5f, p.nz * 0.5f + 0.5f, 1.0f);
             else
                 o = A;
-            hl_store_px(out_row, x, out_f16, o);
+            hl_store_px(out_px, out_f16, o);
         }
         return;
     }
@@ -322,7 +331,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     // the fused adjoint revisits the forward walk's lights from LDS when they
     // fit (kStage per wave), else walks the lists again
     KeptState ks;
-    KeptLights* kept = MODE == kModeFusedL2 ? &sm.kept[wave] : nullptr;
+    KeptLights* kept = MODE == kModeFusedL2 ? &as->kept[wave] : nullptr;
 
     if (MODE == kModeCount) {
         uint32_t pairs = 0;
@@ -372,7 +381,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         }
     }
     if (MODE != kModeBackward) {
-        walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
+        walk_lights<GRID, MODE == kModeFusedL2>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
             if (!mine) return;
             Term t;
             if (eval_term<false>(L, p, t)) {
@@ -395,7 +404,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = lo[c] + amb[c];
     if (kFwd) {
-        if (valid) hl_store_px(out_row, x, out_f16, make_float4(o[0], o[1], o[2], A.w));
+        if (valid) hl_store_px(out_px, out_f16, make_float4(o[0], o[1], o[2], A.w));
         return;
     }
 
@@ -404,7 +413,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     if (MODE == kModeFusedL2) {
         float l2 = 0.0f;
         if (valid) {
-            if (lead) hl_store_px(out_row, x, out_f16, make_float4(o[0], o[1], o[2], A.w));
+            if (lead) hl_store_px(out_px, out_f16, make_float4(o[0], o[1], o[2], A.w));
             const float4 tg = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
             const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
             gl[0] = 2.0f * e0;
@@ -456,17 +465,17 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     }
     if (kSplit) {
         // the waves' partial per-pixel gradients, added in wave order
-        sm.red[wave][0][lane] = dA[0];
-        sm.red[wave][1][lane] = dA[1];
-        sm.red[wave][2][lane] = dA[2];
-        sm.red[wave][3][lane] = dR;
-        sm.red[wave][4][lane] = dM;
+        as->red[wave][0][lane] = dA[0];
+        as->red[wave][1][lane] = dA[1];
+        as->red[wave][2][lane] = dA[2];
+        as->red[wave][3][lane] = dR;
+        as->red[wave][4][lane] = dM;
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            dA[c] = ((sm.red[0][c][lane] + sm.red[1][c][lane]) + sm.red[2][c][lane]) + sm.red[3][c][lane];
-        dR = ((sm.red[0][3][lane] + sm.red[1][3][lane]) + sm.red[2][3][lane]) + sm.red[3][3][lane];
-        dM = ((sm.red[0][4][lane] + sm.red[1][4][lane]) + sm.red[2][4][lane]) + sm.red[3][4][lane];
+            dA[c] = ((as->red[0][c][lane] + as->red[1][c][lane]) + as->red[2][c][lane]) + as->red[3][c][lane];
+        dR = ((as->red[0][3][lane] + as->red[1][3][lane]) + as->red[2][3][lane]) + as->red[3][3][lane];
+        dM = ((as->red[0][4][lane] + as->red[1][4][lane]) + as->red[2][4][lane]) + as->red[3][4][lane];
         __syncthreads();
     }
     if (valid && lead) {
@@ -556,14 +565,23 @@ constexpr int shade_min_blocks() {
                : (MODE == kModeForward || MODE == kModeShadowed || MODE == kModeExtended ? 4 : 1);
 }
 
+// MODE: a ShadeMode, with kModeOutF16 set for an RGBA16F target
 template <int FA, int FN, int GRID, int MODE>
-__global__ __launch_bounds__(256, (shade_min_blocks<MODE, FA>())) void k_shade(ShadeArgs a, uint32_t* __restrict__ queue,
-                                               uint32_t* __restrict__ queue_next) {
+__global__ __launch_bounds__(256, (shade_min_blocks<(MODE & ~kModeOutF16), FA>())) void k_shade(
+    ShadeArgs a, uint32_t* __restrict__ queue, uint32_t* __restrict__ queue_next) {
+    constexpr int M = MODE & ~kModeOutF16;
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
+    AdjSmem* as = nullptr;
+    if constexpr (M == kModeFusedL2 || M == kModeBackward) {
+        __shared__ AdjSmem adj;
+        as = &adj;
+    }
     if (blockIdx.x == 0 && threadIdx.x < kShadeQueues) queue_next[threadIdx.x * kShadeQueueStride] = 0u;
     load_view_const(a, sm);  // published by the first claim's barrier
-    claim_items(shade_items(MODE, a.num_blocks), queue, cs, [&](uint32_t t) { shade_tile<FA, FN, GRID, MODE>(a, t, sm); });
+    claim_items(shade_items(M, a.num_blocks), queue, cs, [&](uint32_t t) {
+        shade_tile<FA, FN, GRID, M, (MODE & kModeOutF16) != 0>(a, t, sm, as);
+    });
 }
 
 // Many views, one persistent launch: the items are the block tiles of the
@@ -584,6 +602,11 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_bat
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
     __shared__ ShadeArgs sa;
+    AdjSmem* as = nullptr;
+    if constexpr (MODE == kModeFusedL2) {
+        __shared__ AdjSmem adj;
+        as = &adj;
+    }
     uint32_t first = 1u, end = 0u;  // item range of the staged view (none yet)
     claim_items(num_items, queue, cs, [&](uint32_t it) {
         it = hl_uniform(it);  // block-uniform: every thread read the same claim
@@ -609,7 +632,7 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_bat
 #pragma unroll
         for (uint32_t i = 0; i < sizeof(ShadeArgs) / 4; ++i)
             ((uint32_t*)&ra)[i] = __builtin_amdgcn_readfirstlane(((const uint32_t*)&sa)[i]);
-        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(ra, it - first, sm);
+        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(ra, it - first, sm, as);
     });
 }
 
@@ -665,11 +688,13 @@ template <int FA, int FN, int MODE>
 static hipError_t launch_grid(hipStream_t s, const ShadeArgs& a, uint32_t* queue, uint32_t* queue_next) {
     if (a.grid_mode == HL_GRID_TILES)
         k_shade<FA, FN, HL_GRID_TILES, MODE>
-            <<<persistent_blocks<FA, FN, HL_GRID_TILES, MODE>(shade_items(MODE, a.num_blocks)), 256, 0, s>>>(
+            <<<persistent_blocks<FA, FN, HL_GRID_TILES, MODE>(shade_items(MODE & ~kModeOutF16, a.num_blocks)), 256,
+               0, s>>>(
                 a, queue, queue_next);
     else
         k_shade<FA, FN, HL_GRID_CLUSTERS, MODE>
-            <<<persistent_blocks<FA, FN, HL_GRID_CLUSTERS, MODE>(shade_items(MODE, a.num_blocks)), 256, 0, s>>>(
+            <<<persistent_blocks<FA, FN, HL_GRID_CLUSTERS, MODE>(shade_items(MODE & ~kModeOutF16, a.num_blocks)),
+               256, 0, s>>>(
                 a, queue, queue_next);
     return hipGetLastError();
 }
@@ -686,6 +711,14 @@ static hipError_t launch_fmt(hipStream_t s, int set, const ShadeArgs& a, uint32_
 hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& a, uint32_t* queue,
                         uint32_t* queue_next) {
     if (a.num_blocks == 0) return hipSuccess;
+    if (a.out_f16) {  // the modes that write the radiance alone take an RGBA16F target
+        switch (mode) {
+            case kModeForward: return launch_fmt<kModeForward | kModeOutF16>(s, gbuf_set, a, queue, queue_next);
+            case kModeShadowed: return launch_fmt<kModeShadowed | kModeOutF16>(s, gbuf_set, a, queue, queue_next);
+            case kModeExtended: return launch_fmt<kModeExtended | kModeOutF16>(s, gbuf_set, a, queue, queue_next);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (mode) {
         case kModeForward: return launch_fmt<kModeForward>(s, gbuf_set, a, queue, queue_next);
         case kModeCount: return launch_fmt<kModeCount>(s, gbuf_set, a, queue, queue_next);
