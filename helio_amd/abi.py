@@ -29,6 +29,7 @@ HL_PIXEL_GRAD_PLANES = 5
 HL_BATCH_MAX_VIEWS, HL_BATCH_MAX_MULTIVIEW_PIXELS = 255, 262144
 HL_NOALLOC_MAX_LIGHTS = 65536
 HL_NOALLOC_SCRATCH_BYTES = 16384 + 80 * HL_NOALLOC_MAX_LIGHTS
+HL_EXTERNAL_FD, HL_EXTERNAL_MAP_ALIGN = 1, 256
 
 # GpuLight (crates/libhelio/src/light.rs:60-106), 128 bytes.
 GPU_LIGHT_DTYPE = np.dtype([
@@ -270,13 +271,17 @@ SIGNATURES: dict[str, tuple] = {
     "hl_batch_launch": (None, [vp]),
     "hl_batch_get_config": (None, [vp, u32, P(hl_view_config)]),
     "hl_plan_destroy": (None, [vp]),
+    "hl_external_import": (None, [vp, C.c_int, C.c_uint64, u32, P(vp)]),
+    "hl_external_map": (None, [vp, C.c_uint64, C.c_uint64, P(vp)]),
+    "hl_external_release": (None, [vp]),
 }
 
 # Entry points that take the context first (for error messages).
 _CTX_FIRST = {n for n, (_, a) in SIGNATURES.items() if a and a[0] is vp and n.startswith("hl_")
               and n not in ("hl_context_destroy", "hl_get_stream", "hl_last_error",
                             "hl_event_destroy", "hl_batch_capture", "hl_batch_launch",
-                            "hl_batch_get_config", "hl_plan_destroy")}
+                            "hl_batch_get_config", "hl_plan_destroy", "hl_external_map",
+                            "hl_external_release")}
 
 
 def library_path() -> Path:

@@ -22,7 +22,7 @@ CSRC = PKG / "csrc"
 ORACLE = ROOT / "oracle"
 
 HIP_SOURCES = ["hl_api.hip", "hl_cull.hip", "hl_shade.hip", "hl_batch.hip", "hl_gbuffer.hip",
-               "hl_forward.hip", "hl_reflect.hip"]
+               "hl_forward.hip", "hl_reflect.hip", "hl_interop.hip"]
 ARCH = os.environ.get("HELIO_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -386,6 +386,7 @@ HL_API hl_status hl_context_destroy(hl_context* c) {
     if (!c) return HL_EINVAL;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    while (!c->externals.empty()) (void)release_external(c->externals.back());
     for (DevBuf* b : {&c->hot, &c->cull, &c->geom, &c->bits, &c->summary, &c->raw, &c->markq, &c->shadeq,
                       &c->words, &c->accum, &c->vogel, &c->rcirr, &c->tilebin})
         if (b->p) (void)hipFree(b->p);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "hl_device.hpp"
 
@@ -29,6 +30,8 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+struct hl_external;
+
 struct hl_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -42,13 +45,20 @@ struct hl_context {
     // the lights the records in `hot` were last built from (stream order)
     const GpuLight* hot_src = nullptr;
     uint32_t hot_n = 0;
+    // imported memory not yet released (hl_external_import); released with
+    // the context
+    std::vector<hl_external*> externals;
     // words: [0] overflow flags, [1] feature flags, [2..3] pair counter (u64)
 };
+
 
 namespace hl {
 
 hl_status fail(hl_context* c, hl_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 hl_status hip_fail(hl_context* c, hipError_t e, const char* what);
+// Releases an imported allocation (hl_interop.hip): destroys the import and
+// closes the library's descriptor.  The caller has synchronised the stream.
+hl_status release_external(hl_external* e);
 // Grows a scratch buffer (never shrinks).  zero: a new allocation is cleared
 // (for buffers the kernels keep zero between calls).
 hl_status ensure(hl_context* c, DevBuf& b, size_t bytes, bool zero = false);

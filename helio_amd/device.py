@@ -116,6 +116,36 @@ class Context:
         return ms.value
 
 
+class ExternalMemory:
+    """Memory of another API on this GPU, imported from a file descriptor
+    (hl_external_import); ``map`` gives device pointers into it for the *_d
+    arguments (the reference's frame slots are wgpu buffers,
+    crates/libhelio/src/frame.rs:228-234).  The caller keeps its descriptor."""
+
+    def __init__(self, ctx: Context, fd: int, size: int):
+        self.ctx = ctx
+        self.size = int(size)
+        h = C.c_void_p()
+        ctx.lib.call("hl_external_import", ctx.h, int(fd), self.size, abi.HL_EXTERNAL_FD, C.byref(h))
+        self.h = h.value
+
+    def map(self, offset: int, size: int) -> int:
+        p = C.c_void_p()
+        self.ctx.lib.call("hl_external_map", self.h, int(offset), int(size), C.byref(p), ctx=self.ctx.h)
+        return p.value
+
+    def release(self) -> None:
+        if self.h:
+            h, self.h = self.h, None
+            self.ctx.lib.call("hl_external_release", h, ctx=self.ctx.h)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.release()
+
+
 @dataclass
 class DeviceScene:
     """A HostScene resident in HBM."""

@@ -72,6 +72,37 @@ HL_API hl_status hl_event_destroy(hl_event* ev);
 HL_API hl_status hl_event_record(hl_context* ctx, hl_event* ev);
 HL_API hl_status hl_event_elapsed_ms(hl_event* start, hl_event* end, float* ms);
 
+/* ---------------------------------------------------------------- interop */
+/* Memory of another API on the same GPU as device pointers for the *_d
+ * arguments.  Replaces the wgpu resources the reference's passes bind: the
+ * frame slots tile_light_lists, tile_light_counts and pre_aa
+ * (crates/libhelio/src/frame.rs:228-234), handed over at
+ * RenderGraph::replace_pass_at (crates/helio-core/src/graph/execution.rs:193).
+ * A Vulkan VkDeviceMemory allocated exportable gives the descriptor with
+ * vkGetMemoryFdKHR (VK_EXTERNAL_MEMORY_HANDLE_TYPE_OPAQUE_FD_BIT or
+ * _DMA_BUF_BIT_EXT); a HIP allocation made with hipMemCreate and exported with
+ * hipMemExportToShareableHandle (POSIX descriptor) gives one too.
+ *  - hl_external_import: the caller keeps fd (the library imports and later
+ *    closes a duplicate).  size: bytes to import, at most the descriptor's
+ *    (a dma-buf's size is checked; a descriptor without one is refused).
+ *  - hl_external_map: [offset, offset + size) of the import as a device
+ *    pointer, valid until hl_external_release; offset a multiple of
+ *    HL_EXTERNAL_MAP_ALIGN.  Linear memory only: an optimally tiled image is
+ *    copied to or from an exported buffer on the other API's queue.
+ *  - hl_external_release: waits for the context stream, then releases the
+ *    import and every pointer mapped from it.  hl_context_destroy releases
+ *    what is left.
+ *  - Ordering between the two APIs is the caller's: wait for the other
+ *    API's writes (its fence) before the calls that read the memory, and for
+ *    the context stream (hl_synchronize) before the other API reads what the
+ *    calls wrote.  HIP's external semaphores are not available on Linux. */
+typedef struct hl_external hl_external;
+#define HL_EXTERNAL_FD 1u            /* an opaque-fd or dma-buf file descriptor */
+#define HL_EXTERNAL_MAP_ALIGN 256u   /* wgpu's minimum storage-buffer offset alignment */
+HL_API hl_status hl_external_import(hl_context* ctx, int fd, uint64_t size, uint32_t kind, hl_external** out);
+HL_API hl_status hl_external_map(hl_external* ext, uint64_t offset, uint64_t size, void** out_d);
+HL_API hl_status hl_external_release(hl_external* ext);
+
 /* --------------------------------------------------------------- textures */
 /* Pixel formats the G-buffer attachments may use.  The reference binds
  * albedo/orm as Rgba8Unorm and normal/emissive as Rgba16Float
