@@ -159,16 +159,18 @@ def bench_4k(ctx, args):
     ctx.synchronize()
     if cull.overflow():
         raise RuntimeError("cluster pool overflow")
-    q = shade.count_pairs(ds, grid)
+    # Q, and the (wavefront, light) visits of the shading walk (each visit
+    # evaluates one light for a wavefront's 64 pixels)
+    q, visits = shade.walk_stats(ds, grid)
     cnt = cull.cgrid.counts.download(np.uint32)
     ncell, nlist = len(cnt), int(cnt.sum())
-    log(f"4K plain: Q={q} cluster entries={nlist}")
+    log(f"4K plain: Q={q} wave-light visits={visits} cluster entries={nlist}")
     # the shading call uses the light records the frame's cull built
     from helio_amd import abi
     total, (cull_ms, shade_ms) = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
                                             lambda: shade.execute(ds, grid, flags=abi.HL_SHADE_LIGHTS_FROM_CULL))
     res = {"scene": sc, "params": params, "pool_cap": pool_cap, "q": q, "ncell": ncell, "nlist": nlist, "counts": cnt,
-           "ms_frame": total, "cull_ms": cull_ms, "shade_ms": shade_ms}
+           "ms_frame": total, "cull_ms": cull_ms, "shade_ms": shade_ms, "visits": visits}
     # the same frame into the pass's RGBA16F pre_aa target (HL_SHADE_OUT_RGBA16F)
     shade16 = DeferredLightPass(ctx, sc.width, sc.height, out_format=abi.HL_FMT_RGBA16_FLOAT)
     shade16.prepare(ds)
@@ -374,15 +376,30 @@ def bench_adjoint(ctx, args):
     cull.prepare(ds)
     shade.prepare(ds)
     grid = cull.publish()
+    cull.execute(ds)
+    q, visits = shade.walk_stats(ds, grid)
     grads = AdjointBuffers(ctx, gt.width, gt.height, len(guess.lights))
     fb, (fb_cull, fb_shade) = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
                                          lambda: shade.forward_backward_l2(ds, grid, target, grads))
     fw, (_, fw_shade) = time_steps(ctx, args.steps, args.warmup, lambda: cull.execute(ds),
                                    lambda: shade.execute(ds, grid))
     loss = float(grads.loss.download(np.float64)[0])
-    log(f"512 adjoint: fwd+adjoint {fb:.4f} ms/step, forward {fw:.4f} ms/step, loss {loss:.4g}")
+    log(f"512 adjoint: fwd+adjoint {fb:.4f} ms/step, forward {fw:.4f} ms/step, loss {loss:.4g}, Q={q}")
+    # algorithmic bytes of the fused launch: per pixel the RGBA32F G-buffer
+    # (4 x 16 B) + depth (4), the target read (16), the output written (16)
+    # and the five gradient planes (20); per light its record (80 B) and the
+    # gradient slots (kGradSlots = 8 copies of 4 doubles)
+    npx, nl = gt.width * gt.height, len(guess.lights)
+    nbytes = npx * (64 + 4 + 16 + 16 + 20) + nl * (80 + 8 * 32)
+    gbs = nbytes / (fb_shade * 1e-3) / 1e9
     return {"ms_per_step": fb, "cull_ms": fb_cull, "fused_forward_adjoint_ms": fb_shade,
             "forward_only_ms_per_step": fw, "forward_shade_ms": fw_shade, "loss": loss,
+            "pairs_q": q, "wave_light_visits": visits, "lane_utilisation": q / (64 * visits) if visits else None,
+            "fused_evals_per_s": q / (fb_shade * 1e-3),
+            "fused_roofline": {"bound": "latency (VALU + FP64 atomics at 3 waves per SIMD)", "achieved": gbs,
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                               "bytes_per_launch": nbytes,
+                               "kernel": "k_shade (fused forward + L2 adjoint, clusters, fp32 G-buffer)"},
             "workload": "512x512 fp32 G-buffer, 256 point lights, clusters "
                         f"{params.grid_x}x{params.grid_y}x{params.grid_z}; per step: cluster cull + "
                         "forward + L2 loss + gradients wrt albedo/roughness/metallic per pixel and "
@@ -485,6 +502,9 @@ def main() -> int:
         "cull_ms": r4k["cull_ms"],
         "shade_ms": shade_ms,
         "shade_evals_per_s": q / (shade_ms * 1e-3),
+        "walk": {"wave_light_visits": r4k["visits"], "lane_utilisation": q / (64 * r4k["visits"]),
+                 "note": "each visit evaluates one light for a wavefront's 64 pixels; Q / (64 visits) is the "
+                         "share of those lanes that hold an in-range pixel"},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_shade (forward, clusters, fp16 G-buffer)",

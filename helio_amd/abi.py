@@ -258,6 +258,8 @@ SIGNATURES: dict[str, tuple] = {
                                        u32, u32]),
     "hl_count_pixel_light_pairs": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32,
                                           P(hl_light_grid), P(C.c_uint64)]),
+    "hl_shade_walk_stats": (None, [vp, vp, P(DeferredGlobals), P(hl_gbuffer), vp, u32, P(hl_light_grid),
+                                   P(C.c_uint64), P(C.c_uint64)]),
     "hl_forward_lit": (None, [vp, vp, P(ForwardLitGlobals), vp, u32, vp, u32, vp, vp, u32,
                               P(hl_light_grid), vp, u32]),
     "hl_forward_lit_count_pairs": (None, [vp, vp, P(ForwardLitGlobals), vp, u32, vp, u32, vp, vp, u32,

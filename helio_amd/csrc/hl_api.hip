@@ -845,12 +845,11 @@ HL_API hl_status hl_deferred_forward_ex(hl_context* c, const GpuCameraUniforms* 
                        out_pitch_bytes, flags);
 }
 
-HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUniforms* camera_d,
-                                            const DeferredGlobals* g, const hl_gbuffer* gb,
-                                            const GpuLight* lights_d, uint32_t num_lights,
-                                            const hl_light_grid* grid, uint64_t* pairs) {
+HL_API hl_status hl_shade_walk_stats(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
+                                     const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights,
+                                     const hl_light_grid* grid, uint64_t* pairs, uint64_t* wave_light_visits) {
     if (!c) return HL_EINVAL;
-    if (!pairs) return fail(c, HL_EINVAL, "pairs is null");
+    if (!pairs || !wave_light_visits) return fail(c, HL_EINVAL, "pairs or wave_light_visits is null");
     DeferredGlobals g0;
     if (!g) return fail(c, HL_EINVAL, "globals are null");
     g0 = *g;
@@ -859,15 +858,28 @@ HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUnifor
     int set;
     hl_status s = prepare_shade(c, camera_d, &g0, gb, lights_d, num_lights, grid, &a, &set, false);
     if (s != HL_OK) return s;
+    // words 2-3: pairs, 6-7: visits (4-5, the binned tile cull's overflow
+    // word, stay zero)
     unsigned long long* ctr = (unsigned long long*)(ctx_words(c) + 2);
-    HL_HIP(c, launch_fill_u32(c->stream, (uint32_t*)ctr, 0, 2));
+    HL_HIP(c, launch_fill_u32(c->stream, (uint32_t*)ctr, 0, 6));
     a.pair_counter = ctr;
     if ((s = shade_launch(c, set, kModeCount, a)) != HL_OK) return s;
-    unsigned long long v = 0;
-    HL_HIP(c, hipMemcpyAsync(&v, ctr, 8, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long v[3] = {0, 0, 0};
+    HL_HIP(c, hipMemcpyAsync(v, ctr, sizeof v, hipMemcpyDeviceToHost, c->stream));
     HL_HIP(c, hipStreamSynchronize(c->stream));
-    *pairs = v;
+    *pairs = v[0];
+    *wave_light_visits = v[2];
     return HL_OK;
+}
+
+HL_API hl_status hl_count_pixel_light_pairs(hl_context* c, const GpuCameraUniforms* camera_d,
+                                            const DeferredGlobals* g, const hl_gbuffer* gb,
+                                            const GpuLight* lights_d, uint32_t num_lights,
+                                            const hl_light_grid* grid, uint64_t* pairs) {
+    if (!c) return HL_EINVAL;
+    if (!pairs) return fail(c, HL_EINVAL, "pairs is null");
+    uint64_t visits = 0;
+    return hl_shade_walk_stats(c, camera_d, g, gb, lights_d, num_lights, grid, pairs, &visits);
 }
 
 }  // extern "C"

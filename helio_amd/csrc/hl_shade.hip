@@ -127,8 +127,9 @@ struct ShadeSmem {
 // LDS of the adjoint modes only (declared by their kernels alone, so that the
 // forward's occupancy is not charged for it)
 struct AdjSmem {
-    KeptLights kept[4];   // the fused adjoint's second pass
-    float red[4][5][64];  // the per-wave partial sums of a pixel tile
+    KeptLights kept[4];      // fused: the lights each wave's walk visited
+    float gl[4][3][64];      // fused: the waves' partial radiance sums
+    float red[1][4][5][64];  // the waves' partial per-pixel gradient sums
 };
 
 // Fills sm.vc from the view's arguments; the caller synchronises the block
@@ -145,51 +146,15 @@ __device__ __forceinline__ void load_view_const(const ShadeArgs& a, ShadeSmem& s
 
 // F16: the output target is RGBA16F (a compile-time choice, so that the
 // RGBA32F kernels hold no code of the other format)
-template <int FA, int FN, int GRID, int MODE, bool F16 = false>
-__device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem* as) {
-    const ViewConst& vc = sm.vc;
-    b = hl_uniform(b);  // the item is the block's: its tile arithmetic stays scalar
-    const uint32_t lane = threadIdx.x & 63u, wave = hl_uniform(threadIdx.x >> 6);
-    constexpr bool kAdj = MODE == kModeBackward || MODE == kModeFusedL2;
-    // The adjoint modes split a wave tile's adjoint pass over the block's four
-    // waves (every fourth light each; the items are wave tiles, four per block
-    // tile): its per-light work is several times the forward's, and the
-    // heaviest tiles would otherwise set the launch's length.  The fused
-    // mode's forward pass stays whole in every wave, so that its output is the
-    // forward's bit for bit.  The other modes give each wave its own tile of
-    // the block tile.
-    constexpr bool kSplit = kAdj;
-    const uint32_t twl = a.tw_log2, tw = 1u << twl;
-    const uint32_t th = 64u >> twl;
-    const uint32_t bt = kSplit ? b >> 2 : b, sub = kSplit ? (b & 3u) : wave;
-    const uint32_t bx = bt % a.block_tiles_x, by = bt / a.block_tiles_x;
-    const uint32_t x = bx * (4u * tw) + sub * tw + (lane & (tw - 1u));
-    const uint32_t y = by * th + (lane >> twl);
-    const bool inside = x < a.width && y < a.height;
-    const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
-    const bool valid = inside && depth < 1.0f;
-    const bool lead = !kSplit || wave == 0u;  // writes the tile's shared results
-    constexpr bool kRC = MODE == kModeExtended;
-    constexpr bool kExt = MODE == kModeShadowed || kRC;  // exact N, shadow_factor
-    constexpr bool kFwd = MODE == kModeForward || kExt;
-    constexpr bool kWrites = kFwd || MODE == kModeFusedL2;
-    // the pixel's address, computed once (8 B per pixel for RGBA16F, else 16)
-    constexpr bool out_f16 = F16;
-    uint8_t* out_px = (kWrites && inside) ? reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch +
-                                                (size_t)x * (out_f16 ? 8u : 16u)
-                                          : nullptr;
-    const size_t plane = (size_t)a.width * a.height;
-    const size_t pix = (size_t)y * a.width + x;
-    if (inside && !valid && lead) {
-        if (kWrites && (a.flags & HL_SHADE_CLEAR_SKY))
-            hl_store_px(out_px, out_f16, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-        if (kAdj)
-            for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) hl_g(a.d_pixel)[k * plane + pix] = 0.0f;
-    }
-    if (hl_ballot(valid) == 0) return;
-
-    // ---- per-pixel set-up (deferred_lighting.wgsl:999-1086)
-    Pixel p;
+// The per-pixel set-up of one lane (deferred_lighting.wgsl:999-1086): the
+// world position and view vector from the depth, the pixel's cell, and the
+// decoded G-buffer texels with the material terms.  A and E are the decoded
+// albedo and emissive texels.  Lanes without a pixel (valid false) get
+// harmless values.
+template <int FA, int FN, int GRID, int MODE>
+__device__ __forceinline__ void setup_pixel(const ShadeArgs& a, const ViewConst& vc, uint32_t x, uint32_t y,
+                                            float depth, bool valid, Pixel& p, uint32_t& cell, float4& A,
+                                            float4& E) {
     // the four G-buffer fetches go out together; the set-up that needs the
     // depth only (world position, view vector, cell) runs while they are in
     // flight, and the texels are decoded after it
@@ -238,7 +203,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     }
 
     // ---- cell of this pixel
-    uint32_t cell = 0;
+    cell = 0;
     if (GRID == HL_GRID_TILES) {
         cell = (y >> 4) * a.tile_stride_x + (x >> 4);
     } else {
@@ -265,7 +230,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     }
 
     // zero bits decode to 0 in every format; lanes without a pixel keep N = +y
-    const float4 A = hl_decode<FA>(rA), O = hl_decode<FA>(rO), E = hl_decode<FN>(rE);
+    A = hl_decode<FA>(rA);
+    E = hl_decode<FN>(rE);
+    const float4 O = hl_decode<FA>(rO);
     float4 Nr = hl_decode<FN>(rN);
     if (!valid) Nr = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
     {
@@ -287,7 +254,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     p.rough = O.y;
     p.metal = O.z;
     p.one_m = 1.0f - p.metal;
-    if (kAdj) {
+    if (MODE == kModeBackward || MODE == kModeFusedL2) {
         // the differentiated forward resolves F0 from albedo and metallic
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -313,6 +280,93 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         for (int c = 0; c < 3; ++c) p.kd[c] = p.one_m * p.alb[c] * kInvPi;
     }
 
+}
+
+// Ambient + emissive (deferred_lighting.wgsl:1196-1231), added to the sum of
+// the lights at the end; mixv is the sky hemisphere mix.
+__device__ __forceinline__ void ambient_terms(const ShadeArgs& a, const Pixel& p, const float4& E, float mixv[3],
+                                              float amb[3]) {
+    const float ht = p.ny * 0.5f + 0.5f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float sky = a.sky[c];
+        mixv[c] = sky * 0.15f * (1.0f - ht) + sky * ht;
+        amb[c] = mixv[c] * p.alb[c] * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
+    }
+}
+
+// One light's adjoint for the pixels of the lanes with mine set: the
+// per-pixel gradients go to dA, dR, dM; the light's position and intensity
+// gradients, summed over the wave, to its slot copy `slot` (FP64 atomics by
+// lanes 0..3).  gl is dL/d(radiance) of the lane's pixel.
+__device__ __forceinline__ void adjoint_light(const ShadeArgs& a, const HotLight& L, uint32_t li, bool mine,
+                                              const Pixel& p, const float gl[3], float dA[3], float& dR, float& dM,
+                                              uint32_t slot) {
+    const uint32_t lane = hl_lane();
+    float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool act = false;
+    if (mine) {
+        Term t;
+        act = eval_term<true>(L, p, t);
+        if (act) adjoint_term(L, p, t, gl, dA, dR, dM, g);
+    }
+    if (hl_ballot(act) == 0) return;
+    const float sx = dpp_sum(g.x), sy = dpp_sum(g.y), sz = dpp_sum(g.z), si = dpp_sum(g.w);
+    if (lane < 4u) {
+        const float v = lane == 0u ? sx : (lane == 1u ? sy : (lane == 2u ? sz : si));
+        atomicAdd(a.d_light + 4 * ((size_t)slot * a.num_lights + li) + lane, (double)v);
+    }
+}
+
+// Every mode but the fused forward + adjoint (fused_tile).
+template <int FA, int FN, int GRID, int MODE, bool F16 = false>
+__device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem* as) {
+    static_assert(MODE != kModeFusedL2, "fused_tile");
+    const ViewConst& vc = sm.vc;
+    b = hl_uniform(b);  // the item is the block's: its tile arithmetic stays scalar
+    const uint32_t lane = threadIdx.x & 63u, wave = hl_uniform(threadIdx.x >> 6);
+    constexpr bool kAdj = MODE == kModeBackward;
+    // The backward mode splits a wave tile's adjoint over the block's four
+    // waves (every fourth light each; the items are wave tiles, four per block
+    // tile): its per-light work is several times the forward's, and the
+    // heaviest tiles would otherwise set the launch's length.  The other modes
+    // give each wave its own tile of the block tile.
+    constexpr bool kSplit = kAdj;
+    const uint32_t twl = a.tw_log2, tw = 1u << twl;
+    const uint32_t th = 64u >> twl;
+    const uint32_t bt = kSplit ? b >> 2 : b, sub = kSplit ? (b & 3u) : wave;
+    const uint32_t bx = bt % a.block_tiles_x, by = bt / a.block_tiles_x;
+    const uint32_t x = bx * (4u * tw) + sub * tw + (lane & (tw - 1u));
+    const uint32_t y = by * th + (lane >> twl);
+    const bool inside = x < a.width && y < a.height;
+    const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
+    const bool valid = inside && depth < 1.0f;
+    const bool lead = !kSplit || wave == 0u;  // writes the tile's shared results
+    constexpr bool kRC = MODE == kModeExtended;
+    constexpr bool kExt = MODE == kModeShadowed || kRC;  // exact N, shadow_factor
+    constexpr bool kFwd = MODE == kModeForward || kExt;
+    constexpr bool kWrites = kFwd;
+    // the pixel's address, computed once (8 B per pixel for RGBA16F, else 16)
+    constexpr bool out_f16 = F16;
+    uint8_t* out_px = (kWrites && inside) ? reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch +
+                                                (size_t)x * (out_f16 ? 8u : 16u)
+                                          : nullptr;
+    const size_t plane = (size_t)a.width * a.height;
+    const size_t pix = (size_t)y * a.width + x;
+    if (inside && !valid && lead) {
+        if (kWrites && (a.flags & HL_SHADE_CLEAR_SKY))
+            hl_store_px(out_px, out_f16, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        if (kAdj)
+            for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) hl_g(a.d_pixel)[k * plane + pix] = 0.0f;
+    }
+    if (hl_ballot(valid) == 0) return;
+
+    // ---- per-pixel set-up (deferred_lighting.wgsl:999-1086)
+    Pixel p;
+    uint32_t cell;
+    float4 A, E;
+    setup_pixel<FA, FN, GRID, MODE>(a, vc, x, y, depth, valid, p, cell, A, E);
+
     // ---- debug views (deferred_lighting.wgsl:1032-1038)
     if (kFwd && a.debug_mode != 0u) {
         if (valid) {
@@ -328,15 +382,13 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 
     HotLight* s_l = sm.light[wave];
     float lo[3] = {0.0f, 0.0f, 0.0f};
-    // the fused adjoint revisits the forward walk's lights from LDS when they
-    // fit (kStage per wave), else walks the lists again
-    KeptState ks;
-    KeptLights* kept = MODE == kModeFusedL2 ? &as->kept[wave] : nullptr;
 
     if (MODE == kModeCount) {
         uint32_t pairs = 0;
+        uint32_t visits = 0;  // (wave, light) visits of the walk: uniform
         walk_lights<GRID>(a, s_l, p, valid, cell,
                           [&](const HotLight& L, uint32_t, bool mine) {
+                              ++visits;
                               if (!mine || __builtin_isinf(L.p0.w)) {
                                   pairs += mine ? 1u : 0u;
                                   return;
@@ -348,19 +400,14 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         // exact: at most 64 pixels x lights per cell per wave, far below 2^24
         const uint32_t tot = (uint32_t)dpp_sum((float)pairs);
         if (lane == 0 && tot) atomicAdd(a.pair_counter, (unsigned long long)tot);
+        // the visits go to the counter two words on (a.pair_counter[2])
+        if (lane == 0 && visits) atomicAdd(a.pair_counter + 2, (unsigned long long)visits);
         return;
     }
 
-    // ambient + emissive (deferred_lighting.wgsl:1196-1231), added to the sum
-    // of the lights at the end
-    const float ht = p.ny * 0.5f + 0.5f;
+    // ambient + emissive, added to the sum of the lights at the end
     float mixv[3], amb[3], o[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const float sky = a.sky[c];
-        mixv[c] = sky * 0.15f * (1.0f - ht) + sky * ht;
-        amb[c] = mixv[c] * p.alb[c] * p.ao + (c == 0 ? E.x : (c == 1 ? E.y : E.z));
-    }
+    ambient_terms(a, p, E, mixv, amb);
     if (kRC && a.rc_fmt != 0u && valid) {
         // radiance-cascade indirect diffuse (deferred_lighting.wgsl:1159-1163,
         // 1189-1205): mix(hemisphere, kD_ibl rc_irr albedo, clamp(4 |rc_irr|, 0, 1))
@@ -381,7 +428,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         }
     }
     if (MODE != kModeBackward) {
-        walk_lights<GRID, MODE == kModeFusedL2>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
+        walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
             if (!mine) return;
             Term t;
             if (eval_term<false>(L, p, t)) {
@@ -399,7 +446,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
                 lo[1] += t.B[1] * (L.p1.y * aw);
                 lo[2] += t.B[2] * (L.p1.z * aw);
             }
-        }, kept, &ks);
+        });
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = lo[c] + amb[c];
@@ -408,22 +455,9 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         return;
     }
 
-    // ---- adjoint
+    // ---- adjoint (backward: dL/d(radiance) given)
     float gl[3] = {0.0f, 0.0f, 0.0f};
-    if (MODE == kModeFusedL2) {
-        float l2 = 0.0f;
-        if (valid) {
-            if (lead) hl_store_px(out_px, out_f16, make_float4(o[0], o[1], o[2], A.w));
-            const float4 tg = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
-            const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
-            gl[0] = 2.0f * e0;
-            gl[1] = 2.0f * e1;
-            gl[2] = 2.0f * e2;
-            l2 = e0 * e0 + e1 * e1 + e2 * e2;
-        }
-        const float wl = dpp_sum(l2);
-        if (lane == 0 && lead) atomicAdd(a.loss, (double)wl);
-    } else if (valid) {
+    if (valid) {
         const float4 g = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
         gl[0] = g.x;
         gl[1] = g.y;
@@ -433,51 +467,24 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
     // per-light gradients go to one of kGradSlots copies (by wave) so that the
     // FP64 atomics of many waves on one light do not serialise on one address
     const uint32_t slot = (b * 4u + wave) % kGradSlots;
-    const auto adj = [&](const HotLight& L, uint32_t li, bool mine) {
-        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        bool act = false;
-        if (mine) {
-            Term t;
-            act = eval_term<true>(L, p, t);
-            if (act) adjoint_term(L, p, t, gl, dA, dR, dM, g);
-        }
-        if (hl_ballot(act) == 0) return;
-        const float sx = dpp_sum(g.x), sy = dpp_sum(g.y), sz = dpp_sum(g.z), si = dpp_sum(g.w);
-        // lanes 0..3 add the four sums to this wave's slot of the light
-        if (lane < 4u) {
-            const float v = lane == 0u ? sx : (lane == 1u ? sy : (lane == 2u ? sz : si));
-            atomicAdd(a.d_light + 4 * ((size_t)slot * a.num_lights + li) + lane, (double)v);
-        }
-    };
-    if (MODE == kModeFusedL2 && ks.ok) {
-        for (uint32_t i = kSplit ? wave : 0u; i < ks.n; i += kSplit ? 4u : 1u) {  // uniform
-            const HotLight L = s_l[i];
-            const uint32_t li = hl_uniform(kept->idx[i]);
-            const uint64_t lanes = kept->lanes[i];
-            adj(L, li, ((lanes >> lane) & 1ull) != 0ull);
-        }
-    } else {
-        uint32_t ord2 = 0;
-        walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
-            if (kSplit && (ord2++ & 3u) != wave) return;
-            adj(L, li, mine);
-        });
-    }
-    if (kSplit) {
-        // the waves' partial per-pixel gradients, added in wave order
-        as->red[wave][0][lane] = dA[0];
-        as->red[wave][1][lane] = dA[1];
-        as->red[wave][2][lane] = dA[2];
-        as->red[wave][3][lane] = dR;
-        as->red[wave][4][lane] = dM;
-        __syncthreads();
+    uint32_t ord = 0;
+    walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
+        if ((ord++ & 3u) != wave) return;
+        adjoint_light(a, L, li, mine, p, gl, dA, dR, dM, slot);
+    });
+    // the waves' partial per-pixel gradients, added in wave order
+    float (*red)[5][64] = as->red[0];
+    red[wave][0][lane] = dA[0];
+    red[wave][1][lane] = dA[1];
+    red[wave][2][lane] = dA[2];
+    red[wave][3][lane] = dR;
+    red[wave][4][lane] = dM;
+    __syncthreads();
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            dA[c] = ((as->red[0][c][lane] + as->red[1][c][lane]) + as->red[2][c][lane]) + as->red[3][c][lane];
-        dR = ((as->red[0][3][lane] + as->red[1][3][lane]) + as->red[2][3][lane]) + as->red[3][3][lane];
-        dM = ((as->red[0][4][lane] + as->red[1][4][lane]) + as->red[2][4][lane]) + as->red[3][4][lane];
-        __syncthreads();
-    }
+    for (int c = 0; c < 3; ++c) dA[c] = ((red[0][c][lane] + red[1][c][lane]) + red[2][c][lane]) + red[3][c][lane];
+    dR = ((red[0][3][lane] + red[1][3][lane]) + red[2][3][lane]) + red[3][3][lane];
+    dM = ((red[0][4][lane] + red[1][4][lane]) + red[2][4][lane]) + red[3][4][lane];
+    __syncthreads();
     if (valid && lead) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) dA[c] += gl[c] * mixv[c] * p.ao;
@@ -488,6 +495,119 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
         dp[3 * plane + pix] = dR;
         dp[4 * plane + pix] = dM;
     }
+}
+
+// The fused forward + L2 adjoint of wave tile b & 3 of block tile b >> 2
+// (the items are wave tiles, four per block tile): radiance into a.out, the
+// loss sum((out - target)^2) into a.loss, and its gradients.  The block's
+// four waves share the tile: each walks its lights (keeping the ones it
+// visits in LDS) and evaluates every fourth one (ordinals = wave mod 4) in
+// the forward and the adjoint alike; the waves' partial sums are added in
+// wave order, so the output depends on the tile shape at the last bits.
+template <int FA, int FN, int GRID>
+__device__ __forceinline__ void fused_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem& as) {
+    constexpr int MODE = kModeFusedL2;
+    const ViewConst& vc = sm.vc;
+    b = hl_uniform(b);
+    const uint32_t lane = threadIdx.x & 63u, wave = hl_uniform(threadIdx.x >> 6);
+    const uint32_t twl = a.tw_log2, tw = 1u << twl, th = 64u >> twl;
+    const uint32_t bt = b >> 2, sub = b & 3u;
+    const uint32_t bx = bt % a.block_tiles_x, by = bt / a.block_tiles_x;
+    const uint32_t x = bx * (4u * tw) + sub * tw + (lane & (tw - 1u));
+    const uint32_t y = by * th + (lane >> twl);
+    const bool inside = x < a.width && y < a.height;
+    const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
+    const bool valid = inside && depth < 1.0f;
+    const bool lead = wave == 0u;  // writes the tile's results
+    const size_t plane = (size_t)a.width * a.height;
+    const size_t pix = (size_t)y * a.width + x;
+    float4* out_px = inside ? (float4*)(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x : nullptr;
+    if (inside && !valid && lead) {
+        if (a.flags & HL_SHADE_CLEAR_SKY) hl_store_nt(out_px, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) hl_g(a.d_pixel)[k * plane + pix] = 0.0f;
+    }
+    if (hl_ballot(valid) == 0) return;  // uniform over the block: one tile
+
+    Pixel p;
+    uint32_t cell;
+    float4 A, E;
+    setup_pixel<FA, FN, GRID, MODE>(a, vc, x, y, depth, valid, p, cell, A, E);
+    float mixv[3], amb[3];
+    ambient_terms(a, p, E, mixv, amb);
+
+    // ---- forward: every fourth light, the partial sums added in wave order
+    HotLight* s_l = sm.light[wave];
+    KeptLights& kept = as.kept[wave];
+    KeptState ks;
+    float lo[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t ord = 0;
+    walk_lights<GRID, true>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
+        if ((ord++ & 3u) != wave || !mine) return;
+        Term t;
+        if (eval_term<false>(L, p, t)) {
+            const float aw = t.atten * t.ndl;
+            lo[0] += t.B[0] * (L.p1.x * aw);
+            lo[1] += t.B[1] * (L.p1.y * aw);
+            lo[2] += t.B[2] * (L.p1.z * aw);
+        }
+    }, &kept, &ks);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) as.gl[wave][c][lane] = lo[c];
+    __syncthreads();
+    float o[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        o[c] = (((as.gl[0][c][lane] + as.gl[1][c][lane]) + as.gl[2][c][lane]) + as.gl[3][c][lane]) + amb[c];
+
+    float gl[3] = {0.0f, 0.0f, 0.0f};
+    float l2 = 0.0f;
+    if (valid) {
+        if (lead) hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+        const float4 tg = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
+        const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
+        gl[0] = 2.0f * e0;
+        gl[1] = 2.0f * e1;
+        gl[2] = 2.0f * e2;
+        l2 = e0 * e0 + e1 * e1 + e2 * e2;
+    }
+    const float wl = dpp_sum(l2);
+    if (lane == 0 && lead) atomicAdd(a.loss, (double)wl);
+
+    // ---- adjoint: the same lights per wave, from LDS when they were kept
+    float dA[3] = {0.0f, 0.0f, 0.0f}, dR = 0.0f, dM = 0.0f;
+    const uint32_t slot = (b * 4u + wave) % kGradSlots;
+    if (ks.ok) {
+        for (uint32_t i = wave; i < ks.n; i += 4u) {  // uniform
+            const HotLight L = s_l[i];
+            const uint32_t li = hl_uniform(kept.idx[i]);
+            const uint64_t lanes = kept.lanes[i];
+            adjoint_light(a, L, li, ((lanes >> lane) & 1ull) != 0ull, p, gl, dA, dR, dM, slot);
+        }
+    } else {
+        uint32_t ord2 = 0;
+        walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
+            if ((ord2++ & 3u) != wave) return;
+            adjoint_light(a, L, li, mine, p, gl, dA, dR, dM, slot);
+        });
+    }
+    float (*red)[5][64] = as.red[0];
+    red[wave][0][lane] = dA[0];
+    red[wave][1][lane] = dA[1];
+    red[wave][2][lane] = dA[2];
+    red[wave][3][lane] = dR;
+    red[wave][4][lane] = dM;
+    __syncthreads();
+    if (valid && lead) {
+        float g[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) g[k] = ((red[0][k][lane] + red[1][k][lane]) + red[2][k][lane]) + red[3][k][lane];
+        const hl_gptr<float> dp = hl_g(a.d_pixel);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dp[c * plane + pix] = g[c] + gl[c] * mixv[c] * p.ao;
+        dp[3 * plane + pix] = g[3];
+        dp[4 * plane + pix] = g[4];
+    }
+    __syncthreads();  // red and gl are rewritten by the next item
 }
 
 // ------------------------------------------------------------------ kernels
@@ -580,7 +700,8 @@ __global__ __launch_bounds__(256, (shade_min_blocks<(MODE & ~kModeOutF16), FA>()
     if (blockIdx.x == 0 && threadIdx.x < kShadeQueues) queue_next[threadIdx.x * kShadeQueueStride] = 0u;
     load_view_const(a, sm);  // published by the first claim's barrier
     claim_items(shade_items(M, a.num_blocks), queue, cs, [&](uint32_t t) {
-        shade_tile<FA, FN, GRID, M, (MODE & kModeOutF16) != 0>(a, t, sm, as);
+        if constexpr (M == kModeFusedL2) fused_tile<FA, FN, GRID>(a, t, sm, *as);
+        else shade_tile<FA, FN, GRID, M, (MODE & kModeOutF16) != 0>(a, t, sm, as);
     });
 }
 
@@ -632,7 +753,8 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_bat
 #pragma unroll
         for (uint32_t i = 0; i < sizeof(ShadeArgs) / 4; ++i)
             ((uint32_t*)&ra)[i] = __builtin_amdgcn_readfirstlane(((const uint32_t*)&sa)[i]);
-        shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(ra, it - first, sm, as);
+        if constexpr (MODE == kModeFusedL2) fused_tile<FA, FN, HL_GRID_CLUSTERS>(ra, it - first, sm, *as);
+        else shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(ra, it - first, sm, as);
     });
 }
 

@@ -416,6 +416,14 @@ class DeferredLightPass:
                           C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid), C.byref(n))
         return n.value
 
+    def walk_stats(self, scene: DeviceScene, grid: abi.hl_light_grid) -> tuple[int, int]:
+        """(Q, wavefront-light visits of the forward walk), hl_shade_walk_stats."""
+        n, v = C.c_uint64(), C.c_uint64()
+        self.ctx.lib.call("hl_shade_walk_stats", self.ctx.h, scene.camera.ptr, C.byref(self.globals),
+                          C.byref(scene.gbuffer), scene.lights.ptr, scene.num_lights, C.byref(grid), C.byref(n),
+                          C.byref(v))
+        return n.value, v.value
+
     def output(self) -> np.ndarray:
         """The target as float32 (RGBA16F values widened exactly)."""
         if self.out_f16:

@@ -449,6 +449,15 @@ HL_API hl_status hl_count_pixel_light_pairs(hl_context* ctx, const GpuCameraUnif
                                             const hl_gbuffer* gbuffer, const GpuLight* lights_d,
                                             uint32_t num_lights, const hl_light_grid* grid,
                                             uint64_t* pairs);
+/* Q as above, and the (wavefront, light) visits of the forward walk: the
+ * lights the shading launch evaluates for 64 pixels at once (per wavefront,
+ * per distinct cell of its pixels, the cell's lights past the test against
+ * the bounds of those pixels).  Q / (64 visits) is the lane utilisation of the
+ * per-light evaluation.  A diagnostic with no reference counterpart. */
+HL_API hl_status hl_shade_walk_stats(hl_context* ctx, const GpuCameraUniforms* camera_d,
+                                     const DeferredGlobals* globals, const hl_gbuffer* gbuffer,
+                                     const GpuLight* lights_d, uint32_t num_lights, const hl_light_grid* grid,
+                                     uint64_t* pairs, uint64_t* wave_light_visits);
 
 /* ------------------------------------------------------------ forward-lit */
 /* One fragment of a forward-shaded surface: what ForwardLit's fs_main holds

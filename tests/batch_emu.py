@@ -446,7 +446,8 @@ def check_shade_items(views, plan, cus: int = 256) -> None:
                 continue
             tw = al[i]["tw"]
             bw, bh = 4 << tw, 64 >> tw
-            nb = -(-views[i].width // bw) * -(-views[i].height // bh)
+            # adjoint views: one item per wave tile (shade_items), four per block tile
+            nb = -(-views[i].width // bw) * -(-views[i].height // bh) * (4 if adj else 1)
             spans.append((nitems, i, nb, bw, bh))
             nitems += nb
         if not spans:
@@ -461,7 +462,8 @@ def check_shade_items(views, plan, cus: int = 256) -> None:
             assert (b >= 0).all() and (b < nb).all()
             v = views[i]
             btx = -(-v.width // bw)
-            bx, by = b % btx, b // btx
+            bt = b >> 2 if adj else b
+            bx, by = bt % btx, bt // btx
             x_max = bx.max() * bw + bw - 1
             y_max = by.max() * bh + bh - 1
             # the tiles cover the view; pixels past the edge are masked (inside test)

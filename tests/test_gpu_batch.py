@@ -4,14 +4,17 @@ adjoint on the two eyes).
 
 Bars:
   * every view's radiance equals the single-view path run with the same
-    cluster grid bit for bit (same kernels, same lists, same order);
+    cluster grid bit for bit (same kernels, same lists, same order); an
+    adjoint view whose wave tile differs from the single-view path's 8 x 8
+    within 1e-6 relative (the fused forward adds its four waves' partial
+    sums per wave tile, so the tile shape moves the last bits);
   * the eyes' gradients equal the single-view fused adjoint within 1e-5
     relative + 1e-7 of the largest magnitude (FP64 atomics reorder sums);
   * each replay of the captured HIP graph equals the eager frame, with eager
     work interleaved between replays;
   * the autotuner returns a valid configuration and its cache round-trips;
   * views given a fixed cluster grid (with a per-cluster cap) get exactly that
-    grid, and equal the single-view path on it bit for bit.
+    grid, and equal the single-view path on it as above.
 """
 import ctypes as C
 import dataclasses
@@ -102,7 +105,10 @@ def test_batch_equals_single_view(ctx, hlib, mv):
             cfg = plan.config(i)
             ref = single_view(ctx, hlib, v, cfg)
             sky = v.scene.host.depth >= 1.0
-            assert np.array_equal(got[i]["out"][~sky], ref["out"][~sky]), f"view {i}"
+            if v.adjoint and (cfg.wave_tile_w, cfg.wave_tile_h) != (8, 8):
+                assert close(got[i]["out"][~sky], ref["out"][~sky], rel=1e-6), f"view {i}"
+            else:
+                assert np.array_equal(got[i]["out"][~sky], ref["out"][~sky]), f"view {i}"
             if v.adjoint:
                 assert abs(got[i]["loss"] - ref["loss"]) <= 1e-6 * abs(ref["loss"])
                 assert close(got[i]["dpx"], ref["dpx"]), f"view {i} pixel grads"
@@ -195,7 +201,10 @@ def test_fixed_cluster_grid(ctx, hlib, mv):
             got = read_view(v, pool_size, hosts[i][1])
             ref = single_view(ctx, hlib, v, cfg, cap=fixed[k][1])
             sky = v.scene.host.depth >= 1.0
-            assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
+            if v.adjoint and (cfg.wave_tile_w, cfg.wave_tile_h) != (8, 8):
+                assert close(got["out"][~sky], ref["out"][~sky], rel=1e-6), f"view {i}"
+            else:
+                assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
             if v.adjoint:
                 assert close(got["dpx"], ref["dpx"]), f"view {i} pixel grads"
                 assert close(got["dint"], ref["dint"]), f"view {i} intensity grads"
@@ -234,17 +243,21 @@ def test_multiview_pixel_limit(ctx, mv):
                          "hl_batch_plan_create refuses multi-view plans above HL_BATCH_MAX_MULTIVIEW_PIXELS")
 def test_full_size_c5(ctx, hlib):
     """Configuration 5 at full size: 48 probe faces + 2 eyes at 2064x2208 in
-    one autotuned plan, every view bit-exact against the single-view path,
-    graph replay equal to the eager frame."""
+    one autotuned plan, every view against the single-view path as in
+    test_batch_equals_single_view."""
     pool, pool_size, views, hosts, _ = make_multiview(ctx, small=False)
     plan = BatchPlan(ctx, pool, pool_size, views, autotune=True, max_trials=4)
     try:
         plan.launch()
         ctx.synchronize()
         for i, v in enumerate(views):
-            ref = single_view(ctx, hlib, v, plan.config(i))
+            cfg = plan.config(i)
+            ref = single_view(ctx, hlib, v, cfg)
             got = read_view(v, pool_size, hosts[i][1])
             sky = v.scene.host.depth >= 1.0
-            assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
+            if v.adjoint and (cfg.wave_tile_w, cfg.wave_tile_h) != (8, 8):
+                assert close(got["out"][~sky], ref["out"][~sky], rel=1e-6), f"view {i}"
+            else:
+                assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
     finally:
         plan.destroy()

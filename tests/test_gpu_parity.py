@@ -335,3 +335,38 @@ def test_rgba16f_target(ctx, hlib, name, shadowed):
         want = outs[0].astype(np.float16)
     assert np.array_equal(outs[1].view(np.uint16), want.view(np.uint16))
     assert np.all(outs[1][sc.depth >= 1.0] == 0)
+
+
+def test_walk_stats(ctx):
+    """hl_shade_walk_stats: with directional lights only (every tile lists
+    them and they pass every bounds test) each 8x8 wavefront with a shaded
+    pixel visits each light once, and Q is lights x shaded pixels; with point
+    lights, Q equals the oracle's and 64 visits cover Q."""
+    print("[walk stats]", flush=True)
+    sc = synth.make_scene("c1_720p_64", num_lights=5)
+    d = np.array([0.35, -1.0, 0.25]) / np.linalg.norm([0.35, -1.0, 0.25])
+    for i in range(5):
+        sc.lights[i]["light_type"] = abi.HL_LIGHT_DIRECTIONAL
+        sc.lights[i]["direction_outer"] = (*d, 0.0)
+    ds, cull, _, _ = hip_tiles(ctx, sc)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    q, visits = shade.walk_stats(ds, cull.publish())
+    valid = sc.depth < 1.0
+    h8, w8 = (sc.height + 7) // 8, (sc.width + 7) // 8
+    pad = np.zeros((h8 * 8, w8 * 8), bool)
+    pad[:sc.height, :sc.width] = valid
+    waves = int(pad.reshape(h8, 8, w8, 8).any(axis=(1, 3)).sum())
+    assert q == 5 * int(valid.sum())
+    assert visits == 5 * waves
+
+    sc = scene("c1_720p_64")
+    ds, cull, lists, counts = hip_tiles(ctx, sc)
+    shade = DeferredLightPass(ctx, sc.width, sc.height)
+    shade.prepare(ds)
+    q, visits = shade.walk_stats(ds, cull.publish())
+    assert q == shade.count_pairs(ds, cull.publish())
+    _, opairs = O.shade(sc, O.tile_grid_host(sc, np.ascontiguousarray(lists.reshape(-1)), counts))
+    assert q == opairs
+    assert 64 * visits >= q > 0
+    print(f"  Q {q}, visits {visits}, lane utilisation {q / (64 * visits):.3f}")
