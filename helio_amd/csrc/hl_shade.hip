@@ -713,22 +713,25 @@ __global__ __launch_bounds__(256, (shade_min_blocks<(MODE & ~kModeOutF16), FA>()
 // which would halve the forward's occupancy in a shared kernel.  A block
 // keeps the item range of its current view; when an item falls outside it
 // (rare: each sub-queue's items run through one view at a time) the block
-// finds the span by bisection and stages that view's arguments and constants
-// in LDS; each item then reads the arguments into scalar registers.
+// finds the span by bisection and stages that view's constants in LDS.  The
+// view's arguments are read where they lie, through a block-uniform index:
+// the loads are scalar, as k_shade's kernel arguments are.
+// views is read through the constant address space: the records are not
+// written while the launch runs, and their uniform loads become scalar loads.
+typedef const __attribute__((address_space(4))) BatchShadeView* ConstViews;
+
 template <int FA, int FN, int MODE>
 __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_batch(
-    const BatchShadeView* __restrict__ views, const uint2* __restrict__ spans, uint32_t nspans,
+    ConstViews __restrict__ views, const uint2* __restrict__ spans, uint32_t nspans,
     uint32_t num_items, uint32_t* __restrict__ queue) {
-    static_assert(sizeof(ShadeArgs) % 4 == 0, "copied as words");
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
-    __shared__ ShadeArgs sa;
     AdjSmem* as = nullptr;
     if constexpr (MODE == kModeFusedL2) {
         __shared__ AdjSmem adj;
         as = &adj;
     }
-    uint32_t first = 1u, end = 0u;  // item range of the staged view (none yet)
+    uint32_t first = 1u, end = 0u, v = 0u;  // item range of the staged view (none yet), its index
     claim_items(num_items, queue, cs, [&](uint32_t it) {
         it = hl_uniform(it);  // block-uniform: every thread read the same claim
         if (it < first || it >= end) {
@@ -740,19 +743,14 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_bat
             }
             first = hl_uniform(spans[lo].x);
             end = hl_uniform(spans[lo + 1].x);
-            const uint32_t v = hl_uniform(spans[lo].y);
-            load_view_const(views[v].sa, sm);
-            const uint32_t* src = (const uint32_t*)&views[v].sa;
-            for (uint32_t i = threadIdx.x; i < sizeof(ShadeArgs) / 4; i += blockDim.x) ((uint32_t*)&sa)[i] = src[i];
+            v = hl_uniform(spans[lo].y);
+            __syncthreads();  // the previous view's constants are no longer read
+            load_view_const(*(const ShadeArgs*)&views[v].sa, sm);
             __syncthreads();
         }
-        // the view's arguments as wave-uniform values (scalar registers), as
-        // k_shade has them from its kernel arguments: read from LDS they
-        // would take vector registers and cap the forward at 4 waves per SIMD
-        ShadeArgs ra;
-#pragma unroll
-        for (uint32_t i = 0; i < sizeof(ShadeArgs) / 4; ++i)
-            ((uint32_t*)&ra)[i] = __builtin_amdgcn_readfirstlane(((const uint32_t*)&sa)[i]);
+        // v again as a scalar: carried across the claim loop it is not known
+        // to be uniform
+        const ShadeArgs& ra = *(const ShadeArgs*)&views[hl_uniform(v)].sa;
         if constexpr (MODE == kModeFusedL2) fused_tile<FA, FN, HL_GRID_CLUSTERS>(ra, it - first, sm, *as);
         else shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(ra, it - first, sm, as);
     });
@@ -762,9 +760,11 @@ template <int FA, int FN>
 static void launch_batch_fmt(hipStream_t s, int mode, const BatchShadeView* views, const uint2* spans,
                              uint32_t nspans, uint32_t num_items, uint32_t* queue, uint32_t grid_blocks) {
     if (mode == kModeFusedL2)
-        k_shade_batch<FA, FN, kModeFusedL2><<<grid_blocks, 256, 0, s>>>(views, spans, nspans, num_items, queue);
+        k_shade_batch<FA, FN, kModeFusedL2><<<grid_blocks, 256, 0, s>>>((ConstViews)views, spans, nspans, num_items,
+                                                                        queue);
     else
-        k_shade_batch<FA, FN, kModeForward><<<grid_blocks, 256, 0, s>>>(views, spans, nspans, num_items, queue);
+        k_shade_batch<FA, FN, kModeForward><<<grid_blocks, 256, 0, s>>>((ConstViews)views, spans, nspans, num_items,
+                                                                        queue);
 }
 
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
