@@ -135,11 +135,11 @@ hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
         HL_HIP(c, launch_fill_u32(s, (uint32_t*)d.d_light_pos_d, 0, 3 * (size_t)p->pool_size));
         HL_HIP(c, launch_fill_u32(s, (uint32_t*)d.d_light_intensity_d, 0, p->pool_size));
     }
-    HL_HIP(c, launch_fill_u32(s, p->d_flags, 0, nv));
+    // the cull's first kernel clears the views' overflow words; the shading
+    // launches leave their claim counters zero for the next frame
     HL_HIP(c, launch_batch_cull(s, p->pool, p->d_cull, p->d_light_base, p->d_cell_base,
                                 p->d_tile_base, nv, p->total_lights, p->total_cells, p->total_tiles,
                                 mark_queue_at(p->d_markq, p->total_lights)));
-    HL_HIP(c, launch_fill_u32(s, p->d_queue, 0, 6 * (size_t)kShadeQueueWords));
     for (int g = 0; g < 6; ++g)
         HL_HIP(c, launch_shade_batch(s, g / 2, (g & 1) ? kModeFusedL2 : kModeForward, p->d_shade,
                                      p->d_spans + p->grp_first[g], p->grp_spans[g], p->grp_count[g],
@@ -337,13 +337,15 @@ hl_status build_plan(hl_plan* p) {
         spans.push_back(make_uint2((uint32_t)n, 0u));
         p->grp_count[g] = (uint32_t)n;
         all_items += n;
-        // 8 blocks of 4 waves per CU, fewer when there are fewer items
-        p->grp_grid[g] = std::min((uint32_t)std::max(1, cus) * 8u, std::max(1u, p->grp_count[g]));
+        // the blocks the kernel keeps resident, fewer when there are fewer items
+        p->grp_grid[g] = std::min((uint32_t)std::max(1, cus) * shade_batch_blocks_per_cu(g / 2, (g & 1) ? kModeFusedL2
+                                                                                                    : kModeForward),
+                                  std::max(1u, p->grp_count[g]));
     }
     p->num_items = (uint32_t)all_items;
     if ((s = plan_alloc(p, (void**)&p->d_spans, sizeof(uint2) * spans.size())) != HL_OK) return s;
     HL_HIP(c, hipMemcpy(p->d_spans, spans.data(), sizeof(uint2) * spans.size(), hipMemcpyHostToDevice));
-    if ((s = plan_alloc(p, (void**)&p->d_queue, 4 * 6 * (size_t)kShadeQueueWords)) != HL_OK) return s;
+    if ((s = plan_alloc(p, (void**)&p->d_queue, 4 * 6 * (size_t)kShadeQueueWords, true)) != HL_OK) return s;
     return HL_OK;
 }
 

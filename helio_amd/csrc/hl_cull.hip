@@ -853,6 +853,8 @@ __global__ __launch_bounds__(256) void k_batch_geom(const BatchCullView* __restr
                                                     const GpuLight* __restrict__ pool,
                                                     const uint32_t* __restrict__ light_base,
                                                     uint32_t geom_blocks) {
+    // the views' overflow words start each frame at zero (nviews <= 255)
+    if (blockIdx.x == 0 && threadIdx.x < nviews) *views[threadIdx.x].flags = 0u;
     if (blockIdx.x >= geom_blocks) {
         const uint32_t l = (blockIdx.x - geom_blocks) * blockDim.x + threadIdx.x;
         if (l >= light_base[nviews]) return;

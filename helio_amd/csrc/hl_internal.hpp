@@ -279,9 +279,13 @@ hipError_t launch_forward_prep(hipStream_t s, const GpuLight* lights, uint32_t n
 hipError_t launch_forward_lit(hipStream_t s, const ForwardArgs& a);
 
 // spans: nspans (first item, view) pairs in item order, then (num_items, 0).
+// queue: kShadeQueueWords claim counters, zero before the first launch; each
+// launch leaves them zero (its last block out clears them).
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
                               const uint2* spans, uint32_t nspans, uint32_t num_items, uint32_t* queue,
                               uint32_t grid_blocks);
+// Blocks of the batched shading kernel of (gbuf_set, mode) resident per CU.
+uint32_t shade_batch_blocks_per_cu(int gbuf_set, int mode);
 // Per-light FP64 gradient accumulators: kGradSlots slot-major copies of 4
 // doubles (dpos.xyz, dI) per light, summed by launch_light_grads_to_f32.
 constexpr uint32_t kGradSlots = 8;

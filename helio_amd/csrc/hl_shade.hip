@@ -754,6 +754,30 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_bat
         if constexpr (MODE == kModeFusedL2) fused_tile<FA, FN, HL_GRID_CLUSTERS>(ra, it - first, sm, *as);
         else shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(ra, it - first, sm, as);
     });
+    // The last block out clears the claim counters for the next launch, so a
+    // captured graph replays this launch unchanged.  Blocks that have finished
+    // claiming are counted in two levels so that no single address takes an
+    // atomic from every block (those serialise): word 1 of sub-queue g's line
+    // counts the blocks with blockIdx % kShadeQueues == g, and the last of
+    // them adds to word 2 of line 0, which counts the finished groups.  No
+    // fence: each claim's atomic has returned before these are issued.
+    __shared__ uint32_t last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = blockIdx.x % kShadeQueues;
+        const uint32_t group_blocks = (gridDim.x - g + kShadeQueues - 1u) / kShadeQueues;
+        const uint32_t groups = gridDim.x < kShadeQueues ? gridDim.x : kShadeQueues;
+        uint32_t l = 0u;
+        if (atomicAdd(queue + g * kShadeQueueStride + 1u, 1u) == group_blocks - 1u)
+            l = atomicAdd(queue + 2u, 1u) == groups - 1u ? 1u : 0u;
+        last = l;
+    }
+    __syncthreads();
+    if (last && threadIdx.x < kShadeQueues) {
+        queue[threadIdx.x * kShadeQueueStride] = 0u;
+        queue[threadIdx.x * kShadeQueueStride + 1u] = 0u;
+        if (threadIdx.x == 0) queue[2] = 0u;
+    }
 }
 
 template <int FA, int FN>
@@ -765,6 +789,23 @@ static void launch_batch_fmt(hipStream_t s, int mode, const BatchShadeView* view
     else
         k_shade_batch<FA, FN, kModeForward><<<grid_blocks, 256, 0, s>>>((ConstViews)views, spans, nspans, num_items,
                                                                         queue);
+}
+
+template <int FA, int FN>
+static uint32_t batch_blocks_per_cu(int mode) {
+    int n = 0;
+    const hipError_t e = mode == kModeFusedL2
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade_batch<FA, FN, kModeFusedL2>, 256, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade_batch<FA, FN, kModeForward>, 256, 0);
+    return e == hipSuccess && n > 0 ? (uint32_t)n : 1u;
+}
+
+uint32_t shade_batch_blocks_per_cu(int gbuf_set, int mode) {
+    switch (gbuf_set) {
+        case kSetRef: return batch_blocks_per_cu<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT>(mode);
+        case kSet16: return batch_blocks_per_cu<HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA16_FLOAT>(mode);
+        default: return batch_blocks_per_cu<HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA32_FLOAT>(mode);
+    }
 }
 
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
