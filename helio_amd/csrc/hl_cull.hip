@@ -27,6 +27,29 @@
 namespace hl {
 
 // ------------------------------------------------------------------- prep
+// The culling record of a light (view-space centre and range, cone terms).
+__device__ __forceinline__ CullLight cull_light(float4 pr, float4 dro, uint32_t type, float range,
+                                                const float* __restrict__ v) {
+    CullLight c;
+    c.vpos = make_float4(hl_x_mat_row(v, 0, pr.x, pr.y, pr.z, 1.0f), hl_x_mat_row(v, 1, pr.x, pr.y, pr.z, 1.0f),
+                         hl_x_mat_row(v, 2, pr.x, pr.y, pr.z, 1.0f), range);
+    c.cone = make_float4(hl_x_mat_row(v, 0, dro.x, dro.y, dro.z, 0.0f), hl_x_mat_row(v, 1, dro.x, dro.y, dro.z, 0.0f),
+                         hl_x_mat_row(v, 2, dro.x, dro.y, dro.z, 0.0f), type == HL_LIGHT_SPOT ? dro.w : kNoCone);
+    c.axis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    c.aux = c.axis;
+    if (c.cone.w != kNoCone && !hl_x_cone_light(c.cone, &c.axis, &c.aux.x)) c.cone.w = kNoCone;
+    return c;
+}
+
+__device__ __forceinline__ CullLight cull_light_at(const GpuLight* __restrict__ lights, uint32_t i,
+                                                   const GpuCameraUniforms* __restrict__ cam) {
+    const GpuLight* l = &lights[i];
+    const float4 pr = *reinterpret_cast<const float4*>(l->position_range);
+    const float4 dro = *reinterpret_cast<const float4*>(l->direction_outer);
+    const uint32_t type = l->light_type;
+    return cull_light(pr, dro, type, type == HL_LIGHT_DIRECTIONAL ? __builtin_inff() : pr.w, cam->view);
+}
+
 __device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
                                          const uint32_t* __restrict__ subset, uint32_t n,
                                          const GpuCameraUniforms* __restrict__ cam,
@@ -44,21 +67,7 @@ __device__ __forceinline__ void prep_one(const GpuLight* __restrict__ lights,
     if (hot)
         hot[i] = make_hot_light(pr.x, pr.y, pr.z, dro.x, dro.y, dro.z, ci, pr.w, dro.w, inner, type,
                                 shadow_word(i, type, l->shadow_index), true);
-    if (cull) {
-        const float* v = cam->view;
-        CullLight c;
-        c.vpos = make_float4(hl_x_mat_row(v, 0, pr.x, pr.y, pr.z, 1.0f),
-                             hl_x_mat_row(v, 1, pr.x, pr.y, pr.z, 1.0f),
-                             hl_x_mat_row(v, 2, pr.x, pr.y, pr.z, 1.0f), range);
-        c.cone = make_float4(hl_x_mat_row(v, 0, dro.x, dro.y, dro.z, 0.0f),
-                             hl_x_mat_row(v, 1, dro.x, dro.y, dro.z, 0.0f),
-                             hl_x_mat_row(v, 2, dro.x, dro.y, dro.z, 0.0f),
-                             type == HL_LIGHT_SPOT ? dro.w : kNoCone);
-        c.axis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        c.aux = c.axis;
-        if (c.cone.w != kNoCone && !hl_x_cone_light(c.cone, &c.axis, &c.aux.x)) c.cone.w = kNoCone;
-        cull[i] = c;
-    }
+    if (cull) cull[i] = cull_light(pr, dro, type, range, cam->view);
     if (flags) {
         uint32_t f = 0;
         // shadow_factor samples the atlas only below MAX_SHADOW_LIGHTS = 42
@@ -170,19 +179,16 @@ __device__ __forceinline__ bool column_rect(const ClusterArgs& a, uint32_t cx, u
 // unprojected at the slice's padded near and far distances); the slice-0
 // thread of each column also writes the column's lateral planes (the tile
 // cull's construction on the column's pixel rectangle).
-__device__ __forceinline__ void geom_cell(const ClusterArgs& a,
-                                          const GpuCameraUniforms* __restrict__ cam,
-                                          ColumnGeom* __restrict__ geom,
-                                          CellBox* __restrict__ boxes, uint32_t cell) {
+// The records of cell (col, k): its view-space box, and with planes its
+// column's record.
+__device__ __forceinline__ void cell_geometry(const ClusterArgs& a, const GpuCameraUniforms* __restrict__ cam,
+                                              uint32_t col, uint32_t k, bool planes, ColumnGeom& g,
+                                              CellBox& bx) {
 #pragma clang fp contract(off)
-    const uint32_t ncol = a.grid_x * a.grid_y;
-    if (cell >= ncol * a.grid_z) return;
-    const uint32_t col = cell % ncol, k = cell / ncol;
     const uint32_t cx = col % a.grid_x, cy = col / a.grid_x;
     float l = 0.0f, r = 0.0f, b = 0.0f, t = 0.0f;
     const bool on = column_rect(a, cx, cy, &l, &r, &b, &t);
     const float p00 = cam->proj[0], p11 = cam->proj[5];
-    CellBox bx;
     if (on) {
         const float zn = a.slice_near[k], zf = a.slice_far[k];
         const float x0 = l * zn / p00, x1 = r * zn / p00, x2 = l * zf / p00, x3 = r * zf / p00;
@@ -196,9 +202,7 @@ __device__ __forceinline__ void geom_cell(const ClusterArgs& a,
     } else {
         bx.lo[0] = bx.lo[1] = bx.lo[2] = bx.hi[0] = bx.hi[1] = bx.hi[2] = 0.0f;
     }
-    boxes[cell] = bx;
-    if (k != 0) return;
-    ColumnGeom g;
+    if (!planes) return;
     g.on = on ? 1u : 0u;
     g.pad[0] = g.pad[1] = g.pad[2] = 0;
     g.l = l;
@@ -213,7 +217,20 @@ __device__ __forceinline__ void geom_cell(const ClusterArgs& a,
     } else {
         for (int i = 0; i < 4; ++i) g.pl[i] = make_float3(0.0f, 0.0f, 0.0f);
     }
-    geom[col] = g;
+}
+
+__device__ __forceinline__ void geom_cell(const ClusterArgs& a,
+                                          const GpuCameraUniforms* __restrict__ cam,
+                                          ColumnGeom* __restrict__ geom,
+                                          CellBox* __restrict__ boxes, uint32_t cell) {
+    const uint32_t ncol = a.grid_x * a.grid_y;
+    if (cell >= ncol * a.grid_z) return;
+    const uint32_t col = cell % ncol, k = cell / ncol;
+    ColumnGeom g;
+    CellBox bx;
+    cell_geometry(a, cam, col, k, k == 0, g, bx);
+    boxes[cell] = bx;
+    if (k == 0) geom[col] = g;
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -845,7 +862,155 @@ __device__ __forceinline__ CellBox* view_boxes(const BatchCullView& v) {
     return (CellBox*)((uint8_t*)v.geom + cluster_geom_offset_boxes_dev(v.ca.grid_x * v.ca.grid_y));
 }
 
+// ------------------------------------------------- clusters, brute force
+// Small problems (cells x lights <= kBruteTests) take two launches instead of
+// the light-parallel chain: every cell tests every light with the membership
+// rule (cell_member), one wave per cell and four cells per block, the
+// lights' culling records built per block into LDS kBfLights at a time, each
+// cell's records built by its own wave in registers.  k_cluster_bf_count
+// stores each cell's member count (and the lights' shading records);
+// k_cluster_bf_fill finds each cell's offset from the capped counts before
+// it, then writes the ascending list.  Membership does not depend on the
+// order of evaluation, so the lists are the light-parallel path's.
+constexpr uint64_t kBruteTests = 1ull << 22;
+constexpr uint32_t kBfCells = 4;  // one per wave
+constexpr uint32_t kBfLights = 256;
+
+struct BfSmem {
+    CullLight light[kBfLights];
+    uint32_t capped[kBfCells];
+    unsigned long long part[4];
+    uint32_t flags;
+};
+
+// The records of the wave's cell (every lane alike).
+struct BfCell {
+    ColumnGeom g;
+    CellBox bx;
+    float pad;
+};
+__device__ __forceinline__ BfCell bf_cell(const ClusterArgs& a, const GpuCameraUniforms* __restrict__ cam,
+                                          uint32_t cell, uint32_t ncell) {
+    BfCell c;
+    const uint32_t ncol = a.grid_x * a.grid_y;
+    if (cell < ncell) {
+        cell_geometry(a, cam, cell % ncol, cell / ncol, true, c.g, c.bx);
+        c.pad = a.slice_r_pad[cell / ncol];
+    } else {  // no cell: no member (the records are not read)
+        c.g = ColumnGeom{};
+        c.bx = CellBox{};
+        c.pad = 0.0f;
+    }
+    return c;
+}
+
+// For each light of the LDS chunk [l0, l0 + kBfLights) in ascending order, in
+// groups of 64: f(group, pass) with pass this lane's light's membership in
+// the wave's cell (uniform calls).
+template <class F>
+__device__ __forceinline__ void bf_chunk(const ClusterArgs& a, uint32_t l0, const BfCell& c, const BfSmem& sm,
+                                         F&& f) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nl = a.num_lights - l0 < kBfLights ? a.num_lights - l0 : kBfLights;
+    for (uint32_t j0 = 0; j0 < nl; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        const bool pass = c.g.on != 0u && j < nl && cell_member(sm.light[j], c.g, c.bx, c.pad);
+        f(j0, pass);
+    }
+}
+
+__device__ __forceinline__ void bf_load_lights(const ClusterArgs& a, const GpuCameraUniforms* __restrict__ cam,
+                                               const GpuLight* __restrict__ lights, uint32_t l0, BfSmem& sm) {
+    __syncthreads();  // the previous chunk is no longer read
+    const uint32_t i = l0 + threadIdx.x;
+    if (i < a.num_lights) sm.light[threadIdx.x] = cull_light_at(lights, i, cam);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_cluster_bf_count(ClusterArgs a, const GpuCameraUniforms* __restrict__ cam,
+                                                          const GpuLight* __restrict__ lights,
+                                                          HotLight* __restrict__ hot,
+                                                          uint32_t* __restrict__ feature_flags,
+                                                          uint32_t* __restrict__ raw, uint32_t* __restrict__ flags,
+                                                          uint32_t* __restrict__ flags2) {
+    __shared__ BfSmem sm;
+    const uint32_t ncell = a.grid_x * a.grid_y * a.grid_z;
+    const uint32_t cell = blockIdx.x * kBfCells + (threadIdx.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // OR-ed by the fill launch
+        *flags = 0u;
+        if (flags2) *flags2 = 0u;
+    }
+    // the shading records, grid-stride over the lights
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.num_lights; i += gridDim.x * blockDim.x)
+        prep_one(lights, nullptr, a.num_lights, cam, hot, nullptr, feature_flags, i);
+    const BfCell c = bf_cell(a, cam, cell, ncell);
+    uint32_t cnt = 0u;
+    for (uint32_t l0 = 0; l0 < a.num_lights; l0 += kBfLights) {
+        bf_load_lights(a, cam, lights, l0, sm);
+        bf_chunk(a, l0, c, sm, [&](uint32_t, bool pass) { cnt += (uint32_t)__popcll(hl_ballot(pass)); });
+    }
+    if ((threadIdx.x & 63u) == 0 && cell < ncell) raw[cell] = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_cluster_bf_fill(ClusterArgs a, const GpuCameraUniforms* __restrict__ cam,
+                                                         const GpuLight* __restrict__ lights,
+                                                         const uint32_t* __restrict__ raw, uint32_t pool_cap,
+                                                         uint32_t* __restrict__ offsets, uint32_t* __restrict__ counts,
+                                                         uint32_t* __restrict__ pool, uint32_t* __restrict__ flags,
+                                                         uint32_t* __restrict__ flags2) {
+    __shared__ BfSmem sm;
+    const uint32_t ncell = a.grid_x * a.grid_y * a.grid_z, first = blockIdx.x * kBfCells;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t cell = first + wave, cap = a.cap;
+    // capped counts of the cells before the block
+    unsigned long long acc = 0;
+    for (uint32_t j = t; j < first; j += blockDim.x) {
+        const uint32_t v = raw[j];
+        acc += (cap && v > cap) ? cap : v;
+    }
+    acc = wave_sum_u64(acc);
+    if (lane == 0) sm.part[wave] = acc;
+    if (t == 0) sm.flags = 0u;
+    if (t < kBfCells) {
+        const uint32_t v = first + t < ncell ? raw[first + t] : 0u;
+        sm.capped[t] = (cap && v > cap) ? cap : v;
+    }
+    const BfCell c = bf_cell(a, cam, cell, ncell);
+    __syncthreads();
+    // the wave's cell: offset, count after the cap and the pool truncation
+    unsigned long long base = sm.part[0] + sm.part[1] + sm.part[2] + sm.part[3];
+    for (uint32_t j = 0; j < wave; ++j) base += sm.capped[j];
+    const uint32_t o = base > pool_cap ? pool_cap : (uint32_t)base;
+    uint32_t cnt = sm.capped[wave], fl = 0u;
+    if (cell < ncell && cap && raw[cell] > cap) fl |= HL_OVERFLOW_CAP;
+    if (cnt > pool_cap - o) {
+        cnt = pool_cap - o;
+        fl |= HL_OVERFLOW_POOL;
+    }
+    if (lane == 0 && cell < ncell) {
+        offsets[cell] = o;
+        counts[cell] = cnt;
+        if (fl) atomicOr(&sm.flags, fl);
+    }
+    uint32_t written = 0u;
+    for (uint32_t l0 = 0; l0 < a.num_lights; l0 += kBfLights) {
+        bf_load_lights(a, cam, lights, l0, sm);
+        bf_chunk(a, l0, c, sm, [&](uint32_t j0, bool pass) {
+            const uint64_t m = hl_ballot(pass);
+            const uint32_t pos = written + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (pass && pos < cnt) pool[o + pos] = l0 + j0 + lane;
+            written += (uint32_t)__popcll(m);
+        });
+    }
+    __syncthreads();
+    if (t == 0 && sm.flags) {
+        atomicOr(flags, sm.flags);
+        if (flags2) atomicOr(flags2, sm.flags);
+    }
+}
+
 // Cells of every view, then (blocks past geom_blocks) the lights of every
+
 // view gathered from the pool, in one launch.
 __global__ __launch_bounds__(256) void k_batch_geom(const BatchCullView* __restrict__ views,
                                                     const uint32_t* __restrict__ cell_base,
@@ -1193,6 +1358,15 @@ hipError_t launch_cluster_cull(hipStream_t s, const ClusterArgs& a, const GpuCam
                                uint32_t* flags, uint32_t* flags2) {
     const uint32_t ncol = a.grid_x * a.grid_y;
     const uint32_t ncell = ncol * a.grid_z;
+    if ((uint64_t)ncell * a.num_lights <= kBruteTests) {
+        const uint32_t blocks = (ncell + kBfCells - 1u) / kBfCells;
+        k_cluster_bf_count<<<blocks, 256, 0, s>>>(a, cam, lights, hot, feature_flags, w.raw, flags, flags2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        k_cluster_bf_fill<<<blocks, 256, 0, s>>>(a, cam, lights, w.raw, pool_cap, offsets, counts, pool, flags,
+                                                 flags2);
+        return hipGetLastError();
+    }
     const uint32_t W = (a.num_lights + 31u) / 32u, SW = (W + 31u) / 32u;
     ColumnGeom* geom = (ColumnGeom*)w.geom;
     CellBox* boxes = (CellBox*)((uint8_t*)w.geom + cluster_geom_offset_boxes(ncol));

@@ -115,12 +115,16 @@ def hip_clusters(ctx, sc, params, pool_cap):
     return ds, cull, off, cnt, pool, cull.overflow()
 
 
-@pytest.mark.parametrize("name,pool_cap", [("c1_720p_64", 1 << 20), ("c2_1080p_1k", 1 << 21),
-                                           ("c3_4k_16k", 1 << 25), ("c3_4k_16k_far", 1 << 25)])
-def test_cluster_cull_bit_exact(ctx, hlib, name, pool_cap):
+# Grids of at most 2^22 cells x lights take the brute-force launches, larger
+# ones the light-parallel chain: the 1080p/1k scene runs on both.
+@pytest.mark.parametrize("name,pool_cap,grid", [("c1_720p_64", 1 << 20, None), ("c2_1080p_1k", 1 << 21, None),
+                                                ("c2_1080p_1k", 1 << 22, (48, 27, 24)),
+                                                ("c4_512_adj", 1 << 20, None),
+                                                ("c3_4k_16k", 1 << 25, None), ("c3_4k_16k_far", 1 << 25, None)])
+def test_cluster_cull_bit_exact(ctx, hlib, name, pool_cap, grid):
     sc = scene(name)
-    print(f"[cluster cull] {name}", flush=True)
-    params = synth.cluster_params(hlib, sc)
+    print(f"[cluster cull] {name} grid={grid}", flush=True)
+    params = synth.cluster_params(hlib, sc) if grid is None else synth.cluster_params(hlib, sc, grid=grid)
     _, _, off, cnt, pool, flags = hip_clusters(ctx, sc, params, pool_cap)
     ooff, ocnt, opool, oflags = O.cull_clusters(sc, params, pool_cap)
     assert flags == oflags
@@ -135,9 +139,12 @@ def test_cluster_cull_bit_exact(ctx, hlib, name, pool_cap):
         assert int(ocnt.max()) >= 2000  # the long-list regime
 
 
-def test_cluster_cap_and_pool_truncation(ctx, hlib):
+@pytest.mark.parametrize("grid", [None, (48, 27, 24)])
+def test_cluster_cap_and_pool_truncation(ctx, hlib, grid):
     sc = scene("c2_1080p_1k")
-    params = synth.cluster_params(hlib, sc, cap=32)
+    print(f"[cluster truncation] grid={grid}", flush=True)
+    params = (synth.cluster_params(hlib, sc, cap=32) if grid is None
+              else synth.cluster_params(hlib, sc, grid=grid, cap=32))
     _, _, off, cnt, pool, flags = hip_clusters(ctx, sc, params, 5000)
     ooff, ocnt, opool, oflags = O.cull_clusters(sc, params, 5000)
     assert flags == oflags and (flags & abi.HL_OVERFLOW_CAP) and (flags & abi.HL_OVERFLOW_POOL)
