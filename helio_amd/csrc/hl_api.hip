@@ -265,10 +265,12 @@ namespace {
 // Shared set-up of the shading entry points: arguments, then the light records
 // (built here, or with reuse those the context's last light cull built from
 // the same lights).
+// defer_prep: the records are built by the caller's own launch (into
+// c->hot, for these lights) instead of launch_prep here.
 hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const DeferredGlobals* g,
                         const hl_gbuffer* gb, const GpuLight* lights_d, uint32_t num_lights,
                         const hl_light_grid* grid, ShadeArgs* a, int* set, bool need_debug0,
-                        bool reuse = false) {
+                        bool reuse = false, bool defer_prep = false) {
     if (!c) return HL_EINVAL;
     if (num_lights && !lights_d) return fail(c, HL_EINVAL, "lights are null");
     hl_status s = build_shade_args(c, camera_d, g, gb, num_lights, grid, a, set, need_debug0);
@@ -285,8 +287,9 @@ hl_status prepare_shade(hl_context* c, const GpuCameraUniforms* camera_d, const 
     c->hot_src = nullptr;  // set again once the records are rebuilt
     if ((s = ensure(c, c->hot, sizeof(HotLight) * (size_t)(num_lights ? num_lights : 1))) != HL_OK) return s;
     a->lights = (const HotLight*)c->hot.p;
-    HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, num_lights, camera_d, (HotLight*)c->hot.p,
-                          nullptr, ctx_words(c) + 1));
+    if (!defer_prep)
+        HL_HIP(c, launch_prep(c->stream, lights_d, nullptr, num_lights, camera_d, (HotLight*)c->hot.p,
+                              nullptr, ctx_words(c) + 1));
     c->hot_src = lights_d;
     c->hot_n = num_lights;
     return HL_OK;
@@ -902,8 +905,10 @@ hl_status run_adjoint(hl_context* c, const GpuCameraUniforms* camera_d, const De
     if (!c) return HL_EINVAL;
     ShadeArgs a;
     int set;
-    hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, n, grid, &a, &set, true);
+    // the records are built below, with the accumulators cleared, in one launch
+    hl_status s = prepare_shade(c, camera_d, g, gb, lights_d, n, grid, &a, &set, true, false, true);
     if (s != HL_OK) return s;
+    c->hot_src = nullptr;  // set again once the launch below is queued
     if ((s = check_rgba32(c, in_d, in_pitch, gb->width, fused ? "target" : "dL_dout")) != HL_OK) return s;
     if (fused) {
         if ((s = check_rgba32(c, out_d, out_pitch, gb->width, "output")) != HL_OK) return s;
@@ -913,8 +918,10 @@ hl_status run_adjoint(hl_context* c, const GpuCameraUniforms* camera_d, const De
     if (n && (!d_pos_d || !d_int_d)) return fail(c, HL_EINVAL, "light gradient outputs are null");
     if ((s = ensure(c, c->accum, sizeof(double) * grad_accum_doubles(n))) != HL_OK) return s;
     double* acc = (double*)c->accum.p;
-    HL_HIP(c, launch_zero_f64(c->stream, acc, grad_accum_doubles(n)));
-    if (fused) HL_HIP(c, launch_zero_f64(c->stream, loss_d, 1));
+    HL_HIP(c, launch_prep_zero(c->stream, lights_d, n, camera_d, (HotLight*)c->hot.p, ctx_words(c) + 1, acc,
+                               grad_accum_doubles(n), fused ? loss_d : nullptr));
+    c->hot_src = lights_d;
+    c->hot_n = n;
     a.dout = in_d;
     a.dout_pitch = in_pitch;
     a.out = out_d;

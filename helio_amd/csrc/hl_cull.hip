@@ -88,6 +88,19 @@ __global__ __launch_bounds__(256) void k_prep_lights(const GpuLight* __restrict_
     prep_one(lights, subset, n, cam, hot, cull, flags, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// The adjoint's set-up in one launch: the shading records, the per-light
+// gradient accumulators cleared, and the loss (when given) cleared.
+__global__ __launch_bounds__(256) void k_prep_zero(const GpuLight* __restrict__ lights, uint32_t n,
+                                                   const GpuCameraUniforms* __restrict__ cam,
+                                                   HotLight* __restrict__ hot, uint32_t* __restrict__ flags,
+                                                   double* __restrict__ acc, size_t acc_n,
+                                                   double* __restrict__ loss) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    prep_one(lights, nullptr, n, cam, hot, nullptr, flags, (uint32_t)(i < n ? i : n));
+    for (size_t j = i; j < acc_n; j += (size_t)gridDim.x * blockDim.x) acc[j] = 0.0;
+    if (i == 0 && loss) *loss = 0.0;
+}
+
 // -------------------------------------------------------------- tile cull
 // The four planes of a 16x16 tile (light_cull.wgsl:123-150).
 __device__ __forceinline__ void tile_planes(const GpuCameraUniforms* __restrict__ cam, const LightCullParams& p,
@@ -1303,6 +1316,18 @@ hipError_t launch_prep(hipStream_t s, const GpuLight* lights, const uint32_t* su
                        uint32_t* flags) {
     if (n == 0) return hipSuccess;
     k_prep_lights<<<(n + 255) / 256, 256, 0, s>>>(lights, subset, n, cam, hot, cull, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_zero(hipStream_t s, const GpuLight* lights, uint32_t n, const GpuCameraUniforms* cam,
+                            HotLight* hot, uint32_t* flags, double* acc, size_t acc_n, double* loss) {
+    // one thread per light (prep_one), the accumulators grid-stride
+    const size_t prep_blocks = ((size_t)n + 255) / 256;
+    size_t zero_blocks = (acc_n + 255) / 256;
+    if (zero_blocks > 4096) zero_blocks = 4096;
+    size_t blocks = prep_blocks > zero_blocks ? prep_blocks : zero_blocks;
+    if (blocks == 0) blocks = 1;
+    k_prep_zero<<<(unsigned)blocks, 256, 0, s>>>(lights, n, cam, hot, flags, acc, acc_n, loss);
     return hipGetLastError();
 }
 

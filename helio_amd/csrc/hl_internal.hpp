@@ -80,6 +80,10 @@ hl_status build_shade_args(hl_context* c, const GpuCameraUniforms* camera_d,
 hipError_t launch_prep(hipStream_t s, const GpuLight* lights, const uint32_t* subset, uint32_t n,
                        const GpuCameraUniforms* cam, HotLight* hot, CullLight* cull,
                        uint32_t* flags);
+// The adjoint's set-up: launch_prep's shading records (no subset, no
+// culling records) with acc[0, acc_n) and *loss (when not null) cleared.
+hipError_t launch_prep_zero(hipStream_t s, const GpuLight* lights, uint32_t n, const GpuCameraUniforms* cam,
+                            HotLight* hot, uint32_t* flags, double* acc, size_t acc_n, double* loss);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
 hipError_t launch_cull_tiles(hipStream_t s, const GpuCameraUniforms* cam, const LightCullParams& p,
                              const CullLight* cl, uint32_t* lists, uint32_t* counts);
