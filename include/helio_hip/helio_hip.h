@@ -533,8 +533,10 @@ HL_API hl_status hl_forward_lit_count_pairs(hl_context* ctx, const GpuCameraUnif
  * differentiated forward resolves F0 from albedo and metallic the way the
  * G-buffer producer does (F0 = clamp(mix(0.04, albedo, metallic), 0, 0.999),
  * crates/passes/3d/helio-pass-gbuffer/shaders/gbuffer.wgsl:324-328) instead of
- * reading the stored F0 channels; with an fp32 G-buffer written by that
- * producer both forwards agree bit for bit.  The no-contribution cut-offs
+ * reading the stored F0 channels (an 8-bit orm.w stores a dielectric's 0.04 as
+ * 10/255, 2 % lower); with an fp32 G-buffer written by that producer both
+ * forwards agree up to the order of the sums (the fused forward adds four
+ * partial sums per pixel: within 1e-6 relative).  The no-contribution cut-offs
  * (dist > range, NdL == 0, all(radiance < 0.002)) are treated as fixed.
  * dL_dout_d: RGBA32F per pixel (alpha ignored), pitch dout_pitch_bytes.
  * Per-light outputs are overwritten: d_light_pos_d[3*i + k], d_light_intensity_d[i].

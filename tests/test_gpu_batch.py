@@ -146,6 +146,68 @@ def test_graph_replays_match_eager(ctx, mv):
         plan.destroy()
 
 
+def metallic_f0_scene(sc):
+    """A copy of sc with its G-buffer decoded to RGBA32F and the specular F0
+    channels (normal.w, orm.w, emissive.w) set to the metallic-workflow F0
+    the adjoint modes differentiate, clamp(0.04 (1 - metallic) + albedo
+    metallic, 0, 0.999): the forward they compute.  (The forward reads F0 as
+    stored: an RGBA8 orm.w holds a dielectric's 0.04 as 10/255.)"""
+    def dec(a, fmt):
+        if fmt == abi.HL_FMT_RGBA8_UNORM:
+            return a.astype(np.float32) / np.float32(255.0)
+        if fmt == abi.HL_FMT_RGBA16_FLOAT:
+            return a.view(np.float16).astype(np.float32)
+        return a.view(np.float32).copy()
+    fa, _, fn, _ = sc.formats
+    alb, orm = dec(sc.albedo, fa), dec(sc.orm, fa)
+    nrm, emi = dec(sc.normal, fn), dec(sc.emissive, fn)
+    m = orm[..., 2]
+    f0 = [np.clip(np.float32(0.04) * (np.float32(1.0) - m) + alb[..., c] * m, np.float32(0.0), np.float32(0.999))
+          for c in range(3)]
+    nrm[..., 3], orm[..., 3], emi[..., 3] = f0
+    f32 = abi.HL_FMT_RGBA32_FLOAT
+    return dataclasses.replace(sc, albedo=np.ascontiguousarray(alb), orm=np.ascontiguousarray(orm),
+                               normal=np.ascontiguousarray(nrm), emissive=np.ascontiguousarray(emi),
+                               formats=(f32, 16, f32, 16))
+
+
+def test_batch_views_against_the_oracle(ctx, hlib, mv):
+    """Batched views against the oracle itself (not only the single-view
+    path): probe faces of several light counts and both eyes, each against
+    the oracle's own cluster cull and FP32 shading on the plan's grid --
+    radiance within 1e-4 relative + 1e-6; the eyes' fused output against the
+    forward on the metallic-workflow F0 it differentiates, and their L2 loss
+    against that radiance's within 1e-4 relative."""
+    import oracle_lib as O
+    from test_gpu_parity import check_radiance
+    pool, pool_size, views, hosts, _ = mv
+    print("[batch] views against the oracle", flush=True)
+    pick = sorted({0, 16, 31, 47} | {i for i, v in enumerate(views) if v.adjoint})
+    plan = BatchPlan(ctx, pool, pool_size, views)
+    try:
+        for i in pick:
+            views[i].out.fill_u32(0)
+        plan.launch()
+        ctx.synchronize()
+        for i in pick:
+            v, (sc, sub) = views[i], hosts[i]
+            cfg = plan.config(i)
+            params = synth.cluster_params(hlib, sc, grid=(cfg.grid_x, cfg.grid_y, cfg.grid_z), cap=0)
+            off, cnt, opool, _ = O.cull_clusters(sc, params, 1 << 22)
+            ref, _ = O.shade(metallic_f0_scene(sc) if v.adjoint else sc,
+                             O.cluster_grid_host(params, off, cnt, opool))
+            got = read_view(v, pool_size, sub)
+            sky = sc.depth >= 1.0
+            r = check_radiance(got["out"], ref, ~sky)
+            print(f"  view {i} ({sc.width}x{sc.height}, {len(sub)} lights): max rel {r:.2e}")
+            if v.adjoint:
+                e = ref[..., :3].astype(np.float64) - 0.25
+                oloss = float((e[~sky] ** 2).sum())
+                assert abs(got["loss"] - oloss) <= 1e-4 * oloss, (got["loss"], oloss)
+    finally:
+        plan.destroy()
+
+
 def test_autotuner_and_cache(ctx, mv, tmp_path):
     pool, pool_size, views, hosts, _ = mv
     print("[batch] autotune", flush=True)
