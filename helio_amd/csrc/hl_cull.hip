@@ -499,7 +499,12 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
         uint32_t base = 0;
         if ((threadIdx.x & 63u) == 0) base = atomicAdd(q.count + sub * kMarkCountStride, extra);
         base = hl_readlane(base, 0);
-        const uint32_t fit = base >= q.seg ? 0u : min(extra, q.seg - base);
+        // room left in the sub-queue, saturating at zero: written as a select
+        // over a wrapping difference (base >= seg ? 0 : seg - base), -O3 folded
+        // the select away once it had marked the difference no-wrap, and an
+        // overflowing light then took fit = extra and wrote past its segment
+        const uint32_t room = __builtin_elementwise_sub_sat(q.seg, base);
+        const uint32_t fit = min(extra, room);
         if (fit) {
             if ((threadIdx.x & 63u) == 0) q.bounds[qlight] = pack_box(m);
             uint2* seg = q.entries + (size_t)sub * q.seg + base;

@@ -326,3 +326,69 @@ def test_full_size_c5(ctx, hlib):
                 assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
     finally:
         plan.destroy()
+
+
+def test_full_size_c5_in_accepted_plans(ctx, hlib):
+    """Configuration 5 at full size as the plans hl_batch_plan_create accepts:
+    23 plans (the eyes and the 512^2 faces alone, the 256^2 faces four to a
+    plan, the 128^2 faces in one), every view against the single-view path
+    as in test_batch_equals_single_view (and the eyes' loss and gradients),
+    one face of each size against the oracle in full, and three bands of
+    rows of both eyes against the oracle (on the metallic-workflow F0 the
+    fused adjoint differentiates)."""
+    import oracle_lib as O
+    from helio_amd.batch import plan_groups
+    from test_gpu_parity import check_radiance
+    print("[batch] full size in accepted plans", flush=True)
+    pool, pool_size, views, hosts, _ = make_multiview(ctx, small=False)
+    for v in views:
+        if v.adjoint:
+            t = np.zeros((v.scene.host.height, v.scene.host.width, 4), np.float32)
+            t[..., :3] = 0.25
+            v.target.upload(t)
+    groups = plan_groups(views)
+    assert len(groups) == 23
+    sizes_seen = set()
+    for gi, g in enumerate(groups):
+        print(f"  plan {gi}: views {g}", flush=True)
+        sel = [views[i] for i in g]
+        plan = BatchPlan(ctx, pool, pool_size, sel)
+        try:
+            plan.launch()
+            ctx.synchronize()
+            for k, i in enumerate(g):
+                v, (sc, sub) = views[i], hosts[i]
+                cfg = plan.config(k)
+                got = read_view(v, pool_size, sub)
+                ref = single_view(ctx, hlib, v, cfg)
+                sky = sc.depth >= 1.0
+                if v.adjoint and (cfg.wave_tile_w, cfg.wave_tile_h) != (8, 8):
+                    assert close(got["out"][~sky], ref["out"][~sky], rel=1e-6), f"view {i}"
+                else:
+                    assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i}"
+                if v.adjoint:
+                    assert abs(got["loss"] - ref["loss"]) <= 1e-6 * abs(ref["loss"]), f"view {i} loss"
+                    assert close(got["dpx"], ref["dpx"]), f"view {i} pixel grads"
+                    assert close(got["dint"], ref["dint"]), f"view {i} intensity grads"
+                size = (sc.width, sc.height)
+                if size in sizes_seen:
+                    continue
+                sizes_seen.add(size)
+                params = synth.cluster_params(hlib, sc, grid=(cfg.grid_x, cfg.grid_y, cfg.grid_z), cap=0)
+                off, cnt, opool, _ = O.cull_clusters(sc, params, 1 << 24)
+                grid = O.cluster_grid_host(params, off, cnt, opool)
+                if v.adjoint:
+                    osc = metallic_f0_scene(sc)
+                    for r0 in (0, sc.height // 2, sc.height - 32):
+                        oref, _ = O.shade(osc, grid, rows=(r0, r0 + 32))
+                        m = np.zeros_like(sky)
+                        m[r0:r0 + 32] = True
+                        r = check_radiance(got["out"], oref, m & ~sky)
+                        print(f"  eye {i} rows {r0}..{r0 + 32}: max rel {r:.2e}", flush=True)
+                else:
+                    oref, _ = O.shade(sc, grid)
+                    r = check_radiance(got["out"], oref, ~sky)
+                    print(f"  face {i} {size}: max rel {r:.2e}", flush=True)
+        finally:
+            plan.destroy()
+    assert len(sizes_seen) == 4

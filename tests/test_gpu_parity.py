@@ -7,6 +7,7 @@ Bars (stated per check):
   * Q (pixel-light pairs past the range test): equal.
 """
 import ctypes as C
+from dataclasses import replace
 
 import numpy as np
 import pytest
@@ -137,6 +138,31 @@ def test_cluster_cull_bit_exact(ctx, hlib, name, pool_cap, grid):
     print(f"  clusters={len(cnt)} entries={total} max={int(ocnt.max())} flags={flags}")
     if name == "c3_4k_16k_far":
         assert int(ocnt.max()) >= 2000  # the long-list regime
+
+
+def test_cluster_cull_mark_queue_overflow(ctx, hlib):
+    """128 large lights in front of the camera on a 64x36x24 grid (light-parallel
+    path): the mark kernel queues each light's chunks past the first two in
+    sub-queue (light % 64) of (4 L + 4096) / 64 entries, and here both lights of
+    at least 32 sub-queues have more queued chunks than that, so the second
+    to claim finds its sub-queue already full and must mark its chunks itself."""
+    s = replace(synth.SPECS["c2_1080p_1k"], name="c2_overflow", num_lights=128,
+                light_box_center=(0.0, 10.0, 40.0), light_box_size=(20.0, 6.0, 10.0),
+                range_min=12.0, range_max=16.0)
+    sc = synth.make_scene(s)
+    params = synth.cluster_params(hlib, sc, grid=(64, 36, 24))
+    print("[cluster cull] mark queue overflow 1920x1080 grid=(64,36,24) lights=128", flush=True)
+    ooff, ocnt, opool, oflags = O.cull_clusters(sc, params, 1 << 22)
+    per = np.bincount(opool[:int(ocnt.sum())], minlength=128)
+    seg = (4 * 128 + 4096 + 63) // 64
+    full = sum(bool(per[i] > 64 * (seg + 2) and per[i + 64] > 64 * (seg + 2)) for i in range(64))
+    assert 64 * 36 * 24 * 128 > 1 << 22 and full >= 32
+    _, _, off, cnt, pool, flags = hip_clusters(ctx, sc, params, 1 << 22)
+    assert flags == oflags
+    assert np.array_equal(cnt, ocnt) and np.array_equal(off, ooff)
+    used = int(ooff[-1] + ocnt[-1])
+    assert np.array_equal(pool[:used], opool[:used])
+    print(f"  sub-queues over full={full} entries={used}")
 
 
 @pytest.mark.parametrize("grid", [None, (48, 27, 24)])
