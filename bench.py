@@ -34,9 +34,10 @@ Further legs, each reported in its own object of the same line:
   reflection_4k   the pass's second draw (hl_deferred_reflection) over the 4K
                   G-buffer: 16 reflection captures over 17 RGBA16F cubes of
                   128^2 x 8 mips, SSR and planar images, RC rough specular
-  multiview       the full-size 50-view set (48 probe faces + 2 XR eyes at
-                  2064x2208) is refused: multi-view plans are limited to
-                  HL_BATCH_MAX_MULTIVIEW_PIXELS pixels (--multiview exits 2)
+  multiview       the full-size 50-view set (48 probe faces + 2
+                  XR eyes at 2064x2208, the eyes with the L2 adjoint) in one
+                  autotuned batched plan, graph and eager, against the
+                  per-view path on the same grids
 """
 from __future__ import annotations
 
@@ -406,6 +407,61 @@ def bench_adjoint(ctx, args):
                         "position/intensity per light"}
 
 
+def bench_multiview(ctx, args):
+    """The full-size 50-view set (48 probe faces at 128^2, 256^2 and 512^2, two
+    2064x2208 XR eyes with the L2 adjoint; 8192-light pool, per-view light
+    subsets) in one autotuned batched plan (one launch chain, one captured
+    graph), against the per-view path the reference takes (each view culled
+    and shaded with its own calls: crates/helio/src/renderer/render.rs:804-873)."""
+    from helio_amd import synth
+    from helio_amd.batch import BatchPlan, make_multiview
+    from helio_amd.device import AdjointBuffers, DeferredLightPass, LightCullPass
+
+    pool, pool_size, views, hosts, _ = make_multiview(ctx, small=False)
+    for v in views:
+        if v.adjoint:
+            t = np.zeros((v.scene.host.height, v.scene.host.width, 4), np.float32)
+            t[..., :3] = 0.25
+            v.target.upload(t)
+    log("50 views full size: autotuning the plan")
+    plan = BatchPlan(ctx, pool, pool_size, views, autotune=True, max_trials=args.tune_trials)
+    try:
+        cfg = {i: plan.config(i) for i in range(len(views))}
+        eager, _ = time_steps(ctx, args.steps, args.warmup, plan.launch)
+        plan.capture()
+        graph, _ = time_steps(ctx, args.steps, args.warmup, plan.launch)
+    finally:
+        plan.destroy()
+    # the per-view path on the same cluster grids
+    calls = []
+    for i, v in enumerate(views):
+        sc, ds = v.scene.host, v.scene
+        c = cfg[i]
+        params = synth.cluster_params(ctx.lib, sc, grid=(c.grid_x, c.grid_y, c.grid_z), cap=0)
+        cull = LightCullPass(ctx, sc.width, sc.height, clusters=params, pool_cap=1 << 24)
+        shade = DeferredLightPass(ctx, sc.width, sc.height)
+        cull.prepare(ds)
+        shade.prepare(ds)
+        grid = cull.publish()
+        calls.append(lambda cull=cull, ds=ds: cull.execute(ds))
+        if v.adjoint:
+            grads = AdjointBuffers(ctx, sc.width, sc.height, v.num_lights)
+            calls.append(lambda shade=shade, ds=ds, grid=grid, t=v.target, g=grads:
+                         shade.forward_backward_l2(ds, grid, t, g))
+        else:
+            calls.append(lambda shade=shade, ds=ds, grid=grid: shade.execute(ds, grid))
+    per_view, _ = time_steps(ctx, args.steps, args.warmup, lambda: [f() for f in calls])
+    px = sum(v.scene.host.width * v.scene.host.height for v in views)
+    log(f"50 views full size: one plan graph {graph:.3f} ms, eager {eager:.3f} ms; per view {per_view:.3f} ms")
+    return {"workload": "48 probe faces (16 each at 128^2, 256^2, 512^2) + 2 eyes at 2064x2208 with the L2 "
+                        "adjoint, 8192-light pool, log-uniform subsets of 32..4096 lights, reference G-buffer "
+                        "formats", "views": len(views), "pixels": px, "plans": 1,
+            "ms_per_frame_graph": graph, "ms_per_frame_eager": eager, "ms_per_frame_per_view": per_view,
+            "speedup_vs_per_view": per_view / graph,
+            "eye_grids": [f"{cfg[i].grid_x}x{cfg[i].grid_y}x{cfg[i].grid_z}" for i, v in enumerate(views)
+                          if v.adjoint]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,22 +475,14 @@ def main() -> int:
     ap.add_argument("--no-shadows", action="store_true", help="skip the 4K shadowed leg")
     ap.add_argument("--no-reflection", action="store_true", help="skip the 4K reflection-composite leg")
     ap.add_argument("--no-far", action="store_true", help="skip the 4K far-camera (long cluster lists) leg")
-    ap.add_argument("--multiview", action="store_true",
-                    help="the full-size 50-view batched leg: refused (its plan hit a device memory fault "
-                         "once, cause not found; multi-view plans are limited to "
-                         "HL_BATCH_MAX_MULTIVIEW_PIXELS pixels)")
+    ap.add_argument("--no-multiview", action="store_true",
+                    help="skip the full-size 50-view leg (one autotuned batched plan against the per-view path)")
     ap.add_argument("--tune-trials", type=int, default=4)
     ap.add_argument("--workload", default="c3_4k_16k")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
-    if args.multiview:
-        print("bench.py --multiview is refused: the full-size 50-view plan (48 probe faces + 2 eyes at "
-              "2064x2208, 14.6 M pixels) hit a device memory fault once and its cause is not found, so "
-              "hl_batch_plan_create limits multi-view plans to HL_BATCH_MAX_MULTIVIEW_PIXELS pixels; the "
-              "1/8-scale 50-view plan is checked by tests/test_gpu_batch.py", file=sys.stderr)
-        return 2
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported; the path would shard by screen tiles and views)",
@@ -462,6 +510,7 @@ def main() -> int:
         fwd = None if args.no_forward else bench_forward_lit(ctx, args)
         shd = None if args.no_shadows else bench_shadowed(ctx, args)
         rfl = None if args.no_reflection else bench_reflection(ctx, args)
+        mv = None if args.no_multiview else bench_multiview(ctx, args)
 
     sc, q = r4k["scene"], r4k["q"]
     ms_frame, shade_ms = r4k["ms_frame"], r4k["shade_ms"]
@@ -529,10 +578,10 @@ def main() -> int:
         res["shadowed_4k"] = shd
     if rfl is not None:
         res["reflection_4k"] = rfl
-    res["multiview"] = {
-        "skipped": "full-size 50-view plan refused (HL_EUNSUPPORTED above HL_BATCH_MAX_MULTIVIEW_PIXELS): it "
-                   "hit a device memory fault once and its cause is not found; the 1/8-scale 50-view plan "
-                   "is checked by the GPU tests"}
+    if mv is not None:
+        res["multiview"] = mv
+    else:
+        res["multiview"] = {"skipped": "--no-multiview"}
     if not args.no_cpu:
         log("CPU baseline")
         res["cpu_baseline"] = cpu_baseline(sc, params, r4k["pool_cap"])

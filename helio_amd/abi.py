@@ -26,7 +26,7 @@ HL_SHADE_LIGHTS_FROM_CULL = 2
 HL_SHADE_OUT_RGBA16F = 4
 HL_OVERFLOW_CAP, HL_OVERFLOW_POOL = 1, 2
 HL_PIXEL_GRAD_PLANES = 5
-HL_BATCH_MAX_VIEWS, HL_BATCH_MAX_MULTIVIEW_PIXELS = 255, 262144
+HL_BATCH_MAX_VIEWS = 255
 HL_NOALLOC_MAX_LIGHTS = 65536
 HL_NOALLOC_SCRATCH_BYTES = 16384 + 80 * HL_NOALLOC_MAX_LIGHTS
 HL_EXTERNAL_FD, HL_EXTERNAL_MAP_ALIGN = 1, 256

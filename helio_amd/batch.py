@@ -109,27 +109,3 @@ def make_multiview(ctx: Context, small: bool = False, adjoint_eyes: bool = True)
         views.append(v)
         hosts.append((sc, sub))
     return pool, len(pool_lights), views, hosts, pool_lights
-
-
-def plan_groups(views: list[BatchView], limit: int = abi.HL_BATCH_MAX_MULTIVIEW_PIXELS) -> list[list[int]]:
-    """The views split into plans that hl_batch_plan_create accepts: a view
-    of more than `limit` pixels alone, the others packed first-fit, largest
-    first, into plans of at most `limit` pixels.  Returns lists of view
-    indices."""
-    px = [v.scene.host.width * v.scene.host.height for v in views]
-    groups: list[list[int]] = []
-    room: list[int] = []
-    for i in sorted(range(len(views)), key=lambda i: -px[i]):
-        if px[i] > limit:
-            groups.append([i])
-            room.append(-1)
-            continue
-        for g in range(len(groups)):
-            if room[g] >= px[i]:
-                groups[g].append(i)
-                room[g] -= px[i]
-                break
-        else:
-            groups.append([i])
-            room.append(limit - px[i])
-    return groups

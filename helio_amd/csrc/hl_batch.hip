@@ -465,15 +465,6 @@ HL_API hl_status hl_batch_plan_create(hl_context* c, const GpuLight* pool_d, uin
     *out = nullptr;
     if (!views || num_views == 0) return fail(c, HL_EINVAL, "no views");
     if (num_views > HL_BATCH_MAX_VIEWS) return fail(c, HL_EINVAL, "at most %u views per plan", HL_BATCH_MAX_VIEWS);
-    if (num_views > 1) {
-        uint64_t px = 0;
-        for (uint32_t v = 0; v < num_views; ++v) px += (uint64_t)views[v].gbuffer.width * views[v].gbuffer.height;
-        if (px > HL_BATCH_MAX_MULTIVIEW_PIXELS)
-            return fail(c, HL_EUNSUPPORTED,
-                        "multi-view plan of %llu pixels: plans of more than one view are limited to %u pixels "
-                        "in total (larger ones are not validated on the device)",
-                        (unsigned long long)px, HL_BATCH_MAX_MULTIVIEW_PIXELS);
-    }
     if (pool_size && !pool_d) return fail(c, HL_EINVAL, "light pool is null");
     HL_HIP(c, hipSetDevice(c->device));
     hl_plan* p = new hl_plan();

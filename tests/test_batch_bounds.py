@@ -12,10 +12,13 @@ checked against the plan's allocations (tests/batch_emu.py).  Both cell
 layouts are covered: views aligned to 256 cells, and packed back to back as
 the batched cull laid them out before the alignment.
 
-The same set hit a device memory fault once (hl_batch_plan_create, in the
-synchronisation after a plan's pool-sizing cull); these checks find no
-out-of-bounds access in either layout, so the plan stays limited to
-HL_BATCH_MAX_MULTIVIEW_PIXELS pixels until a cause is shown.
+These checks follow the kernels' source.  They could not see the mark
+queue's room check as the compiler had built it: written as a select over a
+wrapping difference, it was folded away at -O3, and a light whose sub-queue
+was full wrote its chunk entries past its segment (past the queue buffer for
+sub-queue 63).  The source now states the room as a saturating subtraction;
+tests/test_gpu_parity.py::test_cluster_cull_mark_queue_overflow runs the
+overflow on the device.
 """
 import numpy as np
 import pytest

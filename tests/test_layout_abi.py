@@ -163,7 +163,7 @@ def test_header_constants_match_python_mirror():
         if hasattr(abi, name):
             assert getattr(abi, name) == int(val, 0), name
             seen += 1
-    assert seen >= 12 and abi.HL_BATCH_MAX_MULTIVIEW_PIXELS == 262144
+    assert seen >= 12
     m = re.search(r"#define HL_NOALLOC_SCRATCH_BYTES \((\d+)u \+ (\d+)u \* HL_NOALLOC_MAX_LIGHTS\)", text)
     assert m and int(m.group(1)) + int(m.group(2)) * abi.HL_NOALLOC_MAX_LIGHTS == abi.HL_NOALLOC_SCRATCH_BYTES
 
