@@ -228,11 +228,22 @@ enum ShadeMode : int {
     // output target is RGBA16F (ShadeArgs::out_f16) instead of RGBA32F
     kModeOutF16 = 8
 };
-// Work items of a shading launch over num_blocks block tiles: the adjoint
-// modes split each wave tile's lights over a block's four waves, so their
-// items are wave tiles (four per block tile).
+// The fused forward + L2 adjoint shares a wave tile's lights among the block's
+// four waves on views of fewer than kFusedSplitBlocks block tiles: a 512^2
+// view has 1024 block tiles, 4096 waves of their own for 1024 SIMDs, and the
+// sharing gives it four times as many.  Larger views give each wave a tile of
+// its own (no partial sums across waves, no block barriers).  The rule
+// depends on the view's size only, so every path shading a view picks the
+// same split, and the same sums.
+constexpr uint32_t kFusedSplitBlocks = 4096;
+__host__ __device__ inline bool fused_split(uint32_t num_blocks) { return num_blocks < kFusedSplitBlocks; }
+
+// Work items of a shading launch over num_blocks block tiles: the backward
+// mode, and the fused mode when split, share each wave tile's lights among a
+// block's four waves, so their items are wave tiles (four per block tile).
 __host__ __device__ inline uint32_t shade_items(int mode, uint32_t num_blocks) {
-    return (mode == kModeBackward || mode == kModeFusedL2) ? 4u * num_blocks : num_blocks;
+    return (mode == kModeBackward || (mode == kModeFusedL2 && fused_split(num_blocks))) ? 4u * num_blocks
+                                                                                       : num_blocks;
 }
 
 // Claim counters of one persistent shading launch: kShadeQueues counters,

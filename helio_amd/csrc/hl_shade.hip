@@ -505,7 +505,7 @@ __device__ __forceinline__ void shade_tile(const ShadeArgs& a, uint32_t b, Shade
 // the forward and the adjoint alike; the waves' partial sums are added in
 // wave order, so the output depends on the tile shape at the last bits.
 template <int FA, int FN, int GRID>
-__device__ __forceinline__ void fused_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem& as) {
+__device__ __forceinline__ void fused_tile_split(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem& as) {
     constexpr int MODE = kModeFusedL2;
     const ViewConst& vc = sm.vc;
     b = hl_uniform(b);
@@ -608,6 +608,105 @@ __device__ __forceinline__ void fused_tile(const ShadeArgs& a, uint32_t b, Shade
         dp[4 * plane + pix] = g[4];
     }
     __syncthreads();  // red and gl are rewritten by the next item
+}
+
+// The same for block tile b with a wave tile per wave (views of at least
+// kFusedSplitBlocks block tiles): each wave walks its own tile's lights,
+// evaluates all of them in the forward (keeping them in LDS), and the same
+// ones in the adjoint.  Only wave-level synchronisation: a wave with no
+// geometry returns at once.
+template <int FA, int FN, int GRID>
+__device__ __forceinline__ void fused_tile_own(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem& as) {
+    constexpr int MODE = kModeFusedL2;
+    const ViewConst& vc = sm.vc;
+    b = hl_uniform(b);
+    const uint32_t lane = threadIdx.x & 63u, wave = hl_uniform(threadIdx.x >> 6);
+    const uint32_t twl = a.tw_log2, tw = 1u << twl, th = 64u >> twl;
+    const uint32_t bx = b % a.block_tiles_x, by = b / a.block_tiles_x;
+    const uint32_t x = bx * (4u * tw) + wave * tw + (lane & (tw - 1u));
+    const uint32_t y = by * th + (lane >> twl);
+    const bool inside = x < a.width && y < a.height;
+    const float depth = inside ? hl_load_depth(a.depth, x, y) : 1.0f;
+    const bool valid = inside && depth < 1.0f;
+    const size_t plane = (size_t)a.width * a.height;
+    const size_t pix = (size_t)y * a.width + x;
+    float4* out_px = inside ? (float4*)(reinterpret_cast<uint8_t*>(a.out) + (size_t)y * a.out_pitch) + x : nullptr;
+    if (inside && !valid) {
+        if (a.flags & HL_SHADE_CLEAR_SKY) hl_store_nt(out_px, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        for (int k = 0; k < HL_PIXEL_GRAD_PLANES; ++k) hl_g(a.d_pixel)[k * plane + pix] = 0.0f;
+    }
+    if (hl_ballot(valid) == 0) return;  // uniform over the wave
+
+    Pixel p;
+    uint32_t cell;
+    float4 A, E;
+    setup_pixel<FA, FN, GRID, MODE>(a, vc, x, y, depth, valid, p, cell, A, E);
+    float mixv[3], amb[3];
+    ambient_terms(a, p, E, mixv, amb);
+
+    // ---- forward: every light of the tile, kept in LDS for the adjoint
+    HotLight* s_l = sm.light[wave];
+    KeptLights& kept = as.kept[wave];
+    KeptState ks;
+    float lo[3] = {0.0f, 0.0f, 0.0f};
+    walk_lights<GRID, true>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t, bool mine) {
+        if (!mine) return;
+        Term t;
+        if (eval_term<false>(L, p, t)) {
+            const float aw = t.atten * t.ndl;
+            lo[0] += t.B[0] * (L.p1.x * aw);
+            lo[1] += t.B[1] * (L.p1.y * aw);
+            lo[2] += t.B[2] * (L.p1.z * aw);
+        }
+    }, &kept, &ks);
+    float o[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = lo[c] + amb[c];
+
+    float gl[3] = {0.0f, 0.0f, 0.0f};
+    float l2 = 0.0f;
+    if (valid) {
+        hl_store_nt(out_px, make_float4(o[0], o[1], o[2], A.w));
+        const float4 tg = hl_ld4((const float4*)(reinterpret_cast<const uint8_t*>(a.dout) + (size_t)y * a.dout_pitch) + x);
+        const float e0 = o[0] - tg.x, e1 = o[1] - tg.y, e2 = o[2] - tg.z;
+        gl[0] = 2.0f * e0;
+        gl[1] = 2.0f * e1;
+        gl[2] = 2.0f * e2;
+        l2 = e0 * e0 + e1 * e1 + e2 * e2;
+    }
+    const float wl = dpp_sum(l2);
+    if (lane == 0) atomicAdd(a.loss, (double)wl);
+
+    // ---- adjoint: the same lights, from LDS when they were kept
+    float dA[3] = {0.0f, 0.0f, 0.0f}, dR = 0.0f, dM = 0.0f;
+    const uint32_t slot = (b * 4u + wave) % kGradSlots;
+    if (ks.ok) {
+        for (uint32_t i = 0; i < ks.n; ++i) {  // uniform
+            const HotLight L = s_l[i];
+            const uint32_t li = hl_uniform(kept.idx[i]);
+            const uint64_t lanes = kept.lanes[i];
+            adjoint_light(a, L, li, ((lanes >> lane) & 1ull) != 0ull, p, gl, dA, dR, dM, slot);
+        }
+    } else {
+        walk_lights<GRID>(a, s_l, p, valid, cell, [&](const HotLight& L, uint32_t li, bool mine) {
+            adjoint_light(a, L, li, mine, p, gl, dA, dR, dM, slot);
+        });
+    }
+    if (valid) {
+        const hl_gptr<float> dp = hl_g(a.d_pixel);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dp[c * plane + pix] = dA[c] + gl[c] * mixv[c] * p.ao;
+        dp[3 * plane + pix] = dR;
+        dp[4 * plane + pix] = dM;
+    }
+}
+
+// Item b of the fused forward + L2 adjoint: a wave tile shared by the block
+// on views below kFusedSplitBlocks block tiles, else a block tile.
+template <int FA, int FN, int GRID>
+__device__ __forceinline__ void fused_tile(const ShadeArgs& a, uint32_t b, ShadeSmem& sm, AdjSmem& as) {
+    if (fused_split(a.num_blocks)) fused_tile_split<FA, FN, GRID>(a, b, sm, as);
+    else fused_tile_own<FA, FN, GRID>(a, b, sm, as);
 }
 
 // ------------------------------------------------------------------ kernels

@@ -35,6 +35,7 @@ K_CELL_LANES = 16
 K_BATCH_CELL_ALIGN = 256
 K_SHADE_QUEUES = 64
 K_GRAD_SLOTS = 8
+K_FUSED_SPLIT_BLOCKS = 4096
 SIZEOF_COLUMN_GEOM = 80
 SIZEOF_CELL_BOX = 24
 SIZEOF_HOT = 80
@@ -446,9 +447,13 @@ def check_shade_items(views, plan, cus: int = 256) -> None:
                 continue
             tw = al[i]["tw"]
             bw, bh = 4 << tw, 64 >> tw
-            # adjoint views: one item per wave tile (shade_items), four per block tile
-            nb = -(-views[i].width // bw) * -(-views[i].height // bh) * (4 if adj else 1)
-            spans.append((nitems, i, nb, bw, bh))
+            # adjoint views below K_FUSED_SPLIT_BLOCKS block tiles: one item per
+            # wave tile (shade_items), four per block tile; larger ones one per
+            # block tile
+            blocks = -(-views[i].width // bw) * -(-views[i].height // bh)
+            split = adj and blocks < K_FUSED_SPLIT_BLOCKS
+            nb = blocks * (4 if split else 1)
+            spans.append((nitems, i, nb, bw, bh, split))
             nitems += nb
         if not spans:
             continue
@@ -456,13 +461,13 @@ def check_shade_items(views, plan, cus: int = 256) -> None:
         starts = np.array([s[0] for s in spans] + [nitems])
         it = np.arange(nitems, dtype=np.int64)
         lo = np.searchsorted(starts[1:-1], it, side="right")
-        for k, (first, i, nb, bw, bh) in enumerate(spans):
+        for k, (first, i, nb, bw, bh, split) in enumerate(spans):
             m = lo == k
             b = it[m] - first
             assert (b >= 0).all() and (b < nb).all()
             v = views[i]
             btx = -(-v.width // bw)
-            bt = b >> 2 if adj else b
+            bt = b >> 2 if split else b
             bx, by = bt % btx, bt // btx
             x_max = bx.max() * bw + bw - 1
             y_max = by.max() * bh + bh - 1

@@ -97,3 +97,19 @@ def test_floor_div_exact_for_plan_divisors():
     for d in (1, 2, 3, 5, 7, 16, 24, 32, 33, 35, 47, 48, 64, 65, 69, 127, 129):
         for q in E.floor_div_f32(x, d):
             assert np.array_equal(q, x // d), d
+
+
+def test_emulation_constants_match_the_sources():
+    """Every K_* constant of the emulation has the value of its constexpr in
+    helio_amd/csrc (kMarkInline for K_MARK_INLINE, and so on)."""
+    import re
+    from pathlib import Path
+    src = "".join(p.read_text() for p in sorted((Path(__file__).resolve().parents[1] / "helio_amd" / "csrc")
+                                                 .glob("*.[hc]*")))
+    defs = dict(re.findall(r"constexpr uint32_t (k[A-Za-z0-9]+) = (\d+)u?;", src))
+    names = [n for n in dir(E) if re.fullmatch(r"K_[A-Z0-9_]+", n)]
+    assert len(names) >= 9
+    for n in names:
+        k = "k" + "".join(w.capitalize() for w in n[2:].split("_"))
+        assert k in defs, f"{n}: no constexpr {k}"
+        assert int(defs[k]) == getattr(E, n), n

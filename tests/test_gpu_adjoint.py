@@ -1,6 +1,7 @@
 """GPU adjoint vs the FP64 oracle and central finite differences.
 
-Workload: 512x512 fp32 G-buffer, 256 point lights, target rendered from the
+Workload: 512x512 fp32 G-buffer (and 1024x1024 for the FP64 comparison), 256
+point lights, target rendered from the
 ground truth, guess perturbed (albedo N(0,0.1), roughness N(0,0.1) clamped to
 [0.05,1], light position N(0,0.5), intensity x U[0.7,1.3]); per-pixel L2 loss;
 clustered lists (16x9x24, uncapped, so every pixel sees all its lights).
@@ -26,13 +27,24 @@ import oracle_lib as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def prob():
-    sc, target, _ = A.make_problem(512, 512, 256)
+def make_prob(size):
+    sc, target, _ = A.make_problem(size, size, 256)
     px = A.decode(sc)
     dpx64, dpos64, dint64 = A.backward64(sc, px, sc.lights, target)
     _, loss64 = A.forward64(sc, px, sc.lights, target)
     return sc, target, px, dpx64, dpos64, dint64, loss64
+
+
+@pytest.fixture(scope="module")
+def prob():
+    return make_prob(512)
+
+
+# 1024x1024: 4096 block tiles of 32x8 pixels, where the fused launch gives
+# each wave a tile of its own (512x512 shares each wave tile among four waves)
+@pytest.fixture(scope="module")
+def prob1k():
+    return make_prob(1024)
 
 
 def run_hip(ctx, hlib, sc, target, fused=True):
@@ -79,9 +91,10 @@ def within(hip, ref, scale):
     return err <= tol, err / (np.abs(ref) + 1e-6 * scale)
 
 
-def test_adjoint_matches_fp64(ctx, hlib, prob):
-    sc, target, px, dpx64, dpos64, dint64, loss64 = prob
-    print("[adjoint] 512x512, 256 lights", flush=True)
+@pytest.mark.parametrize("which", ["prob", "prob1k"])
+def test_adjoint_matches_fp64(ctx, hlib, request, which):
+    sc, target, px, dpx64, dpos64, dint64, loss64 = request.getfixturevalue(which)
+    print(f"[adjoint] {sc.width}x{sc.height}, 256 lights", flush=True)
     r = run_hip(ctx, hlib, sc, target)
     sky = sc.depth >= 1.0
     # forward part against the FP32 restatement over the same (all-lights) semantics
