@@ -67,6 +67,11 @@ struct hl_plan {
     uint32_t* d_queue = nullptr;
     uint32_t* d_flags = nullptr;
     void* d_markq = nullptr;  // mark_queue_bytes(total_lights)
+    // the adjoint views' gradient records: cleared by one launch ahead of the
+    // shading, summed into the FP32 outputs by one after it
+    GradView* d_grad = nullptr;
+    uint32_t num_grad = 0, grad_max_n = 0;
+    size_t grad_words = 0;
     uint32_t total_lights = 0, total_cells = 0, total_tiles = 0, num_items = 0;
     // shading items grouped by (G-buffer format set, mode): one persistent
     // launch per non-empty group; group = set * 2 + (adjoint ? 1 : 0)
@@ -127,14 +132,7 @@ double block_cost(const hl_plan* p, uint32_t v) {
 hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
     hl_context* c = p->ctx;
     const uint32_t nv = (uint32_t)p->views.size();
-    for (uint32_t v = 0; v < nv; ++v) {
-        const hl_view_desc& d = p->views[v];
-        if (!is_adjoint(d)) continue;
-        HL_HIP(c, launch_zero_f64(s, (double*)p->bufs[v].acc, grad_accum_doubles(p->nlights[v])));
-        HL_HIP(c, launch_zero_f64(s, d.loss_d, 1));
-        HL_HIP(c, launch_fill_u32(s, (uint32_t*)d.d_light_pos_d, 0, 3 * (size_t)p->pool_size));
-        HL_HIP(c, launch_fill_u32(s, (uint32_t*)d.d_light_intensity_d, 0, p->pool_size));
-    }
+    HL_HIP(c, launch_grad_views_zero(s, p->d_grad, p->num_grad, p->grad_words));
     // the cull's first kernel clears the views' overflow words; the shading
     // launches leave their claim counters zero for the next frame
     HL_HIP(c, launch_batch_cull(s, p->pool, p->d_cull, p->d_light_base, p->d_cell_base,
@@ -144,13 +142,7 @@ hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
         HL_HIP(c, launch_shade_batch(s, g / 2, (g & 1) ? kModeFusedL2 : kModeForward, p->d_shade,
                                      p->d_spans + p->grp_first[g], p->grp_spans[g], p->grp_count[g],
                                      p->d_queue + (size_t)g * kShadeQueueWords, p->grp_grid[g]));
-    for (uint32_t v = 0; v < nv; ++v) {
-        const hl_view_desc& d = p->views[v];
-        if (!is_adjoint(d)) continue;
-        HL_HIP(c, launch_light_grads_to_f32(s, (const double*)p->bufs[v].acc, p->nlights[v],
-                                            d.light_indices_d, d.d_light_pos_d,
-                                            d.d_light_intensity_d));
-    }
+    HL_HIP(c, launch_grad_views_to_f32(s, p->d_grad, p->num_grad, p->grad_max_n));
     return HL_OK;
 }
 
@@ -346,6 +338,23 @@ hl_status build_plan(hl_plan* p) {
     if ((s = plan_alloc(p, (void**)&p->d_spans, sizeof(uint2) * spans.size())) != HL_OK) return s;
     HL_HIP(c, hipMemcpy(p->d_spans, spans.data(), sizeof(uint2) * spans.size(), hipMemcpyHostToDevice));
     if ((s = plan_alloc(p, (void**)&p->d_queue, 4 * 6 * (size_t)kShadeQueueWords, true)) != HL_OK) return s;
+    std::vector<GradView> gv;
+    p->grad_words = 0;
+    p->grad_max_n = 0;
+    for (uint32_t v = 0; v < nv; ++v) {
+        const hl_view_desc& d = p->views[v];
+        if (!is_adjoint(d)) continue;
+        const GradView g{(double*)p->bufs[v].acc, d.loss_d, d.d_light_pos_d, d.d_light_intensity_d,
+                         d.light_indices_d, p->nlights[v], p->pool_size};
+        gv.push_back(g);
+        p->grad_words = std::max({p->grad_words, 2 * grad_accum_doubles(g.n), 3 * (size_t)g.pool});
+        p->grad_max_n = std::max(p->grad_max_n, g.n);
+    }
+    p->num_grad = (uint32_t)gv.size();
+    if (!gv.empty()) {
+        if ((s = plan_alloc(p, (void**)&p->d_grad, sizeof(GradView) * gv.size())) != HL_OK) return s;
+        HL_HIP(c, hipMemcpy(p->d_grad, gv.data(), sizeof(GradView) * gv.size(), hipMemcpyHostToDevice));
+    }
     return HL_OK;
 }
 

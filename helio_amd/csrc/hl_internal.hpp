@@ -304,10 +304,9 @@ uint32_t shade_batch_blocks_per_cu(int gbuf_set, int mode);
 // Per-light FP64 gradient accumulators: kGradSlots slot-major copies of 4
 // doubles (dpos.xyz, dI) per light, summed by launch_light_grads_to_f32.
 constexpr uint32_t kGradSlots = 8;
-inline size_t grad_accum_doubles(uint32_t num_lights) {
+__host__ __device__ inline size_t grad_accum_doubles(uint32_t num_lights) {
     return (size_t)kGradSlots * 4 * (num_lights ? num_lights : 1);
 }
-hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n);
 // The Vogel disk tables of the shadowed shading (kVogelFloats floats).
 constexpr size_t kVogelTableFloats = 2 * 256 + (size_t)257 * 256;
 hipError_t launch_vogel_table(hipStream_t s, float* t);
@@ -315,5 +314,20 @@ hipError_t launch_vogel_table(hipStream_t s, float* t);
 hipError_t launch_rc_irradiance(hipStream_t s, int gbuf_set, const ShadeArgs& a);
 hipError_t launch_light_grads_to_f32(hipStream_t s, const double* acc, uint32_t n,
                                      const uint32_t* subset, float* d_pos, float* d_int);
+// The adjoint views of a batched plan (device records built with the plan):
+// one launch clears every view's accumulators (grad_accum_doubles(n)), loss
+// and pool-sized FP32 gradient outputs ahead of the shading, one sums every
+// view's accumulators into its outputs after it.  max_n: the largest n;
+// max_words: the largest 3 * pool or 2 * grad_accum_doubles(n).
+struct GradView {
+    double* acc;
+    double* loss;
+    float* d_pos;
+    float* d_int;
+    const uint32_t* subset;
+    uint32_t n, pool;
+};
+hipError_t launch_grad_views_zero(hipStream_t s, const GradView* views, uint32_t nv, size_t max_words);
+hipError_t launch_grad_views_to_f32(hipStream_t s, const GradView* views, uint32_t nv, uint32_t max_n);
 
 }  // namespace hl

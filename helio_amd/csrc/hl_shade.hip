@@ -993,12 +993,6 @@ hipError_t launch_shade(hipStream_t s, int gbuf_set, int mode, const ShadeArgs& 
 }
 
 // ------------------------------------------------------ light gradient I/O
-__global__ void k_zero_f64(double* __restrict__ p, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = 0.0;
-}
-
 // FP64 accumulators (kGradSlots copies of 4 per light of the launch, slot
 // major) -> their sums as FP32 outputs indexed by the light's pool index
 // (subset[i], or i).
@@ -1017,6 +1011,36 @@ __global__ void k_light_grads_to_f32(const double* __restrict__ acc, uint32_t n,
     d_pos[3 * (size_t)o + 1] = (float)g[1];
     d_pos[3 * (size_t)o + 2] = (float)g[2];
     d_int[o] = (float)g[3];
+}
+
+// Every adjoint view of a plan at once: view blockIdx.y, its words striding
+// over the x blocks.
+__global__ void k_grad_views_zero(const GradView* __restrict__ views) {
+    const GradView v = views[blockIdx.y];
+    const size_t acc_words = 2 * grad_accum_doubles(v.n), pos_words = 3 * (size_t)v.pool;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < acc_words || i < pos_words; i += stride) {
+        if (i < acc_words) reinterpret_cast<uint32_t*>(v.acc)[i] = 0u;
+        if (i < pos_words) v.d_pos[i] = 0.0f;
+        if (i < v.pool) v.d_int[i] = 0.0f;
+        if (i == 0 && v.loss) *v.loss = 0.0;
+    }
+}
+
+__global__ void k_grad_views_to_f32(const GradView* __restrict__ views) {
+    const GradView v = views[blockIdx.y];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.n) return;
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (uint32_t s = 0; s < kGradSlots; ++s)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) g[c] += v.acc[4 * ((size_t)s * v.n + i) + c];
+    const uint32_t o = v.subset ? v.subset[i] : i;
+    v.d_pos[3 * (size_t)o + 0] = (float)g[0];
+    v.d_pos[3 * (size_t)o + 1] = (float)g[1];
+    v.d_pos[3 * (size_t)o + 2] = (float)g[2];
+    v.d_int[o] = (float)g[3];
 }
 
 // The Vogel tables of kVogelFloats floats: block n, thread i.
@@ -1069,18 +1093,24 @@ hipError_t launch_vogel_table(hipStream_t s, float* t) {
     return hipGetLastError();
 }
 
-hipError_t launch_zero_f64(hipStream_t s, double* p, size_t n) {
-    if (n == 0) return hipSuccess;
-    size_t blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    k_zero_f64<<<(unsigned)blocks, 256, 0, s>>>(p, n);
-    return hipGetLastError();
-}
-
 hipError_t launch_light_grads_to_f32(hipStream_t s, const double* acc, uint32_t n,
                                      const uint32_t* subset, float* d_pos, float* d_int) {
     if (n == 0) return hipSuccess;
     k_light_grads_to_f32<<<(n + 255) / 256, 256, 0, s>>>(acc, n, subset, d_pos, d_int);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_views_zero(hipStream_t s, const GradView* views, uint32_t nv, size_t max_words) {
+    if (nv == 0) return hipSuccess;
+    size_t blocks = (max_words + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    k_grad_views_zero<<<dim3((unsigned)blocks, nv), 256, 0, s>>>(views);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_views_to_f32(hipStream_t s, const GradView* views, uint32_t nv, uint32_t max_n) {
+    if (nv == 0 || max_n == 0) return hipSuccess;
+    k_grad_views_to_f32<<<dim3((max_n + 255) / 256, nv), 256, 0, s>>>(views);
     return hipGetLastError();
 }
 
