@@ -76,6 +76,7 @@ struct hl_plan {
     // shading items grouped by (G-buffer format set, mode): one persistent
     // launch per non-empty group; group = set * 2 + (adjoint ? 1 : 0)
     uint32_t grp_first[6] = {}, grp_spans[6] = {}, grp_count[6] = {}, grp_grid[6] = {};
+    uint32_t grp_view[6] = {};  // the view of a group of one view
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -141,7 +142,8 @@ hl_status enqueue_frame(hl_plan* p, hipStream_t s) {
     for (int g = 0; g < 6; ++g)
         HL_HIP(c, launch_shade_batch(s, g / 2, (g & 1) ? kModeFusedL2 : kModeForward, p->d_shade,
                                      p->d_spans + p->grp_first[g], p->grp_spans[g], p->grp_count[g],
-                                     p->d_queue + (size_t)g * kShadeQueueWords, p->grp_grid[g]));
+                                     p->d_queue + (size_t)g * kShadeQueueWords, p->grp_grid[g],
+                                     p->grp_spans[g] == 1u ? &p->hshade[p->grp_view[g]].sa : nullptr));
     HL_HIP(c, launch_grad_views_to_f32(s, p->d_grad, p->num_grad, p->grad_max_n));
     return HL_OK;
 }
@@ -322,6 +324,7 @@ hl_status build_plan(hl_plan* p) {
             const uint32_t v = o.second;
             if (vset[v] * 2 + (is_adjoint(p->views[v]) ? 1 : 0) != g) continue;
             spans.push_back(make_uint2((uint32_t)n, v));
+            p->grp_view[g] = v;
             n += shade_items(is_adjoint(p->views[v]) ? kModeFusedL2 : kModeForward, p->hshade[v].sa.num_blocks);
         }
         if (n >= 0x80000000ull) return fail(c, HL_ECAPACITY, "%llu block tiles in one launch", (unsigned long long)n);

@@ -295,10 +295,12 @@ hipError_t launch_forward_lit(hipStream_t s, const ForwardArgs& a);
 
 // spans: nspans (first item, view) pairs in item order, then (num_items, 0).
 // queue: kShadeQueueWords claim counters, zero before the first launch; each
-// launch leaves them zero (its last block out clears them).
+// launch leaves them zero (its last block out clears them).  one: the
+// arguments of the only view when nspans == 1 (then passed as a kernel
+// argument), else null.
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
                               const uint2* spans, uint32_t nspans, uint32_t num_items, uint32_t* queue,
-                              uint32_t grid_blocks);
+                              uint32_t grid_blocks, const ShadeArgs* one);
 // Blocks of the batched shading kernel of (gbuf_set, mode) resident per CU.
 uint32_t shade_batch_blocks_per_cu(int gbuf_set, int mode);
 // Per-light FP64 gradient accumulators: kGradSlots slot-major copies of 4

@@ -819,10 +819,14 @@ __global__ __launch_bounds__(256, (shade_min_blocks<(MODE & ~kModeOutF16), FA>()
 // written while the launch runs, and their uniform loads become scalar loads.
 typedef const __attribute__((address_space(4))) BatchShadeView* ConstViews;
 
+// A launch of one view (nspans == 1) takes that view's arguments from `one`,
+// a kernel argument, as k_shade does: read through the view table, the
+// arguments the tile uses in its loops were reloaded there (18x the scalar
+// loads of k_shade on the same tiles) rather than kept in scalar registers.
 template <int FA, int FN, int MODE>
 __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_batch(
     ConstViews __restrict__ views, const uint2* __restrict__ spans, uint32_t nspans,
-    uint32_t num_items, uint32_t* __restrict__ queue) {
+    uint32_t num_items, uint32_t* __restrict__ queue, ShadeArgs one) {
     __shared__ ShadeSmem sm;
     __shared__ ClaimSmem cs;
     AdjSmem* as = nullptr;
@@ -831,7 +835,13 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_bat
         as = &adj;
     }
     uint32_t first = 1u, end = 0u, v = 0u;  // item range of the staged view (none yet), its index
-    claim_items(num_items, queue, cs, [&](uint32_t it) {
+    if (nspans == 1u) {
+        load_view_const(one, sm);  // published by the first claim's barrier
+        claim_items(num_items, queue, cs, [&](uint32_t it) {
+            if constexpr (MODE == kModeFusedL2) fused_tile<FA, FN, HL_GRID_CLUSTERS>(one, it, sm, *as);
+            else shade_tile<FA, FN, HL_GRID_CLUSTERS, MODE>(one, it, sm, as);
+        });
+    } else claim_items(num_items, queue, cs, [&](uint32_t it) {
         it = hl_uniform(it);  // block-uniform: every thread read the same claim
         if (it < first || it >= end) {
             uint32_t lo = 0, hi = nspans;  // spans[lo].x <= it < spans[hi].x
@@ -881,13 +891,14 @@ __global__ __launch_bounds__(256, MODE == kModeForward ? 5 : 1) void k_shade_bat
 
 template <int FA, int FN>
 static void launch_batch_fmt(hipStream_t s, int mode, const BatchShadeView* views, const uint2* spans,
-                             uint32_t nspans, uint32_t num_items, uint32_t* queue, uint32_t grid_blocks) {
+                             uint32_t nspans, uint32_t num_items, uint32_t* queue, uint32_t grid_blocks,
+                             const ShadeArgs& one) {
     if (mode == kModeFusedL2)
         k_shade_batch<FA, FN, kModeFusedL2><<<grid_blocks, 256, 0, s>>>((ConstViews)views, spans, nspans, num_items,
-                                                                        queue);
+                                                                        queue, one);
     else
         k_shade_batch<FA, FN, kModeForward><<<grid_blocks, 256, 0, s>>>((ConstViews)views, spans, nspans, num_items,
-                                                                        queue);
+                                                                        queue, one);
 }
 
 template <int FA, int FN>
@@ -909,20 +920,22 @@ uint32_t shade_batch_blocks_per_cu(int gbuf_set, int mode) {
 
 hipError_t launch_shade_batch(hipStream_t s, int gbuf_set, int mode, const BatchShadeView* views,
                               const uint2* spans, uint32_t nspans, uint32_t num_items, uint32_t* queue,
-                              uint32_t grid_blocks) {
+                              uint32_t grid_blocks, const ShadeArgs* one) {
     if (num_items == 0) return hipSuccess;
+    if ((nspans == 1u) != (one != nullptr)) return hipErrorInvalidValue;
+    const ShadeArgs a = one ? *one : ShadeArgs{};
     switch (gbuf_set) {
         case kSetRef:
             launch_batch_fmt<HL_FMT_RGBA8_UNORM, HL_FMT_RGBA16_FLOAT>(s, mode, views, spans, nspans, num_items,
-                                                                    queue, grid_blocks);
+                                                                    queue, grid_blocks, a);
             break;
         case kSet16:
             launch_batch_fmt<HL_FMT_RGBA16_FLOAT, HL_FMT_RGBA16_FLOAT>(s, mode, views, spans, nspans, num_items,
-                                                                     queue, grid_blocks);
+                                                                     queue, grid_blocks, a);
             break;
         default:
             launch_batch_fmt<HL_FMT_RGBA32_FLOAT, HL_FMT_RGBA32_FLOAT>(s, mode, views, spans, nspans, num_items,
-                                                                     queue, grid_blocks);
+                                                                     queue, grid_blocks, a);
             break;
     }
     return hipGetLastError();

@@ -121,6 +121,39 @@ def test_batch_equals_single_view(ctx, hlib, mv):
         plan.destroy()
 
 
+def test_one_view_plans_equal_single_view(ctx, hlib, mv):
+    """Plans of one view (the shading launch takes the view's arguments as a
+    kernel argument): a 512^2 face (forward) bit for bit, an eye (fused
+    forward + L2 adjoint) as in test_batch_equals_single_view; eager and
+    replayed from the captured graph."""
+    pool, pool_size, views, hosts, _ = mv
+    print("[batch] one-view plans vs single-view", flush=True)
+    for i in (40, len(views) - 1):
+        v, (_, sub) = views[i], hosts[i]
+        plan = BatchPlan(ctx, pool, pool_size, [v])
+        try:
+            cfg = plan.config(0)
+            ref = single_view(ctx, hlib, v, cfg)
+            sky = v.scene.host.depth >= 1.0
+            for rep in range(3):
+                v.out.fill_u32(0)
+                if rep == 1:
+                    plan.capture()
+                plan.launch()
+                ctx.synchronize()
+                got = read_view(v, pool_size, sub)
+                if v.adjoint:
+                    assert close(got["out"][~sky], ref["out"][~sky], rel=1e-6), f"view {i} rep {rep}"
+                    assert abs(got["loss"] - ref["loss"]) <= 1e-6 * abs(ref["loss"]), f"view {i} rep {rep}"
+                    assert close(got["dpx"], ref["dpx"]), f"view {i} rep {rep} pixel grads"
+                    assert close(got["dpos"], ref["dpos"]), f"view {i} rep {rep} light pos grads"
+                    assert close(got["dint"], ref["dint"]), f"view {i} rep {rep} intensity grads"
+                else:
+                    assert np.array_equal(got["out"][~sky], ref["out"][~sky]), f"view {i} rep {rep}"
+        finally:
+            plan.destroy()
+
+
 def test_graph_replays_match_eager(ctx, mv):
     pool, pool_size, views, hosts, _ = mv
     print("[batch] graph replays", flush=True)
