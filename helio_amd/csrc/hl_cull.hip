@@ -499,10 +499,11 @@ __device__ __forceinline__ void mark_light(const ClusterArgs& a,
         uint32_t base = 0;
         if ((threadIdx.x & 63u) == 0) base = atomicAdd(q.count + sub * kMarkCountStride, extra);
         base = hl_readlane(base, 0);
-        // room left in the sub-queue, saturating at zero: written as a select
-        // over a wrapping difference (base >= seg ? 0 : seg - base), -O3 folded
-        // the select away once it had marked the difference no-wrap, and an
-        // overflowing light then took fit = extra and wrote past its segment
+        // room left in the sub-queue, saturating at zero.  Written as
+        // fit = base >= seg ? 0 : min(extra, seg - base), -O3 marked the
+        // difference no-wrap and folded the select away, and an overflowing
+        // light then took fit = extra and wrote past its segment
+        // (tests/test_mark_queue_guard.py checks the compiled form)
         const uint32_t room = __builtin_elementwise_sub_sat(q.seg, base);
         const uint32_t fit = min(extra, room);
         if (fit) {
